@@ -1,0 +1,266 @@
+/*
+ * retainwall.h -- C ABI of the MI355X (gfx950) SPH retaining-wall step.
+ *
+ * The library (libretainwall.so) owns every device array of one simulation
+ * context; the caller owns host buffers.  No device pointer crosses this
+ * boundary, no HIP/torch type appears in a signature, and every entry point
+ * returns an int status (0 = LQ_OK) with the message kept in
+ * lq_last_error(ctx).  One context = one GPU = one HIP stream; a context is
+ * not thread-safe.
+ *
+ * Each entry point names the reference routine it replaces (file:line in the
+ * reference's src/ directory).  The reference reaches all of them as private
+ * methods of LoquatSph / LoquatGrid (loquat.h:97-120, loquat_grid.h:46-66); a
+ * host that keeps those class names calls these functions underneath.
+ */
+#ifndef RETAINWALL_H
+#define RETAINWALL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LQ_MODEL_NUMBER_MAX 10  /* types.h:38 */
+#define LQ_MATERIAL_SIZE_MAX 15 /* types.h:39 */
+#define LQ_RIGID_MOTION_MAX 10  /* types.h:40 */
+#define LQ_TABLE_MAX 10         /* types.h:80-81 (table = scalar[10]) */
+
+enum lq_status {
+  LQ_OK = 0,
+  LQ_ERR_ARG = 1,      /* bad argument / size / unsupported option          */
+  LQ_ERR_HIP = 2,      /* a HIP runtime call failed                         */
+  LQ_ERR_DELETE = 3,   /* npb + npm + np_delete_m != np (loquat_grid.cu:182) */
+  LQ_ERR_DOMAIN = 4,   /* a boundary particle left the cell grid            */
+  LQ_ERR_STATE = 5     /* call out of order (e.g. step before upload)       */
+};
+
+/* types.h:47-50 */
+enum lq_model_mat { LQ_RIGID = 0, LQ_DP_EPP = 1, LQ_MUI_DP_EPP = 2, LQ_HYPO_BASIC = 3, LQ_HYPO_CS = 4 };
+enum lq_fix { LQ_FIX = 0, LQ_NONFIX = 1 };           /* types.h:46 */
+enum lq_shape { LQ_CUBOID = 0, LQ_CIRCLE = 1 };      /* types.h:49 */
+
+/* Cell-key function.  LINEAR reproduces the reference hash
+ * key = cx*ny*nz + cy*nz + cz (+ncell_material for soil), loquat_grid.cu:88-90,
+ * bit for bit.  MORTON interleaves the cell coordinates instead (extension). */
+enum lq_key_mode { LQ_KEY_LINEAR = 0, LQ_KEY_MORTON = 1 };
+
+/* Interaction kernel variant.  REFERENCE_ORDER walks the 27 cells per thread
+ * exactly as particle_interaction.cu:117-140; NEIGHBOR_LIST (default) first
+ * compacts each particle's neighbours in LDS (same order) and then evaluates
+ * the pair terms with all lanes busy. */
+enum lq_interaction_variant { LQ_INTERACTION_NEIGHBOR_LIST = 0, LQ_INTERACTION_REFERENCE_ORDER = 1 };
+
+/* RigidMotion, types.h:66-76 (the per-step imposed motion of one body). */
+typedef struct lq_rigid_motion {
+  int32_t is_trans;
+  int32_t is_rotat;
+  float trans_vel[3];
+  float trans_acc[3];
+  int32_t pattern; /* lq_shape */
+  float pos_rp[3];
+  float rotat_vel[3];
+  float rotat_acc[3];
+} lq_rigid_motion;
+
+/* MotionTable, types.h:81-97: one body's piecewise-constant acceleration
+ * tables, read from <case>.motion (loquat_io.cu:276-288). */
+typedef struct lq_motion_table {
+  int32_t is_translation;
+  int32_t table_size_trans;
+  float trans_vel[LQ_TABLE_MAX][3];
+  float trans_time[LQ_TABLE_MAX];
+  float trans_acc[LQ_TABLE_MAX][3];
+  int32_t is_rotation;
+  int32_t pattern;
+  int32_t table_size_rotat;
+  float pos_rp[3];
+  float rotat_vel[3];
+  float rotat_time[LQ_TABLE_MAX];
+  float rotat_acc[LQ_TABLE_MAX][3];
+} lq_motion_table;
+
+/* POD mirror of Parameters (types.h:100-165): same names, units and meaning.
+ * model_par[k][0..14] follows GenCase.cu:52-68 for HypoCS.  Derived fields
+ * (h, i_h, eta, bound_thickness, vol_def, cell_num, ncell*, kernel_*) are
+ * filled by lq_initialize_parameters (loquat.cu:130-190). */
+typedef struct lq_params {
+  int32_t dim;
+  int32_t periodbound;
+  float planeleft[4];
+  float planeright[4];
+  int32_t n_motion;
+  lq_rigid_motion mv[LQ_RIGID_MOTION_MAX];
+  int32_t np, npb, npm, np_delete_m;
+  float domain_size[3];
+  float domain_min[3];
+  int32_t cell_num[3];
+  int32_t ncell, ncell_material, ncell_delete_m;
+  float h, i_h, dr, hdr, vol_def, bound_thickness;
+  float gravity[3];
+  float rho0_m, damping;
+  int32_t n_model;
+  float model_par[LQ_MODEL_NUMBER_MAX][LQ_MATERIAL_SIZE_MAX];
+  float kernel_normalization_par, kernel_gradient_normalization_par;
+  int32_t stress_shepard_filter, dynamic_time_step, integration_method, viscosity_type;
+  float artificial_viscosity_alpha_m, cs_m, eta, cfl;
+  float artificial_pressure_coefficient_material, kernel_zero, kernel_dr;
+  float time_max, time, dt, output_frequency;
+  int32_t output_number, output_format;
+  /* --- extensions with no reference field --- */
+  int32_t key_mode;            /* lq_key_mode */
+  int32_t interaction_variant; /* lq_interaction_variant */
+} lq_params;
+
+/* Host AoS views in the .dat column semantics (loquat_io.cu:229-254).
+ * Positions are absolute on upload.  str/stn are xx yy zz xy xz yz. */
+typedef struct lq_host_state {
+  const int32_t* codes;   /* [np][4] block id, Fix/NonFix, model, motion id */
+  const float* pospres;   /* [np][4] x y z p                                */
+  const float* velrhop;   /* [np][4] vx vy vz rho                           */
+  const float* idmassidx; /* [np][4] id mass idx M_crt (idx is overwritten) */
+  const float* str;       /* [np][6]                                        */
+  const float* stn;       /* [np][6]                                        */
+  const float* normal;    /* [npb][4] nx ny nz slip(-1 no-slip)             */
+} lq_host_state;
+
+/* Field ids for lq_get_field / lq_set_field.  Every field is per sorted slot
+ * with np rows in the host AoS layout noted; positions are cell-relative. */
+enum lq_field {
+  LQ_F_CODES = 0,     /* int32[np][4]                                      */
+  LQ_F_CELLPOS = 1,   /* int32[np][4]  cx cy cz key                        */
+  LQ_F_POS = 2,       /* f32[np][4]    relative x y z, p                   */
+  LQ_F_POS_ORI = 3,   /* f32[np][4]    accumulated displacement            */
+  LQ_F_VEL = 4,       /* f32[np][4]    vx vy vz rho                        */
+  LQ_F_IDM = 5,       /* f32[np][4]    id mass idx M_crt                   */
+  LQ_F_POS_INI = 6,   /* f32[np][4]                                        */
+  LQ_F_VEL_INI = 7,   /* f32[np][4]                                        */
+  LQ_F_ACCAD = 8,     /* f32[np][4]    ax ay az drho/dt                    */
+  LQ_F_SPIN = 9,      /* f32[np][4]    w_xy w_xz w_yz 0 (SkewTensor order) */
+  LQ_F_CVAR = 10,     /* f32[np][4]    K d slip wall-id  (contact_var)     */
+  LQ_F_NORMAL = 11,   /* f32[np][4]    boundary rows meaningful            */
+  LQ_F_STR = 12,      /* f32[np][6]                                        */
+  LQ_F_STN = 13,      /* f32[np][6]                                        */
+  LQ_F_STR_INI = 14,  /* f32[np][6]                                        */
+  LQ_F_STN_INI = 15,  /* f32[np][6]                                        */
+  LQ_F_ACESTN = 16,   /* f32[np][6]    strain rate D                       */
+  LQ_F_CMAG = 17,     /* f32[np][9]    force(xx xy xz) normal(yx yy yz) wall-vel(zx zy zz) */
+  LQ_F_PART_CEL = 18, /* int32[np]     sorted keys                         */
+  LQ_F_PART_IDX = 19, /* int32[np]     gather map new slot -> old slot     */
+  LQ_F_CELL_BEG = 20, /* int32[ncell]                                      */
+  LQ_F_CELL_END = 21, /* int32[ncell]                                      */
+  LQ_F_ORIG_ID = 22,  /* int32[np]     original .dat line of each slot     */
+  LQ_F_COUNT = 23
+};
+
+typedef struct lq_ctx lq_ctx;
+
+/* ---- context ------------------------------------------------------------ */
+
+/* sizeof(lq_params), sizeof(lq_motion_table), sizeof(lq_rigid_motion),
+ * sizeof(lq_host_state) into sizes[0..n) so bindings can check their layout.
+ * No reference counterpart (binding aid). */
+int lq_abi_sizes(size_t* sizes, int n);
+
+/* InitializeParameters, loquat.cu:130-190: fill the derived fields of *par
+ * in place from dim, dr, hdr, domain_size, kernel choice. */
+int lq_initialize_parameters(lq_params* par);
+
+/* LoquatSph ctor + AllocateGpuMemory + GridConfig (loquat.cu:29-69, 193-220;
+ * loquat_grid.cu:59-68).  par must already be initialised. */
+int lq_create(const lq_params* par, int device, lq_ctx** out);
+void lq_destroy(lq_ctx* ctx);
+const char* lq_last_error(const lq_ctx* ctx);
+
+/* CopyDataFromHostToDevice (loquat.cu:223-234), then
+ * PositionFromAbsolute2Relative (loquat_grid.cu:253-271). */
+int lq_upload(lq_ctx* ctx, const lq_host_state* host);
+
+/* LoquatIo::MotionInfoALL (loquat_io.cu:266-303): motion tables per body.
+ * The tables live on the device and advance in lq_rigid_body_update;
+ * lq_get_motion_tables reads their current state back. */
+int lq_set_motion_tables(lq_ctx* ctx, int n, const lq_motion_table* tables);
+int lq_get_motion_tables(lq_ctx* ctx, int n, lq_motion_table* tables);
+
+/* Raw field access for checkpoints, IO and tests (host AoS layouts above). */
+int lq_get_field(lq_ctx* ctx, int field, void* dst, size_t bytes);
+int lq_set_field(lq_ctx* ctx, int field, const void* src, size_t bytes);
+
+/* Current parameters as seen by the device (time, dt, mv, npm, np_delete_m). */
+int lq_get_params(lq_ctx* ctx, lq_params* out);
+/* Host-side time (FP64, loquat.cu:558) and step counter. */
+int lq_get_time(lq_ctx* ctx, double* time, int64_t* step);
+int lq_set_time(lq_ctx* ctx, double time, int64_t step);
+
+/* ---- step pieces (each enqueues work on the context stream) -------------- */
+
+/* StepCalculation, loquat.cu:366-374: dt = cfl*h/cs, par.time = time. */
+int lq_step_calculation(lq_ctx* ctx);
+/* LoquatGrid::NeighborSearch, loquat_grid.cu:147-191: hash, stable sort,
+ * begin/end, deleted-particle bookkeeping (device-side, no host sync). */
+int lq_neighbor_search(lq_ctx* ctx);
+/* LoquatSph::SortArraysPrimary / SortArraysAll, loquat.cu:254-287 (contact
+ * state is carried with its particle as well). */
+int lq_sort_arrays_primary(lq_ctx* ctx);
+int lq_sort_arrays_all(lq_ctx* ctx);
+/* LoquatSph::RigidBodyUpdate, adami_boundary.cu:175-203 (runs on device). */
+int lq_rigid_body_update(lq_ctx* ctx);
+/* LoquatSph::AdamiBoundary, adami_boundary.cu:89-169. */
+int lq_adami_boundary(lq_ctx* ctx);
+/* LoquatSph::FindCornerParticle, adami_boundary.cu:209-306. */
+int lq_find_corner_particle(lq_ctx* ctx);
+/* LoquatSph::StressRegularization, field_smoothing.cu:74-101. */
+int lq_stress_regularization(lq_ctx* ctx);
+/* ParticleInteractionMaterial, particle_interaction.cu:260-305, with the
+ * wall-contact force resolved in two deterministic passes. */
+int lq_particle_interaction(lq_ctx* ctx);
+/* LoquatSph::Predictor, predictor_corrector.cu:52-103 (snapshot + update). */
+int lq_predictor(lq_ctx* ctx);
+/* LoquatSph::Corrector, predictor_corrector.cu:109-214. */
+int lq_corrector(lq_ctx* ctx);
+
+/* One or more whole time steps (loquat.cu:548-561 without Saving): the
+ * predictor and corrector sections in order.  use_graph != 0 captures one
+ * step in a hipGraph and replays it.  No host synchronisation inside. */
+int lq_step(lq_ctx* ctx, int nsteps, int use_graph);
+
+/* Wall earth-pressure probe, utilities.cu:578-670: F = sum 0.001*fx [kN]
+ * over soil slots with x_abs <= box[0], box[1] <= z_abs <= box[2].  The
+ * reference hard-codes box = {0.04+bt+3dr, bt+0.04, bt+1.04}.  Synchronises. */
+int lq_wall_force(lq_ctx* ctx, const float box[3], double* force_kn);
+
+/* Absolute positions (PositionFromRelative2Absolute, loquat_grid.cu:273-289)
+ * into dst f32[np][4].  Synchronises. */
+int lq_positions_absolute(lq_ctx* ctx, float* dst, size_t bytes);
+
+/* Counts after deletions: npb, npm, np_delete_m.  Synchronises. */
+int lq_get_counts(lq_ctx* ctx, int32_t* npb, int32_t* npm, int32_t* np_delete_m);
+
+/* Block until the context stream is idle; reports device-side errors
+ * (deletion bookkeeping, boundary out of grid). */
+int lq_synchronize(lq_ctx* ctx);
+
+/* ---- instrumentation (no reference counterpart) --------------------------
+ * When enabled, lq_step and the piecewise calls record HIP events around
+ * each kernel class on the context stream; lq_kernel_times returns the
+ * accumulated milliseconds and launch counts per class. */
+enum lq_kernel_class {
+  LQ_K_NEIGHBOR_SEARCH = 0, LQ_K_SORT_ARRAYS = 1, LQ_K_ADAMI = 2, LQ_K_CORNER = 3,
+  LQ_K_INTERACTION = 4, LQ_K_CONTACT = 5, LQ_K_PREDICTOR = 6, LQ_K_CORRECTOR = 7,
+  LQ_K_SHEPARD = 8, LQ_K_CLASS_COUNT = 9
+};
+int lq_enable_timing(lq_ctx* ctx, int enable);
+int lq_kernel_times(lq_ctx* ctx, double* ms, int64_t* launches, int n);
+
+/* Neighbour list of every soil slot (debug/parity): for soil slot s the
+ * original ids (.dat line) of all particles j with q_ij < 2, in traversal
+ * order, written to ids[s*max_nb ...], counts[s] = number found (may exceed
+ * max_nb; only the first max_nb are stored).  Synchronises. */
+int lq_neighbor_lists(lq_ctx* ctx, int max_nb, int32_t* counts, int32_t* ids, size_t ids_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RETAINWALL_H */

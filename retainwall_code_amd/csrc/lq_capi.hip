@@ -1,0 +1,839 @@
+// lq_capi.hip -- extern "C" boundary of libretainwall.so (include/retainwall.h).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <algorithm>
+
+#include "lq_internal.hpp"
+#include "lq_launch.hpp"
+
+using namespace lq;
+
+struct TimedPair {
+  int cls;
+  hipEvent_t a, b;
+};
+
+struct lq_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  lq_params par{};
+  DevParams hp{};          // host image of the static part
+  DevParams* dP = nullptr;
+  Buffers B{};
+  // alternate *_ini set for SortArraysAll
+  float4 *pos_ini_alt = nullptr, *vel_ini_alt = nullptr, *str_ini_a_alt = nullptr, *stn_ini_a_alt = nullptr;
+  float2 *str_ini_b_alt = nullptr, *stn_ini_b_alt = nullptr;
+  std::vector<void*> allocs;
+  std::string err;
+  bool uploaded = false;
+  int nbits = 0;
+  int npm0 = 0;
+  long long host_step = 0;
+  // instrumentation
+  bool timing = false;
+  std::vector<TimedPair> pending;
+  std::vector<hipEvent_t> pool;
+  double ms[LQ_K_CLASS_COUNT] = {0};
+  int64_t launches[LQ_K_CLASS_COUNT] = {0};
+  // graph
+  hipGraphExec_t graph = nullptr;
+  long long layout_version = 0, graph_version = -1;
+  // reductions
+  int wf_blocks = 1;
+  double* wf_out = nullptr;
+  double* wf_host = nullptr;
+  int* nl_counts = nullptr;
+};
+
+#define HIPCHECK(ctx, call)                                                                  \
+  do {                                                                                       \
+    hipError_t e_ = (call);                                                                  \
+    if (e_ != hipSuccess) {                                                                  \
+      (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                        \
+      return LQ_ERR_HIP;                                                                     \
+    }                                                                                        \
+  } while (0)
+
+static int fail(lq_ctx* c, int code, const std::string& m) {
+  if (c) c->err = m;
+  return code;
+}
+
+template <class T> static int dalloc(lq_ctx* c, T** p, size_t n) {
+  void* q = nullptr;
+  size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) {
+    c->err = std::string("hipMalloc: ") + hipGetErrorString(e);
+    return LQ_ERR_HIP;
+  }
+  hipMemsetAsync(q, 0, bytes, c->stream);
+  c->allocs.push_back(q);
+  *p = (T*)q;
+  return 0;
+}
+
+// ------------------------------------------------------- timing helpers --
+static hipEvent_t ev_get(lq_ctx* c) {
+  if (!c->pool.empty()) {
+    hipEvent_t e = c->pool.back();
+    c->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  hipEventCreate(&e);
+  return e;
+}
+struct Section {
+  lq_ctx* c;
+  int cls;
+  hipEvent_t a = nullptr;
+  Section(lq_ctx* c_, int cls_) : c(c_), cls(cls_) {
+    if (c->timing) {
+      a = ev_get(c);
+      hipEventRecord(a, c->stream);
+    }
+  }
+  ~Section() {
+    if (c->timing) {
+      hipEvent_t b = ev_get(c);
+      hipEventRecord(b, c->stream);
+      c->pending.push_back({cls, a, b});
+    }
+  }
+};
+static void resolve_timing(lq_ctx* c) {
+  if (c->pending.empty()) return;
+  hipStreamSynchronize(c->stream);
+  for (auto& p : c->pending) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, p.a, p.b);
+    c->ms[p.cls] += ms;
+    c->launches[p.cls] += 1;
+    c->pool.push_back(p.a);
+    c->pool.push_back(p.b);
+  }
+  c->pending.clear();
+}
+
+// --------------------------------------------------------------- params --
+extern "C" int lq_abi_sizes(size_t* sizes, int n) {
+  size_t v[4] = {sizeof(lq_params), sizeof(lq_motion_table), sizeof(lq_rigid_motion), sizeof(lq_host_state)};
+  for (int k = 0; k < n && k < 4; k++) sizes[k] = v[k];
+  return 0;
+}
+
+// InitializeParameters, loquat.cu:130-190.
+extern "C" int lq_initialize_parameters(lq_params* p) {
+  if (!p || p->dr <= 0.f || p->hdr <= 0.f) return LQ_ERR_ARG;
+  p->h = p->dr * p->hdr;
+  p->i_h = 1.0f / p->h;
+  p->eta = 0.01f * p->h * p->h;
+  p->bound_thickness = float(std::ceil(2.0 * double(p->hdr)) * double(p->dr));
+  const double hd = double(p->h), pi = 3.14159265358979323846;
+  switch (p->dim) {
+    case 1:
+      p->vol_def = p->dr;
+      p->cell_num[0] = 1;
+      p->cell_num[1] = 1;
+      p->kernel_normalization_par = float(3.0 / (32 * hd));
+      p->kernel_gradient_normalization_par = float(-15.0 / (32 * std::pow(hd, 3)));
+      break;
+    case 2:
+      p->vol_def = float(std::pow(double(p->dr), 2));
+      p->cell_num[1] = 1;
+      p->cell_num[0] = int(0.5f * p->i_h * p->domain_size[0]) + 1;
+      p->kernel_normalization_par = float(7.0 / (32.0 * pi * std::pow(hd, 2)));
+      p->kernel_gradient_normalization_par = float(-35.0 / (32 * pi * std::pow(hd, 4)));
+      break;
+    case 3:
+      p->vol_def = float(std::pow(double(p->dr), 3));
+      p->cell_num[0] = int(0.5f * p->i_h * p->domain_size[0]) + 1;
+      p->cell_num[1] = int(0.5f * p->i_h * p->domain_size[1]) + 1;
+      p->kernel_normalization_par = float(42.0 / (256.0 * pi * std::pow(hd, 3)));
+      p->kernel_gradient_normalization_par = float(-5.0 * 42 / (256.0 * pi * std::pow(hd, 5)));
+      break;
+    default:
+      return LQ_ERR_ARG;
+  }
+  p->cell_num[2] = int(0.5f * p->i_h * p->domain_size[2]) + 1;
+  const int layer = p->cell_num[0] * p->cell_num[1] * p->cell_num[2];
+  p->ncell_material = layer;
+  p->ncell_delete_m = 2 * layer;
+  p->ncell = p->ncell_delete_m + 1;
+  p->np_delete_m = 0;
+  const float q = p->dr * p->i_h;
+  p->kernel_zero = p->kernel_normalization_par * 2.0f * 2.0f * 2.0f * 2.0f * 0.5f;
+  p->kernel_dr = p->kernel_normalization_par * (2.0f - q) * (2.0f - q) * (2.0f - q) * (2.0f - q) * (q + 0.5f);
+  return LQ_OK;
+}
+
+static void fill_static(const lq_params& p, DevParams& d) {
+  std::memset(&d, 0, sizeof(d));
+  d.np = p.np; d.npb = p.npb;
+  d.cnx = p.cell_num[0]; d.cny = p.cell_num[1]; d.cnz = p.cell_num[2];
+  d.ncell = p.ncell; d.ncell_material = p.ncell_material; d.ncell_delete_m = p.ncell_delete_m;
+  d.key_mode = p.key_mode; d.shepard = p.stress_shepard_filter; d.viscosity_type = p.viscosity_type;
+  d.n_motion = p.n_motion; d.n_model = p.n_model;
+  d.h = p.h; d.i_h = p.i_h; d.dr = p.dr; d.vol_def = p.vol_def; d.bt = p.bound_thickness; d.eta = p.eta;
+  d.two_h = 2.0f * p.h;
+  d.gx = p.gravity[0]; d.gy = p.gravity[1]; d.gz = p.gravity[2];
+  d.damping = p.damping;
+  d.knorm = p.kernel_normalization_par; d.kgrad = p.kernel_gradient_normalization_par;
+  d.kernel_zero = p.kernel_zero; d.kernel_dr = p.kernel_dr;
+  d.art_exp = p.kernel_zero / p.kernel_dr;
+  d.alpha_cs = p.artificial_viscosity_alpha_m * p.cs_m;
+  d.art_p = p.artificial_pressure_coefficient_material;
+  d.cfl = p.cfl; d.cs = p.cs_m;
+  d.band = p.dr * 1.0001f;      // particle_interaction.cu:147
+  d.half_dr = p.dr / 2;         // particle_interaction.cu:157
+  d.mu = 0.58108f;              // particle_interaction.cu:188-189
+  for (int k = 0; k < LQ_MODEL_NUMBER_MAX; k++) {
+    for (int j = 0; j < LQ_MATERIAL_SIZE_MAX; j++) d.model_par[k][j] = p.model_par[k][j];
+    // constitutive_model.h:226-230, hoisted per material (float arithmetic)
+    const float* m = p.model_par[k];
+    const float Ei = m[0], Vi = m[1], phic = m[2], pref = m[4];
+    HypoConst& h = d.hypo[k];
+    if (pref != 0.f && phic != 0.f && Vi != 1.f) {
+      h.C1 = -Ei / (3.0f * pref * (1.0f + Vi));
+      const float a = 3.0f / (sqrtf(2.0f) * sinf(phic));
+      h.C2 = h.C1 * ((3.0f * Vi + a * sqrtf(1.0f + Vi * Vi)) / (1.0f - Vi) - a * a / 3.0f);
+      h.C3 = a * a * h.C1;
+      h.C4 = a * h.C1;
+      h.fa = h.C2 / h.C1;
+      h.fb = h.C3 / h.C1;
+      h.fc = h.C4 / h.C1;
+      h.sq2sin = a;
+    }
+  }
+  d.npm = p.npm; d.np_delete_m = p.np_delete_m;
+  d.time = p.time; d.dt = p.dt; d.time_d = p.time; d.step = 0;
+  for (int k = 0; k < LQ_RIGID_MOTION_MAX; k++) {
+    const lq_rigid_motion& m = p.mv[k];
+    DevMotion& o = d.mv[k];
+    o.is_trans = m.is_trans; o.is_rotat = m.is_rotat; o.pattern = m.pattern;
+    o.trans_vel = make_float3(m.trans_vel[0], m.trans_vel[1], m.trans_vel[2]);
+    o.trans_acc = make_float3(m.trans_acc[0], m.trans_acc[1], m.trans_acc[2]);
+    o.pos_rp = make_float3(m.pos_rp[0], m.pos_rp[1], m.pos_rp[2]);
+    o.rotat_vel = make_float3(m.rotat_vel[0], m.rotat_vel[1], m.rotat_vel[2]);
+    o.rotat_acc = make_float3(m.rotat_acc[0], m.rotat_acc[1], m.rotat_acc[2]);
+  }
+  int maxkey = p.ncell_delete_m, nb = 0;
+  while ((1LL << nb) <= maxkey) nb++;
+  d.nbits = nb;
+}
+
+static int alloc_arrays(lq_ctx* c, Arrays& A, size_t n) {
+  int st = 0;
+  st |= dalloc(c, &A.codes, n); st |= dalloc(c, &A.cellpos, n); st |= dalloc(c, &A.orig_id, n);
+  st |= dalloc(c, &A.pos, n); st |= dalloc(c, &A.pos_ori, n); st |= dalloc(c, &A.vel, n);
+  st |= dalloc(c, &A.idm, n); st |= dalloc(c, &A.normal, n);
+  st |= dalloc(c, &A.str_a, n); st |= dalloc(c, &A.stn_a, n); st |= dalloc(c, &A.str_b, n); st |= dalloc(c, &A.stn_b, n);
+  st |= dalloc(c, &A.cvar, n); st |= dalloc(c, &A.cf, n); st |= dalloc(c, &A.cn, n); st |= dalloc(c, &A.cw, n);
+  return st;
+}
+
+extern "C" int lq_create(const lq_params* par, int device, lq_ctx** out) {
+  if (!par || !out) return LQ_ERR_ARG;
+  *out = nullptr;
+  if (par->np <= 0 || par->npb < 0 || par->npm < 0 || par->np != par->npb + par->npm)
+    return LQ_ERR_ARG;  // loquat_io.cu:148-151
+  if (par->dim != 3 || par->periodbound) return LQ_ERR_ARG;
+  if (par->key_mode != LQ_KEY_LINEAR) return LQ_ERR_ARG;
+  if (par->n_motion < 0 || par->n_motion > LQ_RIGID_MOTION_MAX) return LQ_ERR_ARG;
+  if (par->np >= (1 << 26)) return LQ_ERR_ARG;  // neighbour-list slot encoding
+  lq_ctx* c = new lq_ctx();
+  c->device = device;
+  c->par = *par;
+  auto bail = [&](int st) { *out = c; return st; };
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) { c->err = std::string("hipSetDevice: ") + hipGetErrorString(e); return bail(LQ_ERR_HIP); }
+  e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { c->err = std::string("hipStreamCreate: ") + hipGetErrorString(e); return bail(LQ_ERR_HIP); }
+  fill_static(*par, c->hp);
+  c->nbits = c->hp.nbits;
+  c->npm0 = par->npm;
+  const size_t n = size_t(par->np);
+  int st = 0;
+  st |= dalloc(c, &c->dP, 1);
+  st |= alloc_arrays(c, c->B.cur, n);
+  st |= alloc_arrays(c, c->B.alt, n);
+  Buffers& B = c->B;
+  st |= dalloc(c, &B.pos_ini, n); st |= dalloc(c, &B.vel_ini, n);
+  st |= dalloc(c, &B.str_ini_a, n); st |= dalloc(c, &B.stn_ini_a, n);
+  st |= dalloc(c, &B.str_ini_b, n); st |= dalloc(c, &B.stn_ini_b, n);
+  st |= dalloc(c, &c->pos_ini_alt, n); st |= dalloc(c, &c->vel_ini_alt, n);
+  st |= dalloc(c, &c->str_ini_a_alt, n); st |= dalloc(c, &c->stn_ini_a_alt, n);
+  st |= dalloc(c, &c->str_ini_b_alt, n); st |= dalloc(c, &c->stn_ini_b_alt, n);
+  st |= dalloc(c, &B.accad, n); st |= dalloc(c, &B.spin, n);
+  st |= dalloc(c, &B.acestn_a, n); st |= dalloc(c, &B.acestn_b, n);
+  st |= dalloc(c, &B.s3, n); st |= dalloc(c, &B.fdef, n); st |= dalloc(c, &B.deferred, n);
+  st |= dalloc(c, &B.str_tmp_a, n); st |= dalloc(c, &B.str_tmp_b, n);
+  st |= dalloc(c, &B.keys, n); st |= dalloc(c, &B.vals, n);
+  st |= dalloc(c, &B.keys_alt, n); st |= dalloc(c, &B.vals_alt, n);
+  st |= dalloc(c, &B.cell_beg, size_t(par->ncell)); st |= dalloc(c, &B.cell_end, size_t(par->ncell));
+  st |= dalloc(c, &B.hist, size_t(256) * radix_tiles(par->np));
+  c->wf_blocks = std::max(1, std::min(1024, (par->np + 255) / 256));
+  st |= dalloc(c, &B.partial, size_t(c->wf_blocks));
+  st |= dalloc(c, &c->wf_out, 1);
+  if (st) return bail(LQ_ERR_HIP);
+  B.part_cel = B.keys;
+  B.part_idx = B.vals;
+  if (hipHostMalloc((void**)&c->wf_host, sizeof(double)) != hipSuccess) return bail(fail(c, LQ_ERR_HIP, "hipHostMalloc"));
+  HIPCHECK(c, hipMemcpyAsync(c->dP, &c->hp, sizeof(DevParams), hipMemcpyHostToDevice, c->stream));
+  launch_fill_int(c->stream, B.cell_beg, par->ncell, -1);
+  launch_fill_int(c->stream, B.cell_end, par->ncell, -1);
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  *out = c;
+  return LQ_OK;
+}
+
+extern "C" void lq_destroy(lq_ctx* c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->graph) hipGraphExecDestroy(c->graph);
+  for (auto& p : c->pending) { c->pool.push_back(p.a); c->pool.push_back(p.b); }
+  for (auto e : c->pool) hipEventDestroy(e);
+  for (void* p : c->allocs) hipFree(p);
+  if (c->wf_host) hipHostFree(c->wf_host);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+extern "C" const char* lq_last_error(const lq_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+// ----------------------------------------------------------- host <-> dev --
+template <class T> static int h2d(lq_ctx* c, T* dst, const std::vector<T>& v) {
+  HIPCHECK(c, hipMemcpyAsync(dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  return 0;
+}
+template <class T> static int d2h(lq_ctx* c, std::vector<T>& v, const T* src, size_t n) {
+  v.resize(n);
+  HIPCHECK(c, hipMemcpyAsync(v.data(), src, n * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+  return 0;
+}
+
+// CopyDataFromHostToDevice (loquat.cu:223-234) + PositionFromAbsolute2Relative.
+extern "C" int lq_upload(lq_ctx* c, const lq_host_state* h) {
+  if (!c || !h || !h->codes || !h->pospres || !h->velrhop || !h->idmassidx || !h->str || !h->stn)
+    return fail(c, LQ_ERR_ARG, "lq_upload: null buffer");
+  if (c->par.npb > 0 && !h->normal) return fail(c, LQ_ERR_ARG, "lq_upload: normal required when npb > 0");
+  hipSetDevice(c->device);
+  const int np = c->par.np, npb = c->par.npb;
+  std::vector<int4> codes(np);
+  std::vector<int> oid(np);
+  std::vector<float4> pos(np), vel(np), idm(np), nrm(np), sa(np), na(np);
+  std::vector<float2> sb(np), nb(np);
+  for (int i = 0; i < np; i++) {
+    codes[i] = make_int4(h->codes[4 * i], h->codes[4 * i + 1], h->codes[4 * i + 2], h->codes[4 * i + 3]);
+    pos[i] = make_float4(h->pospres[4 * i], h->pospres[4 * i + 1], h->pospres[4 * i + 2], h->pospres[4 * i + 3]);
+    vel[i] = make_float4(h->velrhop[4 * i], h->velrhop[4 * i + 1], h->velrhop[4 * i + 2], h->velrhop[4 * i + 3]);
+    // idmassidx.z = line index (loquat_io.cu:240,253)
+    idm[i] = make_float4(h->idmassidx[4 * i], h->idmassidx[4 * i + 1], float(i), h->idmassidx[4 * i + 3]);
+    const float* s = h->str + 6 * i;
+    const float* t = h->stn + 6 * i;
+    sa[i] = make_float4(s[0], s[1], s[2], s[3]);
+    sb[i] = make_float2(s[4], s[5]);
+    na[i] = make_float4(t[0], t[1], t[2], t[3]);
+    nb[i] = make_float2(t[4], t[5]);
+    nrm[i] = i < npb ? make_float4(h->normal[4 * i], h->normal[4 * i + 1], h->normal[4 * i + 2], h->normal[4 * i + 3])
+                     : make_float4(0.f, 0.f, 0.f, 0.f);
+    oid[i] = i;
+  }
+  Arrays& A = c->B.cur;
+  int st = 0;
+  st |= h2d(c, A.codes, codes); st |= h2d(c, A.pos, pos); st |= h2d(c, A.pos_ori, pos);
+  st |= h2d(c, A.vel, vel); st |= h2d(c, A.idm, idm); st |= h2d(c, A.normal, nrm);
+  st |= h2d(c, A.str_a, sa); st |= h2d(c, A.str_b, sb); st |= h2d(c, A.stn_a, na); st |= h2d(c, A.stn_b, nb);
+  st |= h2d(c, A.orig_id, oid);
+  if (st) return st;
+  const float scale = 0.5f * c->par.i_h;
+  launch_abs2rel(c->stream, c->dP, np, A, scale);
+  HIPCHECK(c, hipGetLastError());
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  c->uploaded = true;
+  c->layout_version++;
+  return LQ_OK;
+}
+
+static void tab_to_dev(const lq_motion_table& a, DevTable& m) {
+  std::memset(&m, 0, sizeof(m));
+  m.is_translation = a.is_translation; m.table_size_trans = a.table_size_trans;
+  m.is_rotation = a.is_rotation; m.pattern = a.pattern; m.table_size_rotat = a.table_size_rotat;
+  for (int j = 0; j < LQ_TABLE_MAX; j++) {
+    m.trans_vel[j] = make_float3(a.trans_vel[j][0], a.trans_vel[j][1], a.trans_vel[j][2]);
+    m.trans_acc[j] = make_float3(a.trans_acc[j][0], a.trans_acc[j][1], a.trans_acc[j][2]);
+    m.trans_time[j] = a.trans_time[j];
+    m.rotat_acc[j] = make_float3(a.rotat_acc[j][0], a.rotat_acc[j][1], a.rotat_acc[j][2]);
+    m.rotat_time[j] = a.rotat_time[j];
+  }
+  m.pos_rp = make_float3(a.pos_rp[0], a.pos_rp[1], a.pos_rp[2]);
+  m.rotat_vel = make_float3(a.rotat_vel[0], a.rotat_vel[1], a.rotat_vel[2]);
+}
+static void tab_from_dev(const DevTable& m, lq_motion_table& a) {
+  std::memset(&a, 0, sizeof(a));
+  a.is_translation = m.is_translation; a.table_size_trans = m.table_size_trans;
+  a.is_rotation = m.is_rotation; a.pattern = m.pattern; a.table_size_rotat = m.table_size_rotat;
+  for (int j = 0; j < LQ_TABLE_MAX; j++) {
+    a.trans_vel[j][0] = m.trans_vel[j].x; a.trans_vel[j][1] = m.trans_vel[j].y; a.trans_vel[j][2] = m.trans_vel[j].z;
+    a.trans_acc[j][0] = m.trans_acc[j].x; a.trans_acc[j][1] = m.trans_acc[j].y; a.trans_acc[j][2] = m.trans_acc[j].z;
+    a.trans_time[j] = m.trans_time[j];
+    a.rotat_acc[j][0] = m.rotat_acc[j].x; a.rotat_acc[j][1] = m.rotat_acc[j].y; a.rotat_acc[j][2] = m.rotat_acc[j].z;
+    a.rotat_time[j] = m.rotat_time[j];
+  }
+  a.pos_rp[0] = m.pos_rp.x; a.pos_rp[1] = m.pos_rp.y; a.pos_rp[2] = m.pos_rp.z;
+  a.rotat_vel[0] = m.rotat_vel.x; a.rotat_vel[1] = m.rotat_vel.y; a.rotat_vel[2] = m.rotat_vel.z;
+}
+
+static int pull_params(lq_ctx* c, DevParams& d) {
+  HIPCHECK(c, hipMemcpyAsync(&d, c->dP, sizeof(DevParams), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int lq_set_motion_tables(lq_ctx* c, int n, const lq_motion_table* t) {
+  if (!c || n < 0 || n > LQ_RIGID_MOTION_MAX || (n > 0 && !t)) return fail(c, LQ_ERR_ARG, "lq_set_motion_tables");
+  hipSetDevice(c->device);
+  DevTable tabs[LQ_RIGID_MOTION_MAX];
+  for (int k = 0; k < n; k++) tab_to_dev(t[k], tabs[k]);
+  size_t off = offsetof(DevParams, tab);
+  HIPCHECK(c, hipMemcpyAsync((char*)c->dP + off, tabs, sizeof(DevTable) * n, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return LQ_OK;
+}
+
+extern "C" int lq_get_motion_tables(lq_ctx* c, int n, lq_motion_table* t) {
+  if (!c || n < 0 || n > LQ_RIGID_MOTION_MAX || (n > 0 && !t)) return fail(c, LQ_ERR_ARG, "lq_get_motion_tables");
+  hipSetDevice(c->device);
+  DevParams d;
+  int st = pull_params(c, d);
+  if (st) return st;
+  for (int k = 0; k < n; k++) tab_from_dev(d.tab[k], t[k]);
+  return LQ_OK;
+}
+
+extern "C" int lq_get_params(lq_ctx* c, lq_params* out) {
+  if (!c || !out) return fail(c, LQ_ERR_ARG, "lq_get_params");
+  hipSetDevice(c->device);
+  DevParams d;
+  int st = pull_params(c, d);
+  if (st) return st;
+  *out = c->par;
+  out->npm = d.npm; out->np_delete_m = d.np_delete_m;
+  out->time = d.time; out->dt = d.dt;
+  for (int k = 0; k < LQ_RIGID_MOTION_MAX; k++) {
+    const DevMotion& m = d.mv[k];
+    lq_rigid_motion& o = out->mv[k];
+    o.trans_vel[0] = m.trans_vel.x; o.trans_vel[1] = m.trans_vel.y; o.trans_vel[2] = m.trans_vel.z;
+    o.trans_acc[0] = m.trans_acc.x; o.trans_acc[1] = m.trans_acc.y; o.trans_acc[2] = m.trans_acc.z;
+    o.pos_rp[0] = m.pos_rp.x; o.pos_rp[1] = m.pos_rp.y; o.pos_rp[2] = m.pos_rp.z;
+    o.rotat_vel[0] = m.rotat_vel.x; o.rotat_vel[1] = m.rotat_vel.y; o.rotat_vel[2] = m.rotat_vel.z;
+    o.rotat_acc[0] = m.rotat_acc.x; o.rotat_acc[1] = m.rotat_acc.y; o.rotat_acc[2] = m.rotat_acc.z;
+  }
+  return LQ_OK;
+}
+
+extern "C" int lq_get_time(lq_ctx* c, double* time, int64_t* step) {
+  if (!c) return LQ_ERR_ARG;
+  hipSetDevice(c->device);
+  DevParams d;
+  int st = pull_params(c, d);
+  if (st) return st;
+  if (time) *time = d.time_d;
+  if (step) *step = d.step;
+  return LQ_OK;
+}
+
+extern "C" int lq_set_time(lq_ctx* c, double time, int64_t step) {
+  if (!c) return LQ_ERR_ARG;
+  hipSetDevice(c->device);
+  HIPCHECK(c, hipMemcpyAsync((char*)c->dP + offsetof(DevParams, time_d), &time, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  long long s = step;
+  HIPCHECK(c, hipMemcpyAsync((char*)c->dP + offsetof(DevParams, step), &s, sizeof(long long), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  c->host_step = step;
+  return LQ_OK;
+}
+
+// Field access: host AoS <-> device SoA.
+struct FieldView {
+  int width;      // host values per row
+  bool is_int;
+  size_t rows;
+};
+
+static FieldView field_view(const lq_ctx* c, int f) {
+  const size_t np = c->par.np, nc = c->par.ncell;
+  switch (f) {
+    case LQ_F_CODES: case LQ_F_CELLPOS: return {4, true, np};
+    case LQ_F_POS: case LQ_F_POS_ORI: case LQ_F_VEL: case LQ_F_IDM: case LQ_F_POS_INI: case LQ_F_VEL_INI:
+    case LQ_F_ACCAD: case LQ_F_SPIN: case LQ_F_CVAR: case LQ_F_NORMAL: return {4, false, np};
+    case LQ_F_STR: case LQ_F_STN: case LQ_F_STR_INI: case LQ_F_STN_INI: case LQ_F_ACESTN: return {6, false, np};
+    case LQ_F_CMAG: return {9, false, np};
+    case LQ_F_PART_CEL: case LQ_F_PART_IDX: case LQ_F_ORIG_ID: return {1, true, np};
+    case LQ_F_CELL_BEG: case LQ_F_CELL_END: return {1, true, nc};
+    default: return {0, false, 0};
+  }
+}
+
+static float4* f4_ptr(lq_ctx* c, int f) {
+  Arrays& A = c->B.cur;
+  Buffers& B = c->B;
+  switch (f) {
+    case LQ_F_POS: return A.pos; case LQ_F_POS_ORI: return A.pos_ori; case LQ_F_VEL: return A.vel;
+    case LQ_F_IDM: return A.idm; case LQ_F_POS_INI: return B.pos_ini; case LQ_F_VEL_INI: return B.vel_ini;
+    case LQ_F_ACCAD: return B.accad; case LQ_F_SPIN: return B.spin; case LQ_F_CVAR: return A.cvar;
+    case LQ_F_NORMAL: return A.normal;
+    default: return nullptr;
+  }
+}
+static void sym_ptrs(lq_ctx* c, int f, float4** a, float2** b) {
+  Arrays& A = c->B.cur;
+  Buffers& B = c->B;
+  switch (f) {
+    case LQ_F_STR: *a = A.str_a; *b = A.str_b; break;
+    case LQ_F_STN: *a = A.stn_a; *b = A.stn_b; break;
+    case LQ_F_STR_INI: *a = B.str_ini_a; *b = B.str_ini_b; break;
+    case LQ_F_STN_INI: *a = B.stn_ini_a; *b = B.stn_ini_b; break;
+    case LQ_F_ACESTN: *a = B.acestn_a; *b = B.acestn_b; break;
+    default: *a = nullptr; *b = nullptr;
+  }
+}
+static int* i1_ptr(lq_ctx* c, int f) {
+  switch (f) {
+    case LQ_F_PART_CEL: return c->B.part_cel; case LQ_F_PART_IDX: return c->B.part_idx;
+    case LQ_F_ORIG_ID: return c->B.cur.orig_id; case LQ_F_CELL_BEG: return c->B.cell_beg;
+    case LQ_F_CELL_END: return c->B.cell_end;
+    default: return nullptr;
+  }
+}
+
+extern "C" int lq_get_field(lq_ctx* c, int f, void* dst, size_t bytes) {
+  if (!c || !dst) return fail(c, LQ_ERR_ARG, "lq_get_field: null");
+  FieldView v = field_view(c, f);
+  if (v.width == 0) return fail(c, LQ_ERR_ARG, "lq_get_field: unknown field");
+  if (bytes != v.rows * v.width * 4) return fail(c, LQ_ERR_ARG, "lq_get_field: size mismatch");
+  hipSetDevice(c->device);
+  const size_t n = v.rows;
+  int st = 0;
+  if (f == LQ_F_CODES || f == LQ_F_CELLPOS) {
+    std::vector<int4> t;
+    st = d2h(c, t, f == LQ_F_CODES ? c->B.cur.codes : c->B.cur.cellpos, n);
+    if (st) return st;
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    std::memcpy(dst, t.data(), n * 16);
+  } else if (v.width == 1) {
+    HIPCHECK(c, hipMemcpyAsync(dst, i1_ptr(c, f), n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+  } else if (v.width == 4) {
+    HIPCHECK(c, hipMemcpyAsync(dst, f4_ptr(c, f), n * 16, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+  } else if (v.width == 6) {
+    float4* a; float2* b;
+    sym_ptrs(c, f, &a, &b);
+    std::vector<float4> ta; std::vector<float2> tb;
+    if ((st = d2h(c, ta, a, n)) || (st = d2h(c, tb, b, n))) return st;
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    float* o = (float*)dst;
+    for (size_t i = 0; i < n; i++) {
+      o[6 * i] = ta[i].x; o[6 * i + 1] = ta[i].y; o[6 * i + 2] = ta[i].z; o[6 * i + 3] = ta[i].w;
+      o[6 * i + 4] = tb[i].x; o[6 * i + 5] = tb[i].y;
+    }
+  } else {  // CMAG
+    std::vector<float4> f0, f1, f2;
+    if ((st = d2h(c, f0, c->B.cur.cf, n)) || (st = d2h(c, f1, c->B.cur.cn, n)) || (st = d2h(c, f2, c->B.cur.cw, n)))
+      return st;
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    float* o = (float*)dst;
+    for (size_t i = 0; i < n; i++) {
+      o[9 * i] = f0[i].x; o[9 * i + 1] = f0[i].y; o[9 * i + 2] = f0[i].z;
+      o[9 * i + 3] = f1[i].x; o[9 * i + 4] = f1[i].y; o[9 * i + 5] = f1[i].z;
+      o[9 * i + 6] = f2[i].x; o[9 * i + 7] = f2[i].y; o[9 * i + 8] = f2[i].z;
+    }
+  }
+  return LQ_OK;
+}
+
+extern "C" int lq_set_field(lq_ctx* c, int f, const void* src, size_t bytes) {
+  if (!c || !src) return fail(c, LQ_ERR_ARG, "lq_set_field: null");
+  FieldView v = field_view(c, f);
+  if (v.width == 0) return fail(c, LQ_ERR_ARG, "lq_set_field: unknown field");
+  if (bytes != v.rows * v.width * 4) return fail(c, LQ_ERR_ARG, "lq_set_field: size mismatch");
+  hipSetDevice(c->device);
+  const size_t n = v.rows;
+  c->layout_version++;
+  if (f == LQ_F_CODES || f == LQ_F_CELLPOS) {
+    HIPCHECK(c, hipMemcpyAsync(f == LQ_F_CODES ? c->B.cur.codes : c->B.cur.cellpos, src, n * 16, hipMemcpyHostToDevice, c->stream));
+  } else if (v.width == 1) {
+    HIPCHECK(c, hipMemcpyAsync(i1_ptr(c, f), src, n * 4, hipMemcpyHostToDevice, c->stream));
+  } else if (v.width == 4) {
+    HIPCHECK(c, hipMemcpyAsync(f4_ptr(c, f), src, n * 16, hipMemcpyHostToDevice, c->stream));
+  } else if (v.width == 6) {
+    float4* a; float2* b;
+    sym_ptrs(c, f, &a, &b);
+    const float* s = (const float*)src;
+    std::vector<float4> ta(n); std::vector<float2> tb(n);
+    for (size_t i = 0; i < n; i++) {
+      ta[i] = make_float4(s[6 * i], s[6 * i + 1], s[6 * i + 2], s[6 * i + 3]);
+      tb[i] = make_float2(s[6 * i + 4], s[6 * i + 5]);
+    }
+    HIPCHECK(c, hipMemcpyAsync(a, ta.data(), n * 16, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(b, tb.data(), n * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+  } else {
+    const float* s = (const float*)src;
+    std::vector<float4> f0(n), f1(n), f2(n);
+    for (size_t i = 0; i < n; i++) {
+      f0[i] = make_float4(s[9 * i], s[9 * i + 1], s[9 * i + 2], 0.f);
+      f1[i] = make_float4(s[9 * i + 3], s[9 * i + 4], s[9 * i + 5], 0.f);
+      f2[i] = make_float4(s[9 * i + 6], s[9 * i + 7], s[9 * i + 8], 0.f);
+    }
+    HIPCHECK(c, hipMemcpyAsync(c->B.cur.cf, f0.data(), n * 16, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(c->B.cur.cn, f1.data(), n * 16, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(c->B.cur.cw, f2.data(), n * 16, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+  }
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return LQ_OK;
+}
+
+// ------------------------------------------------------------ step pieces --
+#define NEED_UPLOAD(c) \
+  do { if (!(c)) return LQ_ERR_ARG; if (!(c)->uploaded) return fail((c), LQ_ERR_STATE, "call lq_upload first"); } while (0)
+
+static int enq_step_calc(lq_ctx* c) { launch_step_calc(c->stream, c->dP); return 0; }
+static int enq_ns(lq_ctx* c) {
+  Section s(c, LQ_K_NEIGHBOR_SEARCH);
+  const int* k; const int* v;
+  launch_neighbor_search(c->stream, c->dP, c->par.np, c->nbits, c->par.ncell, c->B.cur, c->B, &k, &v);
+  return 0;
+}
+static int enq_sort_primary(lq_ctx* c) {
+  Section s(c, LQ_K_SORT_ARRAYS);
+  launch_gather(c->stream, c->dP, c->par.np, c->B.cur, c->B.alt, c->B.part_idx, c->B);
+  std::swap(c->B.cur, c->B.alt);
+  c->layout_version++;
+  return 0;
+}
+static int enq_sort_all(lq_ctx* c) {
+  Section s(c, LQ_K_SORT_ARRAYS);
+  Buffers& B = c->B;
+  launch_gather(c->stream, c->dP, c->par.np, B.cur, B.alt, B.part_idx, B);
+  launch_gather_ini(c->stream, c->dP, c->par.np, B.part_idx, B, c->pos_ini_alt, c->vel_ini_alt, c->str_ini_a_alt,
+                    c->str_ini_b_alt, c->stn_ini_a_alt, c->stn_ini_b_alt);
+  std::swap(B.cur, B.alt);
+  std::swap(B.pos_ini, c->pos_ini_alt); std::swap(B.vel_ini, c->vel_ini_alt);
+  std::swap(B.str_ini_a, c->str_ini_a_alt); std::swap(B.str_ini_b, c->str_ini_b_alt);
+  std::swap(B.stn_ini_a, c->stn_ini_a_alt); std::swap(B.stn_ini_b, c->stn_ini_b_alt);
+  c->layout_version++;
+  return 0;
+}
+static int enq_adami(lq_ctx* c) {
+  Section s(c, LQ_K_ADAMI);
+  if (c->npm0 > 0) launch_adami(c->stream, c->dP, c->par.npb, c->B.cur, c->B.cell_beg, c->B.cell_end);
+  return 0;
+}
+static int enq_corner(lq_ctx* c) {
+  Section s(c, LQ_K_CORNER);
+  if (c->par.npb > 0) launch_corner(c->stream, c->dP, c->npm0, c->B.cur, c->B.cell_beg, c->B.cell_end);
+  return 0;
+}
+static int enq_shepard(lq_ctx* c) {
+  Section s(c, LQ_K_SHEPARD);
+  launch_shepard(c->stream, c->dP, c->npm0, c->B.cur, c->B.cell_beg, c->B.cell_end, c->B.str_tmp_a, c->B.str_tmp_b);
+  return 0;
+}
+static int enq_interaction(lq_ctx* c) {
+  launch_reset_deferred(c->stream, c->dP);
+  {
+    Section s(c, LQ_K_INTERACTION);
+    launch_interaction(c->stream, c->dP, c->npm0, c->B.cur, c->B, c->B.cell_beg, c->B.cell_end,
+                       c->par.interaction_variant);
+  }
+  {
+    Section s(c, LQ_K_CONTACT);
+    launch_contact(c->stream, c->dP, c->npm0, c->B.cur, c->B, c->B.cell_beg, c->B.cell_end);
+  }
+  return 0;
+}
+static int enq_predictor(lq_ctx* c) {
+  Section s(c, LQ_K_PREDICTOR);
+  launch_predictor(c->stream, c->dP, c->par.np, c->B.cur, c->B);
+  return 0;
+}
+static int enq_corrector(lq_ctx* c) {
+  Section s(c, LQ_K_CORRECTOR);
+  launch_corrector(c->stream, c->dP, c->par.npb, c->npm0, c->par.n_motion, c->B.cur, c->B);
+  return 0;
+}
+
+static int check_launch(lq_ctx* c) {
+  HIPCHECK(c, hipGetLastError());
+  return 0;
+}
+
+extern "C" int lq_step_calculation(lq_ctx* c) { NEED_UPLOAD(c); hipSetDevice(c->device); enq_step_calc(c); return check_launch(c); }
+extern "C" int lq_neighbor_search(lq_ctx* c) { NEED_UPLOAD(c); hipSetDevice(c->device); enq_ns(c); return check_launch(c); }
+extern "C" int lq_sort_arrays_primary(lq_ctx* c) { NEED_UPLOAD(c); hipSetDevice(c->device); enq_sort_primary(c); return check_launch(c); }
+extern "C" int lq_sort_arrays_all(lq_ctx* c) { NEED_UPLOAD(c); hipSetDevice(c->device); enq_sort_all(c); return check_launch(c); }
+extern "C" int lq_rigid_body_update(lq_ctx* c) {
+  NEED_UPLOAD(c);
+  hipSetDevice(c->device);
+  if (c->par.n_motion > 0) launch_rigid_body_update(c->stream, c->dP);
+  return check_launch(c);
+}
+extern "C" int lq_adami_boundary(lq_ctx* c) { NEED_UPLOAD(c); hipSetDevice(c->device); enq_adami(c); return check_launch(c); }
+extern "C" int lq_find_corner_particle(lq_ctx* c) { NEED_UPLOAD(c); hipSetDevice(c->device); enq_corner(c); return check_launch(c); }
+extern "C" int lq_stress_regularization(lq_ctx* c) { NEED_UPLOAD(c); hipSetDevice(c->device); enq_shepard(c); return check_launch(c); }
+extern "C" int lq_particle_interaction(lq_ctx* c) { NEED_UPLOAD(c); hipSetDevice(c->device); enq_interaction(c); return check_launch(c); }
+extern "C" int lq_predictor(lq_ctx* c) { NEED_UPLOAD(c); hipSetDevice(c->device); enq_predictor(c); return check_launch(c); }
+extern "C" int lq_corrector(lq_ctx* c) { NEED_UPLOAD(c); hipSetDevice(c->device); enq_corrector(c); return check_launch(c); }
+
+// One step (loquat.cu:366-436, 548-560).  The corrector-section neighbour
+// search and SortArraysAll are skipped: the predictor never re-bins a
+// particle and no key changes between the two searches, so the second
+// stable sort is the identity permutation (asserted by the oracle tests).
+static int enq_step(lq_ctx* c) {
+  const int shep = c->par.stress_shepard_filter;
+  enq_step_calc(c);
+  enq_ns(c);
+  enq_sort_primary(c);
+  if (c->par.n_motion > 0) launch_rigid_body_update(c->stream, c->dP);
+  enq_adami(c);
+  enq_corner(c);
+  if (shep != 0 && c->host_step % shep == 0 && c->host_step != 0) enq_shepard(c);
+  enq_interaction(c);
+  enq_predictor(c);
+  enq_adami(c);
+  enq_corner(c);
+  enq_interaction(c);
+  enq_corrector(c);
+  launch_advance(c->stream, c->dP);
+  c->host_step++;
+  return 0;
+}
+
+extern "C" int lq_step(lq_ctx* c, int nsteps, int use_graph) {
+  NEED_UPLOAD(c);
+  if (nsteps < 0) return fail(c, LQ_ERR_ARG, "lq_step: nsteps < 0");
+  hipSetDevice(c->device);
+  const bool graph_ok = use_graph && !c->timing && c->par.stress_shepard_filter == 0;
+  int done = 0;
+  if (graph_ok && nsteps >= 2) {
+    if (c->graph == nullptr || c->graph_version != c->layout_version) {
+      if (c->graph) { hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+      hipGraph_t g;
+      HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      enq_step(c);
+      enq_step(c);  // two steps: the cur/alt buffer swap returns to its start
+      c->host_step -= 2;
+      HIPCHECK(c, hipStreamEndCapture(c->stream, &g));
+      HIPCHECK(c, hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0));
+      hipGraphDestroy(g);
+      c->graph_version = c->layout_version;
+    }
+    while (nsteps - done >= 2) {
+      HIPCHECK(c, hipGraphLaunch(c->graph, c->stream));
+      c->host_step += 2;
+      done += 2;
+    }
+  }
+  for (; done < nsteps; done++) enq_step(c);
+  return check_launch(c);
+}
+
+extern "C" int lq_synchronize(lq_ctx* c) {
+  if (!c) return LQ_ERR_ARG;
+  hipSetDevice(c->device);
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  DevParams d;
+  int st = pull_params(c, d);
+  if (st) return st;
+  if (d.err == DERR_DELETE) return fail(c, LQ_ERR_DELETE, "ERROR in deleting particles (npb + npm + np_delete_m != np)");
+  if (d.err == DERR_DOMAIN) return fail(c, LQ_ERR_DOMAIN, "a boundary particle left the cell grid");
+  return LQ_OK;
+}
+
+extern "C" int lq_get_counts(lq_ctx* c, int32_t* npb, int32_t* npm, int32_t* ndel) {
+  if (!c) return LQ_ERR_ARG;
+  int st = lq_synchronize(c);
+  if (st) return st;
+  DevParams d;
+  st = pull_params(c, d);
+  if (st) return st;
+  if (npb) *npb = d.npb;
+  if (npm) *npm = d.npm;
+  if (ndel) *ndel = d.np_delete_m;
+  return LQ_OK;
+}
+
+extern "C" int lq_wall_force(lq_ctx* c, const float box[3], double* F) {
+  NEED_UPLOAD(c);
+  if (!box || !F) return fail(c, LQ_ERR_ARG, "lq_wall_force: null");
+  hipSetDevice(c->device);
+  launch_wall_force(c->stream, c->dP, c->par.np, c->B.cur, box, c->B.partial, c->wf_blocks, c->wf_out);
+  HIPCHECK(c, hipGetLastError());
+  HIPCHECK(c, hipMemcpyAsync(c->wf_host, c->wf_out, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  *F = *c->wf_host;
+  return LQ_OK;
+}
+
+extern "C" int lq_positions_absolute(lq_ctx* c, float* dst, size_t bytes) {
+  NEED_UPLOAD(c);
+  if (!dst || bytes != size_t(c->par.np) * 16) return fail(c, LQ_ERR_ARG, "lq_positions_absolute: size");
+  hipSetDevice(c->device);
+  float4* tmp = c->B.fdef;  // scratch of np float4
+  launch_rel2abs(c->stream, c->dP, c->par.np, c->B.cur, tmp);
+  HIPCHECK(c, hipGetLastError());
+  HIPCHECK(c, hipMemcpyAsync(dst, tmp, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return LQ_OK;
+}
+
+extern "C" int lq_enable_timing(lq_ctx* c, int enable) {
+  if (!c) return LQ_ERR_ARG;
+  resolve_timing(c);
+  c->timing = enable != 0;
+  for (int k = 0; k < LQ_K_CLASS_COUNT; k++) { c->ms[k] = 0; c->launches[k] = 0; }
+  return LQ_OK;
+}
+
+extern "C" int lq_kernel_times(lq_ctx* c, double* ms, int64_t* launches, int n) {
+  if (!c) return LQ_ERR_ARG;
+  hipSetDevice(c->device);
+  resolve_timing(c);
+  for (int k = 0; k < n && k < LQ_K_CLASS_COUNT; k++) {
+    if (ms) ms[k] = c->ms[k];
+    if (launches) launches[k] = c->launches[k];
+  }
+  return LQ_OK;
+}
+
+extern "C" int lq_neighbor_lists(lq_ctx* c, int max_nb, int32_t* counts, int32_t* ids, size_t ids_bytes) {
+  NEED_UPLOAD(c);
+  if (max_nb <= 0 || !counts || !ids) return fail(c, LQ_ERR_ARG, "lq_neighbor_lists: args");
+  hipSetDevice(c->device);
+  DevParams d;
+  int st = pull_params(c, d);
+  if (st) return st;
+  const size_t n = size_t(d.npm);
+  if (ids_bytes < n * max_nb * 4) return fail(c, LQ_ERR_ARG, "lq_neighbor_lists: ids buffer too small");
+  int *dc = nullptr, *di = nullptr;
+  HIPCHECK(c, hipMalloc(&dc, std::max<size_t>(n, 1) * 4));
+  hipError_t e = hipMalloc(&di, std::max<size_t>(n * max_nb, 1) * 4);
+  if (e != hipSuccess) { hipFree(dc); return fail(c, LQ_ERR_HIP, "hipMalloc ids"); }
+  launch_neighbor_list(c->stream, c->dP, int(n), c->B.cur, c->B.cell_beg, c->B.cell_end, max_nb, dc, di);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(counts, dc, n * 4, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(ids, di, n * max_nb * 4, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  hipFree(dc);
+  hipFree(di);
+  if (e != hipSuccess) return fail(c, LQ_ERR_HIP, std::string("lq_neighbor_lists: ") + hipGetErrorString(e));
+  return LQ_OK;
+}

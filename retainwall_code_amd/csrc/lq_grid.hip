@@ -1,0 +1,247 @@
+// lq_grid.hip -- cell hash, stable LSD radix sort, cell ranges, reorder.
+//
+// Replaces LoquatGrid::NeighborSearch (loquat_grid.cu:147-191), whose sort is
+// CUB DeviceRadixSort::SortPairs over all 32 key bits.  Here the sort is a
+// hand-written stable LSD radix sort over 8-bit digits that runs only over
+// the bits the largest key needs (ceil(log2(ncell))), with per-tile digit
+// ranking done by 64-lane wave ballots.
+#include "lq_internal.hpp"
+#include "lq_launch.hpp"
+
+namespace lq {
+
+constexpr int RS_THREADS = 256;
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
+
+__device__ __forceinline__ int cell_key(const DevParams* P, int4 c) { return lin_key(P, c.x, c.y, c.z); }
+
+// cuGetParticleCellIndex, loquat_grid.cu:71-95.  Also flags whether any key
+// differs from the previous sorted key at the same slot.
+__global__ void k_hash(DevParams* __restrict__ P, Arrays A, int* __restrict__ keys, int* __restrict__ vals,
+                       const int* __restrict__ prev_sorted) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P->np) return;
+  int4 c = A.cellpos[i];
+  int key;
+  if (c.w == P->ncell_delete_m) {
+    key = P->ncell_delete_m;
+  } else {
+    key = cell_key(P, c);
+    if (A.idm[i].x >= 100.0f) key += P->ncell_material;  // MATERIAL_NUMBER, types.h:37
+    c.w = key;
+    A.cellpos[i] = c;
+  }
+  keys[i] = key;
+  vals[i] = i;
+  if (prev_sorted != nullptr && key != prev_sorted[i]) P->sort_changed = 1;
+}
+
+__global__ void k_fill_int(int* __restrict__ a, int n, int v) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] = v;
+}
+__global__ void k_clear_flag(DevParams* P) { P->sort_changed = 0; }
+
+__global__ void __launch_bounds__(RS_THREADS) k_radix_hist(const int* __restrict__ keys, int n, int shift,
+                                                           int* __restrict__ hist, int nblocks) {
+  __shared__ int cnt[256];
+  const int t = threadIdx.x;
+  cnt[t] = 0;
+  __syncthreads();
+  const int base = blockIdx.x * RS_TILE;
+#pragma unroll 4
+  for (int k = 0; k < RS_ITEMS; k++) {
+    int idx = base + k * RS_THREADS + t;
+    if (idx < n) atomicAdd(&cnt[(keys[idx] >> shift) & 255], 1);
+  }
+  __syncthreads();
+  hist[t * nblocks + blockIdx.x] = cnt[t];
+}
+
+// In-place exclusive scan of n ints by one 1024-thread block.
+__global__ void __launch_bounds__(1024) k_scan_excl(int* __restrict__ data, int n) {
+  __shared__ int sh[1024];
+  const int t = threadIdx.x;
+  const int per = (n + 1023) / 1024;
+  const int s = min(t * per, n), e = min(s + per, n);
+  int sum = 0;
+  for (int k = s; k < e; k++) sum += data[k];
+  sh[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    int v = t >= off ? sh[t - off] : 0;
+    __syncthreads();
+    sh[t] += v;
+    __syncthreads();
+  }
+  int run = sh[t] - sum;
+  for (int k = s; k < e; k++) {
+    int v = data[k];
+    data[k] = run;
+    run += v;
+  }
+}
+
+__global__ void __launch_bounds__(RS_THREADS) k_radix_scatter(const int* __restrict__ kin, const int* __restrict__ vin,
+                                                              int* __restrict__ kout, int* __restrict__ vout, int n,
+                                                              int shift, const int* __restrict__ offs, int nblocks) {
+  __shared__ int run[256];
+  __shared__ int wcnt[RS_THREADS / 64][256];
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  run[t] = offs[t * nblocks + blockIdx.x];
+  const unsigned long long lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  const int base = blockIdx.x * RS_TILE;
+  for (int k = 0; k < RS_ITEMS; k++) {
+    const int idx = base + k * RS_THREADS + t;
+    const bool valid = idx < n;
+    const int key = valid ? kin[idx] : 0;
+    const int d = (key >> shift) & 255;
+#pragma unroll
+    for (int ww = 0; ww < RS_THREADS / 64; ww++) wcnt[ww][t] = 0;
+    __syncthreads();
+    unsigned long long peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      const bool bit = (d >> b) & 1;
+      unsigned long long m = __ballot(bit);
+      peers &= bit ? m : ~m;
+    }
+    const int rank = __popcll(peers & lt);
+    if (valid && rank == 0) wcnt[w][d] = __popcll(peers);
+    __syncthreads();
+    if (valid) {
+      int pos = run[d] + rank;
+      for (int ww = 0; ww < w; ww++) pos += wcnt[ww][d];
+      kout[pos] = key;
+      vout[pos] = vin[idx];
+    }
+    __syncthreads();
+    int add = 0;
+#pragma unroll
+    for (int ww = 0; ww < RS_THREADS / 64; ww++) add += wcnt[ww][t];
+    run[t] += add;
+    __syncthreads();
+  }
+}
+
+// cuFindBegEndInCell, loquat_grid.cu:116-144 (half-open runs).
+__global__ void k_beg_end(const DevParams* __restrict__ P, const int* __restrict__ sorted, int* __restrict__ cbeg,
+                          int* __restrict__ cend) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = P->np;
+  if (i >= n) return;
+  int c = sorted[i];
+  if (i == 0 || c != sorted[i - 1]) {
+    cbeg[c] = i;
+    if (i > 0) cend[sorted[i - 1]] = i;
+  }
+  if (i == n - 1) cend[c] = n;
+}
+
+// Deleted-particle bookkeeping, loquat_grid.cu:165-185, kept on the device.
+__global__ void k_delete_count(DevParams* P, const int* __restrict__ cbeg, const int* __restrict__ cend) {
+  const int nd = P->ncell_delete_m;
+  int beg = cbeg[nd];
+  if (beg != -1) {
+    int now = cend[nd] - beg - P->np_delete_m;
+    P->npm -= now;
+    P->np_delete_m += now;
+  }
+  if (P->npb + P->npm + P->np_delete_m != P->np) set_err(P, DERR_DELETE);
+}
+
+// SortArraysPrimary / SortArraysAll (loquat.cu:254-287) as one gather of
+// every per-particle array into the alternate buffer set.  Rows past
+// npb+npm keep their stress/strain (the reference gathers only npb+npm rows).
+__global__ void __launch_bounds__(256) k_gather(const DevParams* __restrict__ P, Arrays src, Arrays dst,
+                                                const int* __restrict__ idx, Buffers B, int with_ini) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P->np) return;
+  const int s = idx[i];
+  dst.codes[i] = src.codes[s];
+  dst.cellpos[i] = src.cellpos[s];
+  dst.orig_id[i] = src.orig_id[s];
+  dst.pos[i] = src.pos[s];
+  dst.pos_ori[i] = src.pos_ori[s];
+  dst.vel[i] = src.vel[s];
+  dst.idm[i] = src.idm[s];
+  dst.cvar[i] = src.cvar[s];
+  dst.cf[i] = src.cf[s];
+  dst.cn[i] = src.cn[s];
+  dst.cw[i] = src.cw[s];
+  dst.normal[i] = i < P->npb ? src.normal[s] : src.normal[i];
+  const bool st = i < P->npb + P->npm;
+  const int ss = st ? s : i;
+  dst.str_a[i] = src.str_a[ss];
+  dst.str_b[i] = src.str_b[ss];
+  dst.stn_a[i] = src.stn_a[ss];
+  dst.stn_b[i] = src.stn_b[ss];
+}
+
+// The *_ini arrays of SortArraysAll are permuted in place through scratch.
+__global__ void __launch_bounds__(256) k_gather_ini(const DevParams* __restrict__ P, const int* __restrict__ idx,
+                                                    const float4* __restrict__ pi, const float4* __restrict__ vi,
+                                                    const float4* __restrict__ sia, const float2* __restrict__ sib,
+                                                    const float4* __restrict__ nia, const float2* __restrict__ nib,
+                                                    float4* __restrict__ po, float4* __restrict__ vo,
+                                                    float4* __restrict__ soa, float2* __restrict__ sob,
+                                                    float4* __restrict__ noa, float2* __restrict__ nob) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P->np) return;
+  const int s = idx[i];
+  po[i] = pi[s];
+  vo[i] = vi[s];
+  const bool st = i < P->npb + P->npm;
+  const int ss = st ? s : i;
+  soa[i] = sia[ss]; sob[i] = sib[ss];
+  noa[i] = nia[ss]; nob[i] = nib[ss];
+}
+
+static inline int blocks(long long n, int b) { return int((n + b - 1) / b); }
+
+void launch_fill_int(hipStream_t s, int* a, int n, int v) {
+  if (n > 0) hipLaunchKernelGGL(k_fill_int, dim3(std::min(blocks(n, 256), 4096)), dim3(256), 0, s, a, n, v);
+}
+
+int radix_tiles(int n) { return blocks(n, RS_TILE); }
+
+// Hash + stable radix sort + ranges + deletion bookkeeping.  Leaves the
+// sorted keys/values in *sorted_keys / *sorted_vals (pointers into B).
+void launch_neighbor_search(hipStream_t s, DevParams* P, int np, int nbits, int ncell, Arrays A, Buffers& B,
+                            const int** sorted_keys, const int** sorted_vals) {
+  hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, s, P);
+  hipLaunchKernelGGL(k_hash, dim3(blocks(np, 256)), dim3(256), 0, s, P, A, B.keys, B.vals, (const int*)nullptr);
+  int* kin = B.keys; int* vin = B.vals;
+  int* kout = B.keys_alt; int* vout = B.vals_alt;
+  const int nb = radix_tiles(np);
+  for (int shift = 0; shift < nbits; shift += 8) {
+    hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(RS_THREADS), 0, s, (const int*)kin, np, shift, B.hist, nb);
+    hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, s, B.hist, 256 * nb);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(RS_THREADS), 0, s, (const int*)kin, (const int*)vin, kout, vout,
+                       np, shift, (const int*)B.hist, nb);
+    std::swap(kin, kout);
+    std::swap(vin, vout);
+  }
+  B.part_cel = kin;
+  B.part_idx = vin;
+  *sorted_keys = kin;
+  *sorted_vals = vin;
+  launch_fill_int(s, B.cell_beg, ncell, -1);
+  launch_fill_int(s, B.cell_end, ncell, -1);
+  hipLaunchKernelGGL(k_beg_end, dim3(blocks(np, 256)), dim3(256), 0, s, P, (const int*)kin, B.cell_beg, B.cell_end);
+  hipLaunchKernelGGL(k_delete_count, dim3(1), dim3(1), 0, s, P, (const int*)B.cell_beg, (const int*)B.cell_end);
+}
+
+void launch_gather(hipStream_t s, const DevParams* P, int np, Arrays src, Arrays dst, const int* idx, const Buffers& B) {
+  if (np > 0) hipLaunchKernelGGL(k_gather, dim3(blocks(np, 256)), dim3(256), 0, s, P, src, dst, idx, B, 0);
+}
+
+void launch_gather_ini(hipStream_t s, const DevParams* P, int np, const int* idx, Buffers& B, float4* t4a,
+                       float4* t4b, float4* t4c, float2* t2a, float4* t4d, float2* t2b) {
+  if (np <= 0) return;
+  hipLaunchKernelGGL(k_gather_ini, dim3(blocks(np, 256)), dim3(256), 0, s, P, idx, (const float4*)B.pos_ini,
+                     (const float4*)B.vel_ini, (const float4*)B.str_ini_a, (const float2*)B.str_ini_b,
+                     (const float4*)B.stn_ini_a, (const float2*)B.stn_ini_b, t4a, t4b, t4c, t2a, t4d, t2b);
+}
+
+}  // namespace lq

@@ -1,0 +1,109 @@
+// lq_internal.hpp -- device-side data model of the gfx950 SPH step.
+//
+// Layout in HBM (structure of arrays, one row per sorted particle slot):
+//   float4 pos      relative x y z (to the 2h cell), p
+//   float4 vel      vx vy vz rho
+//   float4 idm      id mass idx M_crt
+//   int4   codes    block id, Fix/NonFix, model, motion id
+//   int4   cellpos  cx cy cz key
+//   float4 str_a/stn_a (xx yy zz xy) + float2 str_b/stn_b (xz yz)
+//   float4 cvar     K d slip wall-id (contact_var)
+//   float4 cf/cn/cw contact force / contact normal / wall velocity (contact_mag rows)
+// All per-step scalars (time, dt, current wall motion, npm) live in one
+// device-resident DevParams block so a whole step runs without host
+// round trips and can be captured in a hipGraph.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include "retainwall.h"
+
+namespace lq {
+
+struct DevMotion {  // RigidMotion, types.h:66-76
+  int is_trans, is_rotat, pattern, pad;
+  float3 trans_vel, trans_acc, pos_rp, rotat_vel, rotat_acc;
+};
+
+struct DevTable {  // MotionTable, types.h:81-97 (device copy, mutated each step)
+  int is_translation, table_size_trans, is_rotation, pattern, table_size_rotat, pad0, pad1, pad2;
+  float3 trans_vel[LQ_TABLE_MAX];
+  float3 trans_acc[LQ_TABLE_MAX];
+  float trans_time[LQ_TABLE_MAX];
+  float3 pos_rp, rotat_vel;
+  float3 rotat_acc[LQ_TABLE_MAX];
+  float rotat_time[LQ_TABLE_MAX];
+};
+
+// Per-material HypoCS constants hoisted out of constitutive_model.h:226-230.
+struct HypoConst {
+  float C1, C2, C3, C4, fa, fb, fc, sq2sin;
+};
+
+struct DevParams {
+  // ---- static (set at create) ----
+  int np, npb, cnx, cny, cnz, ncell, ncell_material, ncell_delete_m;
+  int key_mode, shepard, viscosity_type, n_motion, n_model, nbits, pad0, pad1;
+  float h, i_h, dr, vol_def, bt, eta, two_h, two_h_sq_lim;
+  float gx, gy, gz, damping, knorm, kgrad, kernel_zero, kernel_dr;
+  float art_exp, alpha_cs, art_p, cfl, cs, band, half_dr, mu;
+  float model_par[LQ_MODEL_NUMBER_MAX][LQ_MATERIAL_SIZE_MAX];
+  HypoConst hypo[LQ_MODEL_NUMBER_MAX];
+  // ---- dynamic (device-updated) ----
+  int npm, np_delete_m, err, n_deferred;
+  int sort_changed, pad2, pad3, pad4;
+  float time, dt;
+  double time_d;
+  long long step;
+  DevMotion mv[LQ_RIGID_MOTION_MAX];
+  DevTable tab[LQ_RIGID_MOTION_MAX];
+};
+
+// Device error codes written to DevParams::err (first error wins).
+enum { DERR_NONE = 0, DERR_DELETE = 1, DERR_DOMAIN = 2 };
+
+struct Arrays {
+  int4 *codes, *cellpos;
+  int* orig_id;
+  float4 *pos, *pos_ori, *vel, *idm, *normal;
+  float4 *str_a, *stn_a;
+  float2 *str_b, *stn_b;
+  float4 *cvar, *cf, *cn, *cw;
+};
+
+struct Buffers {
+  Arrays cur;   // current state
+  Arrays alt;   // gather destination (swapped with cur after each sort)
+  float4 *pos_ini, *vel_ini, *str_ini_a, *stn_ini_a;
+  float2 *str_ini_b, *stn_ini_b;
+  float4 *accad, *spin, *acestn_a;
+  float2* acestn_b;
+  float4* s3;        // per-soil partial acceleration kept for deferred contact
+  float4* fdef;      // pass-B contact force scratch
+  int* deferred;     // compact list of deferred soil slots
+  float4* str_tmp_a; // Shepard output scratch
+  float2* str_tmp_b;
+  // grid / sort
+  int *keys, *vals, *keys_alt, *vals_alt, *part_cel, *part_idx;
+  int *cell_beg, *cell_end;
+  int* hist;         // radix histogram [256][nblocks]
+  double* partial;   // reduction partials
+};
+
+// ------------------------------------------------------------- helpers ----
+__device__ __forceinline__ float kW(float knorm, float q) {  // functions.h:142
+  float t = 2.0f - q;
+  return knorm * t * t * t * t * (q + 0.5f);
+}
+__device__ __forceinline__ float kG(float kgrad, float q) {  // functions.h:143
+  float t = 2.0f - q;
+  return kgrad * t * t * t;
+}
+__device__ __forceinline__ int lin_key(const DevParams* P, int cx, int cy, int cz) {
+  return cx * P->cny * P->cnz + cy * P->cnz + cz;  // loquat_grid.cu:88
+}
+__device__ __forceinline__ bool in_grid(const DevParams* P, int cx, int cy, int cz) {
+  return cx >= 0 && cy >= 0 && cz >= 0 && cx < P->cnx && cy < P->cny && cz < P->cnz;
+}
+__device__ __forceinline__ void set_err(DevParams* P, int code) { atomicCAS(&P->err, 0, code); }
+
+}  // namespace lq

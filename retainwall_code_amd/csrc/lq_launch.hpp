@@ -1,0 +1,36 @@
+// lq_launch.hpp -- host-side launchers shared between the kernel TUs and the C ABI.
+#pragma once
+#include "lq_internal.hpp"
+
+namespace lq {
+
+void launch_abs2rel(hipStream_t s, const DevParams* P, int np, Arrays A, float scale);
+void launch_rel2abs(hipStream_t s, const DevParams* P, int np, Arrays A, float4* out);
+void launch_step_calc(hipStream_t s, DevParams* P);
+void launch_advance(hipStream_t s, DevParams* P);
+void launch_rigid_body_update(hipStream_t s, DevParams* P);
+void launch_adami(hipStream_t s, const DevParams* P, int npb, Arrays A, const int* cb, const int* ce);
+void launch_corner(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const int* cb, const int* ce);
+void launch_shepard(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const int* cb, const int* ce,
+                    float4* oa, float2* ob);
+void launch_reset_deferred(hipStream_t s, DevParams* P);
+void launch_interaction(hipStream_t s, DevParams* P, int npm_max, Arrays A, const Buffers& B, const int* cb,
+                        const int* ce, int variant);
+void launch_contact(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const Buffers& B, const int* cb,
+                    const int* ce);
+void launch_predictor(hipStream_t s, const DevParams* P, int np, Arrays A, const Buffers& B);
+void launch_corrector(hipStream_t s, DevParams* P, int npb, int npm_max, int n_motion, Arrays A, const Buffers& B);
+void launch_wall_force(hipStream_t s, const DevParams* P, int np, Arrays A, const float box[3], double* partial,
+                       int nblocks, double* out);
+void launch_neighbor_list(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const int* cb, const int* ce,
+                          int max_nb, int* counts, int* ids);
+
+void launch_fill_int(hipStream_t s, int* a, int n, int v);
+int radix_tiles(int n);
+void launch_neighbor_search(hipStream_t s, DevParams* P, int np, int nbits, int ncell, Arrays A, Buffers& B,
+                            const int** sorted_keys, const int** sorted_vals);
+void launch_gather(hipStream_t s, const DevParams* P, int np, Arrays src, Arrays dst, const int* idx, const Buffers& B);
+void launch_gather_ini(hipStream_t s, const DevParams* P, int np, const int* idx, Buffers& B, float4* t4a,
+                       float4* t4b, float4* t4c, float2* t2a, float4* t4d, float2* t2b);
+
+}  // namespace lq

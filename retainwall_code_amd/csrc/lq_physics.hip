@@ -1,0 +1,925 @@
+// lq_physics.hip -- per-particle SPH kernels of the retaining-wall step.
+//
+// Built with -ffp-contract=off: every fused multiply-add below is written
+// out (fmaf) so that each operation sequence is fixed and reproducible.
+#include "lq_internal.hpp"
+#include "lq_launch.hpp"
+
+namespace lq {
+
+__device__ __forceinline__ float3 ld3(const float4 v) { return make_float3(v.x, v.y, v.z); }
+__device__ __forceinline__ float3 f3add(float3 a, float3 b) { return make_float3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ float3 f3sub(float3 a, float3 b) { return make_float3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ float3 f3mul(float3 a, float s) { return make_float3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ float3 f3div(float3 a, float s) { return make_float3(a.x / s, a.y / s, a.z / s); }
+__device__ __forceinline__ float f3dot(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float f3norm(float3 a) { return sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); }
+__device__ __forceinline__ float3 f3cross(float3 a, float3 b) {  // functions.h:100
+  return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+struct Sym { float xx, yy, zz, xy, xz, yz; };
+__device__ __forceinline__ Sym ldsym(const float4* a, const float2* b, int i) {
+  float4 u = a[i]; float2 v = b[i];
+  return {u.x, u.y, u.z, u.w, v.x, v.y};
+}
+__device__ __forceinline__ void stsym(float4* a, float2* b, int i, Sym s) {
+  a[i] = make_float4(s.xx, s.yy, s.zz, s.xy);
+  b[i] = make_float2(s.xz, s.yz);
+}
+__device__ __forceinline__ Sym sadd(Sym a, Sym b) {
+  return {a.xx + b.xx, a.yy + b.yy, a.zz + b.zz, a.xy + b.xy, a.xz + b.xz, a.yz + b.yz};
+}
+__device__ __forceinline__ Sym smul(Sym a, float s) {
+  return {a.xx * s, a.yy * s, a.zz * s, a.xy * s, a.xz * s, a.yz * s};
+}
+__device__ __forceinline__ Sym sdiv(Sym a, float s) {
+  return {a.xx / s, a.yy / s, a.zz / s, a.xy / s, a.xz / s, a.yz / s};
+}
+__device__ __forceinline__ float strace(Sym a) { return a.xx + a.yy + a.zz; }
+__device__ __forceinline__ float sddot(Sym a, Sym b) {  // functions.h:95
+  return a.xx * b.xx + a.yy * b.yy + a.zz * b.zz + 2.0f * a.xy * b.xy + 2.0f * a.xz * b.xz + 2.0f * a.yz * b.yz;
+}
+__device__ __forceinline__ float3 smatvec(Sym s, float3 v) {  // functions.h:97
+  return make_float3(s.xx * v.x + s.xy * v.y + s.xz * v.z, s.xy * v.x + s.yy * v.y + s.yz * v.z,
+                     s.xz * v.x + s.yz * v.y + s.zz * v.z);
+}
+
+// Candidate geometry, particle_interaction.cu:30-33:
+// d = pos_i - pos_j - off*2h, r = |d| (r^2 = fma(dz,dz,fma(dy,dy,dx*dx))).
+__device__ __forceinline__ float pair_r(float3 pi, float4 pj, int ox, int oy, int oz, float two_h, float3& d) {
+  d.x = (pi.x - pj.x) - float(ox) * two_h;
+  d.y = (pi.y - pj.y) - float(oy) * two_h;
+  d.z = (pi.z - pj.z) - float(oz) * two_h;
+  return sqrtf(fmaf(d.z, d.z, fmaf(d.y, d.y, d.x * d.x)));
+}
+
+// ------------------------------------------------------------ frames ----
+// cuPositionFromAbsolute2Relative, loquat_grid.cu:253-261.
+__global__ void k_abs2rel(const DevParams* __restrict__ P, Arrays A, float scale) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P->np) return;
+  float4 p = A.pos[i];
+  int cx = int(p.x * scale), cy = int(p.y * scale), cz = int(p.z * scale);
+  A.cellpos[i] = make_int4(cx, cy, cz, 0);
+  A.pos[i] = make_float4(p.x - float(cx) * P->two_h, p.y - float(cy) * P->two_h, p.z - float(cz) * P->two_h, p.w);
+}
+
+// cuPositionFromRelative2Absolute, loquat_grid.cu:273-279.
+__global__ void k_rel2abs(const DevParams* __restrict__ P, Arrays A, float4* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P->np) return;
+  float4 p = A.pos[i];
+  int4 c = A.cellpos[i];
+  out[i] = make_float4(p.x + float(c.x) * P->two_h, p.y + float(c.y) * P->two_h, p.z + float(c.z) * P->two_h, p.w);
+}
+
+// ------------------------------------------------------ step scalars ----
+// StepCalculation, loquat.cu:366-374 (on device).
+__global__ void k_step_calc(DevParams* P) {
+  float dtm = P->h / P->cs;
+  P->dt = P->cfl * dtm;
+  P->time = float(P->time_d);
+}
+// loquat.cu:558-560: time += dt; step++.
+__global__ void k_advance(DevParams* P) {
+  P->time_d += double(P->dt);
+  P->step += 1;
+}
+__global__ void k_reset_deferred(DevParams* P) { P->n_deferred = 0; }
+
+// RigidBodyUpdate, adami_boundary.cu:175-203 (one thread).
+__global__ void k_rigid_body_update(DevParams* P) {
+  const float t = P->time, dt = P->dt;
+  for (int i = 0; i < P->n_motion; i++) {
+    DevTable& m = P->tab[i];
+    DevMotion& mv = P->mv[i];
+    if (m.is_rotation) {
+      for (int j = 0; j < m.table_size_rotat; j++) {
+        if (t < m.rotat_time[j]) {
+          m.rotat_vel = f3add(m.rotat_vel, f3mul(m.rotat_acc[j], dt));
+          float3 vel = f3add(m.trans_vel[j], f3mul(m.trans_acc[j], 0.5f * dt));
+          m.pos_rp = f3add(m.pos_rp, f3mul(vel, dt));
+          mv.rotat_acc = m.rotat_acc[j];
+          mv.rotat_vel = m.rotat_vel;
+          mv.pos_rp = m.pos_rp;
+          break;
+        }
+      }
+    }
+    if (m.is_translation) {
+      for (int j = 0; j < m.table_size_trans; j++) {
+        if (t < m.trans_time[j]) {
+          m.trans_vel[j] = f3add(m.trans_vel[j], f3mul(m.trans_acc[j], dt));
+          mv.trans_acc = m.trans_acc[j];
+          mv.trans_vel = m.trans_vel[j];
+          break;
+        }
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------- Adami ----
+// cuAdamiBoundaryMaterial + loops, adami_boundary.cu:26-158.
+__global__ void __launch_bounds__(256) k_adami(const DevParams* __restrict__ P, Arrays A,
+                                               const int* __restrict__ cbeg, const int* __restrict__ cend) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P->npb) return;
+  const float3 pi = ld3(A.pos[i]);
+  const int4 ci = A.cellpos[i];
+  const bool contact = A.codes[i].x >= 10;
+  const float two_h = P->two_h, i_h = P->i_h;
+  float3 n = make_float3(0.f, 0.f, 0.f);
+  Sym rs = {0, 0, 0, 0, 0, 0};
+  float3 rpos = make_float3(0.f, 0.f, 0.f);
+  float rker = 0.f;
+  float4 rv = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int cz = -1; cz <= 1; cz++)
+    for (int cy = -1; cy <= 1; cy++)
+      for (int cx = -1; cx <= 1; cx++) {
+        int jx = ci.x + cx, jy = ci.y + cy, jz = ci.z + cz;
+        if (!in_grid(P, jx, jy, jz)) continue;
+        int cb = lin_key(P, jx, jy, jz);
+        // boundary range: normal integral (contact walls only)
+        if (contact) {
+          for (int j = cbeg[cb], e = cend[cb]; j < e; j++) {
+            float3 d;
+            float r = pair_r(pi, A.pos[j], cx, cy, cz, two_h, d);
+            float q = r * i_h;
+            if (!(q < 2.0f)) continue;
+            float dwdr = kG(P->kgrad, q);
+            float3 g = f3mul(d, dwdr);
+            n = f3sub(n, f3mul(g, P->vol_def));
+          }
+        }
+        int cm = cb + P->ncell_material;
+        for (int j = cbeg[cm], e = cend[cm]; j < e; j++) {
+          float3 d;
+          float r = pair_r(pi, A.pos[j], cx, cy, cz, two_h, d);
+          float q = r * i_h;
+          if (!(q < 2.0f)) continue;
+          float ker = kW(P->knorm, q);
+          float4 idmj = A.idm[j];
+          float4 vj = A.vel[j];
+          float volj = idmj.y / vj.w;
+          float wv = ker * volj;
+          Sym sj = ldsym(A.str_a, A.str_b, j);
+          rs = sadd(rs, smul(sj, wv));
+          rpos = f3add(rpos, f3mul(d, ker * idmj.y));
+          rker = rker + wv;
+          rv = make_float4(rv.x + vj.x * wv, rv.y + vj.y * wv, rv.z + vj.z * wv, rv.w + vj.w * wv);
+        }
+      }
+  float nn = f3norm(n);
+  if (nn > 0.f) A.cn[i] = make_float4(n.x / nn, n.y / nn, n.z / nn, 0.f);
+  if (rker > 0.f) {
+    float4 nrm = A.normal[i];
+    float fc = nrm.w;
+    float3 ni = ld3(nrm);
+    float3 vref = f3div(make_float3(rv.x, rv.y, rv.z), rker);
+    float3 vref_n = f3mul(ni, f3dot(ni, vref));
+    float3 vref_t = f3sub(vref, vref_n);
+    float3 vrb = make_float3(0.f, 0.f, 0.f);
+    int4 cd = A.codes[i];
+    if (cd.y == LQ_NONFIX) {
+      const DevMotion& m = P->mv[cd.w];
+      vrb = m.trans_vel;
+      if (m.is_rotat) {
+        float3 pa = make_float3(pi.x + float(ci.x) * two_h, pi.y + float(ci.y) * two_h, pi.z + float(ci.z) * two_h);
+        vrb = f3add(vrb, f3cross(m.rotat_vel, f3sub(pa, m.pos_rp)));
+      }
+    }
+    float3 vrb_n = f3mul(ni, f3dot(ni, vrb));
+    float3 vrb_t = f3sub(vrb, vrb_n);
+    float3 vext_n = f3sub(f3mul(vrb_n, 2.0f), vref_n);
+    float3 vext_t = fc == -1.0f ? f3sub(f3mul(vrb_t, 2.0f), vref_t) : vref_t;
+    float3 vext = f3add(vext_n, vext_t);
+    float rho = rv.w / rker;
+    if (contact) {
+      A.vel[i] = make_float4(vrb.x, vrb.y, vrb.z, rho);
+      A.cw[i] = make_float4(vext.x, vext.y, vext.z, 0.f);
+    } else {
+      A.vel[i] = make_float4(vext.x, vext.y, vext.z, rho);
+    }
+    Sym sref = sdiv(rs, rker);
+    float3 gr = f3div(make_float3(P->gx * rpos.x, P->gy * rpos.y, P->gz * rpos.z), rker);
+    sref.xx = sref.xx - gr.x; sref.yy = sref.yy - gr.y; sref.zz = sref.zz - gr.z;
+    stsym(A.str_a, A.str_b, i, sref);
+  } else {
+    stsym(A.str_a, A.str_b, i, Sym{0, 0, 0, 0, 0, 0});
+  }
+}
+
+// ------------------------------------------------------------ corner ----
+// cuFindCornerParticleMaterial + loop, adami_boundary.cu:209-292.
+__global__ void __launch_bounds__(256) k_corner(const DevParams* __restrict__ P, Arrays A,
+                                                const int* __restrict__ cbeg, const int* __restrict__ cend) {
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= P->npm) return;
+  const int i = P->npb + idx;
+  const float3 pi = ld3(A.pos[i]);
+  const int4 ci = A.cellpos[i];
+  const float two_h = P->two_h, i_h = P->i_h;
+  float4 cv = A.cvar[i];
+  float4 cn = A.cn[i];
+  float4 cw = A.cw[i];
+  cv.x = 0.f;
+  cv.y = 999.f;
+  float dij = 999.f;
+  for (int cz = -1; cz <= 1; cz++)
+    for (int cy = -1; cy <= 1; cy++)
+      for (int cx = -1; cx <= 1; cx++) {
+        int jx = ci.x + cx, jy = ci.y + cy, jz = ci.z + cz;
+        if (!in_grid(P, jx, jy, jz)) continue;
+        int c = lin_key(P, jx, jy, jz);
+        for (int j = cbeg[c], e = cend[c]; j < e; j++) {
+          float3 d;
+          float r = pair_r(pi, A.pos[j], cx, cy, cz, two_h, d);
+          float q = r * i_h;
+          if (!(q < 2.0f)) continue;
+          int cj = A.codes[j].x;
+          if (cj < 10) continue;
+          float dwdr = kG(P->kgrad, q);
+          float3 g = f3mul(d, dwdr);
+          if (r < dij && r > 0.f) {
+            float3 na = ld3(A.cn[j]);
+            float3 nm = f3div(d, r);
+            dij = r;
+            cv.y = fabsf(f3dot(d, na));
+            cv.w = float(cj);
+            float4 vj = A.vel[j];
+            cw = make_float4(vj.x, vj.y, vj.z, 0.f);
+            float cosc = fabsf(f3dot(nm, na));
+            if (cosc < 0.866f) cn = make_float4(nm.x, nm.y, nm.z, 0.f);
+            else cn = make_float4(na.x, na.y, na.z, 0.f);
+          }
+          cv.x = cv.x + fabsf((cn.x * g.x + cn.y * g.y + cn.z * g.z) * P->vol_def);
+        }
+      }
+  A.cvar[i] = cv;
+  A.cn[i] = cn;
+  A.cw[i] = cw;
+}
+
+// ----------------------------------------------------------- Shepard ----
+// cuStressRegularizationShepard + loop, field_smoothing.cu:26-90.
+__global__ void __launch_bounds__(256) k_shepard(const DevParams* __restrict__ P, Arrays A,
+                                                 const int* __restrict__ cbeg, const int* __restrict__ cend,
+                                                 float4* __restrict__ oa, float2* __restrict__ ob) {
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= P->npm) return;
+  const int i = P->npb + idx;
+  const float3 pi = ld3(A.pos[i]);
+  const int4 ci = A.cellpos[i];
+  Sym acc = {0, 0, 0, 0, 0, 0};
+  float sk = 0.f;
+  for (int cz = -1; cz <= 1; cz++)
+    for (int cy = -1; cy <= 1; cy++)
+      for (int cx = -1; cx <= 1; cx++) {
+        int jx = ci.x + cx, jy = ci.y + cy, jz = ci.z + cz;
+        if (!in_grid(P, jx, jy, jz)) continue;
+        int c = lin_key(P, jx, jy, jz) + P->ncell_material;
+        for (int j = cbeg[c], e = cend[c]; j < e; j++) {
+          float3 d;
+          float r = pair_r(pi, A.pos[j], cx, cy, cz, P->two_h, d);
+          float q = r * P->i_h;
+          if (!(q < 2.0f)) continue;
+          float ker = kW(P->knorm, q);
+          float vol = A.idm[j].y / A.vel[j].w;
+          float wv = ker * vol;
+          sk = sk + wv;
+          acc = sadd(acc, smul(ldsym(A.str_a, A.str_b, j), wv));
+        }
+      }
+  if (sk != 0.f) acc = sdiv(acc, sk);
+  stsym(oa, ob, i, acc);
+}
+
+__global__ void k_copy_soil_str(const DevParams* __restrict__ P, Arrays A, const float4* __restrict__ oa,
+                                const float2* __restrict__ ob) {
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= P->npm) return;
+  int i = P->npb + idx;
+  A.str_a[i] = oa[i];
+  A.str_b[i] = ob[i];
+}
+
+// ------------------------------------------------------- interaction ----
+// Per-pair terms of cuParticleInteractionMaterialLoopSolution,
+// particle_interaction.cu:26-115, accumulated into the per-particle sums.
+struct IAcc {
+  float accden;
+  float3 as, av, aa;
+  float gv[9];  // xx xy xz yx yy yz zx zy zz
+};
+
+__device__ __forceinline__ void pair_terms(const DevParams* __restrict__ P, IAcc& S, float3 d, float r, float q,
+                                           float3 vi, float rhoi, Sym sri, float artpi, int shift, bool bound,
+                                           float4 vj, float mj_soil, Sym sj, int codej) {
+  float ker = kW(P->knorm, q);
+  float dwdr = kG(P->kgrad, q);
+  float3 g = f3mul(d, dwdr);
+  float rhoj = vj.w;
+  float mj = bound ? P->vol_def * rhoj : mj_soil;
+  float3 dv = f3sub(vi, ld3(vj));
+  if (codej >= 10 && shift == 1) dv = make_float3(0.f, 0.f, 0.f);
+  float volj = mj / rhoj;
+  float3 dvv = f3mul(dv, volj);
+  S.gv[0] = fmaf(-dvv.x, g.x, S.gv[0]); S.gv[1] = fmaf(-dvv.x, g.y, S.gv[1]); S.gv[2] = fmaf(-dvv.x, g.z, S.gv[2]);
+  S.gv[3] = fmaf(-dvv.y, g.x, S.gv[3]); S.gv[4] = fmaf(-dvv.y, g.y, S.gv[4]); S.gv[5] = fmaf(-dvv.y, g.z, S.gv[5]);
+  S.gv[6] = fmaf(-dvv.z, g.x, S.gv[6]); S.gv[7] = fmaf(-dvv.z, g.y, S.gv[7]); S.gv[8] = fmaf(-dvv.z, g.z, S.gv[8]);
+  if (codej >= 10) return;
+  float dvg = fmaf(dv.z, g.z, fmaf(dv.y, g.y, dv.x * g.x));
+  S.accden = fmaf(rhoi * dvg, volj, S.accden);
+  float rhoj2 = rhoj * rhoj;
+  Sym srj = sdiv(sj, rhoj2);
+  Sym st = sadd(sri, srj);
+  float3 f = smatvec(st, g);
+  S.as.x = fmaf(f.x, mj, S.as.x); S.as.y = fmaf(f.y, mj, S.as.y); S.as.z = fmaf(f.z, mj, S.as.z);
+  if (P->viscosity_type == 1) {
+    float visc = fmaf(dv.z, d.z, fmaf(dv.y, d.y, dv.x * d.x));
+    if (visc < 0.f) {
+      float phi = P->h * visc / fmaf(r, r, P->eta);
+      float artv = P->alpha_cs * phi / (0.5f * (rhoi + rhoj));
+      float s = mj * artv;
+      S.av.x = fmaf(g.x, s, S.av.x); S.av.y = fmaf(g.y, s, S.av.y); S.av.z = fmaf(g.z, s, S.av.z);
+    }
+  }
+  if (P->art_p != 0.f) {
+    float trj = strace(srj);
+    float artp = artpi + (trj > 0.f ? trj * P->art_p : 0.f);
+    if (artp > 0.f) {
+      artp = artp * powf(ker / P->kernel_dr, P->art_exp);
+      float s = mj * artp;
+      S.aa.x = fmaf(-g.x, s, S.aa.x); S.aa.y = fmaf(-g.y, s, S.aa.y); S.aa.z = fmaf(-g.z, s, S.aa.z);
+    }
+  }
+}
+
+// Epilogue of cuParticleInteractionMaterial (particle_interaction.cu:284-295)
+// with pass A of the wall contact (LoopContact, :174-237).
+__device__ __forceinline__ void interaction_epilogue(DevParams* __restrict__ P, Arrays A, const Buffers& B, int i,
+                                                     const IAcc& S, float3 vi) {
+  float3 damp = f3mul(vi, P->damping);
+  float3 s3 = f3add(f3add(S.as, S.av), S.aa);
+  float mi = A.idm[i].y;
+  const float3 g = make_float3(P->gx, P->gy, P->gz);
+  float3 acc0 = f3add(f3sub(s3, damp), g);
+  const float dt = P->dt;
+  float3 vp = make_float3(fmaf(acc0.x, dt, vi.x), fmaf(acc0.y, dt, vi.y), fmaf(acc0.z, dt, vi.z));
+  float4 cvi = A.cvar[i];
+  float3 nb = ld3(A.cn[i]);
+  float3 vb = ld3(A.cw[i]);
+  float3 dvc = f3sub(vb, vp);
+  float3 ast = make_float3(0.f, 0.f, 0.f);
+  float dvn = f3dot(dvc, nb);
+  bool deferred = false;
+  if (cvi.y <= P->h * 2.0f && dvn > 0.f) {
+    float3 dvn3 = f3mul(nb, dvn);
+    float3 fn = f3div(f3mul(dvn3, mi), dt);
+    float3 ft = f3div(f3mul(f3sub(dvc, dvn3), mi), dt);
+    float afn = f3norm(fn), aft = f3norm(ft);
+    if (aft > P->mu * afn) { ft = f3mul(ft, P->mu * afn / aft); cvi.z = 1.f; }
+    else cvi.z = 0.f;
+    A.cvar[i] = cvi;
+    float3 fr = f3add(ft, fn);
+    if (cvi.y > P->dr) {
+      deferred = true;
+    } else {
+      A.cf[i] = make_float4(fr.x, fr.y, fr.z, 0.f);
+      ast = f3mul(fr, 1.0f / mi);
+    }
+  }
+  if (deferred) {
+    int k = atomicAdd(&P->n_deferred, 1);
+    B.deferred[k] = i;
+    B.s3[i] = make_float4(s3.x, s3.y, s3.z, mi);
+  }
+  float3 acc = f3add(f3sub(f3add(s3, ast), damp), g);
+  B.accad[i] = make_float4(acc.x, acc.y, acc.z, S.accden);
+  B.acestn_a[i] = make_float4(S.gv[0], S.gv[4], S.gv[8], 0.5f * (S.gv[1] + S.gv[3]));
+  B.acestn_b[i] = make_float2(0.5f * (S.gv[2] + S.gv[6]), 0.5f * (S.gv[5] + S.gv[7]));
+  B.spin[i] = make_float4(0.5f * (S.gv[3] - S.gv[1]), 0.5f * (S.gv[6] - S.gv[2]), 0.5f * (S.gv[7] - S.gv[5]), 0.f);
+}
+
+// Reference-order variant: each thread walks its 27 cells (cz, cy, cx;
+// boundary range then soil range) exactly as particle_interaction.cu:117-140.
+__global__ void __launch_bounds__(256) k_interaction_ref(DevParams* __restrict__ P, Arrays A, Buffers B,
+                                                         const int* __restrict__ cbeg, const int* __restrict__ cend) {
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= P->npm) return;
+  const int i = P->npb + idx;
+  const float3 pi = ld3(A.pos[i]);
+  const int4 ci = A.cellpos[i];
+  const float4 vri = A.vel[i];
+  const float3 vi = ld3(vri);
+  const float rhoi = vri.w;
+  const float rhoi2 = rhoi * rhoi;
+  const Sym sri = sdiv(ldsym(A.str_a, A.str_b, i), rhoi2);
+  const float tri = strace(sri);
+  const float artpi = tri > 0.f ? tri * P->art_p : 0.f;
+  const int shift = int(A.cvar[i].z);
+  const float two_h = P->two_h, i_h = P->i_h;
+  IAcc S;
+  S.accden = 0.f;
+  S.as = S.av = S.aa = make_float3(0.f, 0.f, 0.f);
+#pragma unroll
+  for (int k = 0; k < 9; k++) S.gv[k] = 0.f;
+  for (int cz = -1; cz <= 1; cz++)
+    for (int cy = -1; cy <= 1; cy++)
+      for (int cx = -1; cx <= 1; cx++) {
+        int jx = ci.x + cx, jy = ci.y + cy, jz = ci.z + cz;
+        if (!in_grid(P, jx, jy, jz)) continue;
+        int cb = lin_key(P, jx, jy, jz);
+        for (int pass = 0; pass < 2; pass++) {
+          const bool bound = pass == 0;
+          int c = bound ? cb : cb + P->ncell_material;
+          for (int j = cbeg[c], e = cend[c]; j < e; j++) {
+            float3 d;
+            float r = pair_r(pi, A.pos[j], cx, cy, cz, two_h, d);
+            float q = r * i_h;
+            if (!(q < 2.0f)) continue;
+            pair_terms(P, S, d, r, q, vi, rhoi, sri, artpi, shift, bound, A.vel[j], A.idm[j].y,
+                       ldsym(A.str_a, A.str_b, j), A.codes[j].x);
+          }
+        }
+      }
+  interaction_epilogue(P, A, B, i, S, vi);
+}
+
+// Neighbour-list variant.  Each wave keeps a per-lane list of neighbour
+// slots in LDS: lanes scan candidates in reference order, append hits
+// (pair index j and cell-offset code) and, whenever any lane's list is full
+// or the scan ends, the wave evaluates the stored pairs with every lane busy.
+// Summation order per particle is the reference order, as in k_interaction_ref.
+constexpr int NL_WAVE = 64;
+constexpr int NL_WAVES = 4;
+constexpr int NL_CAP = 24;
+
+__global__ void __launch_bounds__(NL_WAVE * NL_WAVES) k_interaction_nl(DevParams* __restrict__ P, Arrays A, Buffers B,
+                                                                      const int* __restrict__ cbeg,
+                                                                      const int* __restrict__ cend) {
+  __shared__ int s_list[NL_WAVES][NL_CAP][NL_WAVE];
+  const int lane = threadIdx.x & (NL_WAVE - 1);
+  const int wv = threadIdx.x / NL_WAVE;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = idx < P->npm;
+  const int i = P->npb + (active ? idx : 0);
+  const float3 pi = ld3(A.pos[i]);
+  const int4 ci = A.cellpos[i];
+  const float4 vri = A.vel[i];
+  const float3 vi = ld3(vri);
+  const float rhoi = vri.w;
+  const float rhoi2 = rhoi * rhoi;
+  const Sym sri = sdiv(ldsym(A.str_a, A.str_b, i), rhoi2);
+  const float tri = strace(sri);
+  const float artpi = tri > 0.f ? tri * P->art_p : 0.f;
+  const int shift = int(A.cvar[i].z);
+  const float two_h = P->two_h, i_h = P->i_h;
+  const int ncm = P->ncell_material;
+  IAcc S;
+  S.accden = 0.f;
+  S.as = S.av = S.aa = make_float3(0.f, 0.f, 0.f);
+#pragma unroll
+  for (int k = 0; k < 9; k++) S.gv[k] = 0.f;
+  // scan state: cell counter 0..53 (27 cells x {boundary, soil}) and slot
+  int cstep = active ? 0 : 54;
+  int j = 0, jend = 0;
+  int n = 0;
+  while (true) {
+    // fill phase: scan until list full or scan exhausted
+    while (n < NL_CAP) {
+      if (j >= jend) {
+        // advance to next non-empty range
+        bool found = false;
+        while (cstep < 54) {
+          int cell = cstep >> 1, bsel = cstep & 1;
+          int cz = cell / 9 - 1, cy = (cell / 3) % 3 - 1, cx = cell % 3 - 1;
+          cstep++;
+          int jx = ci.x + cx, jy = ci.y + cy, jz = ci.z + cz;
+          if (!in_grid(P, jx, jy, jz)) continue;
+          int c = lin_key(P, jx, jy, jz) + (bsel ? ncm : 0);
+          int b = cbeg[c];
+          if (b < 0) continue;
+          j = b;
+          jend = cend[c];
+          found = true;
+          break;
+        }
+        if (!found) break;
+      }
+      int cur = cstep - 1;
+      int cell = cur >> 1;
+      int cz = cell / 9 - 1, cy = (cell / 3) % 3 - 1, cx = cell % 3 - 1;
+      float3 d;
+      float r = pair_r(pi, A.pos[j], cx, cy, cz, two_h, d);
+      float q = r * i_h;
+      if (q < 2.0f) {
+        s_list[wv][n][lane] = int((unsigned(j) << 6) | unsigned(cur));
+        n++;
+      }
+      j++;
+    }
+    // evaluate phase
+    int nmax = n;
+    for (int o = 32; o > 0; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o));
+    for (int k = 0; k < nmax; k++) {
+      if (k < n) {
+        int e = s_list[wv][k][lane];
+        int jj = int(unsigned(e) >> 6), cur = e & 63;
+        int cell = cur >> 1;
+        bool bound = (cur & 1) == 0;
+        int cz = cell / 9 - 1, cy = (cell / 3) % 3 - 1, cx = cell % 3 - 1;
+        float3 d;
+        float r = pair_r(pi, A.pos[jj], cx, cy, cz, two_h, d);
+        float q = r * i_h;
+        pair_terms(P, S, d, r, q, vi, rhoi, sri, artpi, shift, bound, A.vel[jj], A.idm[jj].y,
+                   ldsym(A.str_a, A.str_b, jj), A.codes[jj].x);
+      }
+    }
+    bool more = (j < jend) || (cstep < 54);
+    n = 0;
+    unsigned long long anymore = __ballot(more);
+    if (anymore == 0ull) break;
+  }
+  if (active) interaction_epilogue(P, A, B, i, S, vi);
+}
+
+// Pass B: same-row redistribution for deferred particles
+// (particle_interaction.cu:142-172, 209-237), reading pass-A values only.
+__global__ void k_contact_redistribute(const DevParams* __restrict__ P, Arrays A, Buffers B,
+                                       const int* __restrict__ cbeg, const int* __restrict__ cend) {
+  const int nd = P->n_deferred;
+  const float two_h = P->two_h;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nd; k += gridDim.x * blockDim.x) {
+    const int i = B.deferred[k];
+    const int4 ci = A.cellpos[i];
+    const float3 pi = ld3(A.pos[i]);
+    const float3 pa = make_float3(pi.x + float(ci.x) * two_h, pi.y + float(ci.y) * two_h, pi.z + float(ci.z) * two_h);
+    const float3 ni = ld3(A.cn[i]);
+    float K = A.cvar[i].x;
+    float3 fr = make_float3(0.f, 0.f, 0.f);
+    int count = 0;
+    for (int cz = -1; cz <= 1; cz++)
+      for (int cy = -1; cy <= 1; cy++)
+        for (int cx = -1; cx <= 1; cx++) {
+          int jx = ci.x + cx, jy = ci.y + cy, jz = ci.z + cz;
+          if (!in_grid(P, jx, jy, jz)) continue;
+          int c = lin_key(P, jx, jy, jz) + P->ncell_material;
+          for (int j = cbeg[c], e = cend[c]; j < e; j++) {
+            float4 cvj = A.cvar[j];
+            if (!(cvj.y <= P->band)) continue;
+            int4 cjj = A.cellpos[j];
+            float4 pj = A.pos[j];
+            float3 pja = make_float3(pj.x + float(cjj.x) * two_h, pj.y + float(cjj.y) * two_h, pj.z + float(cjj.z) * two_h);
+            float3 dc = f3sub(pa, pja);
+            float rcol = fabsf(dc.x * ni.z + dc.y * ni.y + dc.z * ni.x);
+            float Kj = cvj.x;
+            if (rcol <= P->half_dr && Kj != 0.f) {
+              if (K > Kj) K = Kj;
+              float4 fj = A.cf[j];
+              fr = make_float3(fj.x * K / Kj, fj.y * K / Kj, fj.z * K / Kj);
+              count++;
+            }
+          }
+        }
+    if (count == 0) fr = f3mul(fr, 0.f);
+    B.fdef[k] = make_float4(fr.x, fr.y, fr.z, K);
+    float4 s3m = B.s3[i];
+    float3 ast = f3mul(fr, 1.0f / s3m.w);
+    float3 vi = ld3(A.vel[i]);
+    float3 damp = f3mul(vi, P->damping);
+    const float3 g = make_float3(P->gx, P->gy, P->gz);
+    float3 acc = f3add(f3sub(f3add(make_float3(s3m.x, s3m.y, s3m.z), ast), damp), g);
+    float4 old = B.accad[i];
+    B.accad[i] = make_float4(acc.x, acc.y, acc.z, old.w);
+  }
+}
+
+// Pass C: publish pass-B forces and the min-reduced K.
+__global__ void k_contact_commit(const DevParams* __restrict__ P, Arrays A, Buffers B) {
+  const int nd = P->n_deferred;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nd; k += gridDim.x * blockDim.x) {
+    const int i = B.deferred[k];
+    float4 f = B.fdef[k];
+    A.cf[i] = make_float4(f.x, f.y, f.z, 0.f);
+    float4 cv = A.cvar[i];
+    cv.x = f.w;
+    A.cvar[i] = cv;
+  }
+}
+
+// ------------------------------------------------------------ HypoCS ----
+// cuMatModHypo (critical state), constitutive_model.h:209-299.
+__device__ __forceinline__ void hypo_cs(const DevParams* __restrict__ P, int model, float dt, float rho, float3 w,
+                                        Sym D, Sym& sig, float& Mcrt) {
+  const float* mp = P->model_par[model];
+  const HypoConst& hc = P->hypo[model];
+  const float coh = mp[3], pref = mp[4], ec0 = mp[5], lam = mp[6], xi = mp[7], alpha = mp[8];
+  const float emin = mp[9], emax = mp[10], e0 = mp[11], rho0 = mp[12];
+  Sym s = sig;
+  if (coh > 0.f) { s.xx = s.xx - coh; s.yy = s.yy - coh; s.zz = s.zz - coh; }
+  const float Sxx = s.xx, Syy = s.yy, Szz = s.zz, Sxy = s.xy, Sxz = s.xz, Syz = s.yz;
+  // W S - S W with W = SkewTensor2FullTensor(w), symmetric part
+  float WSxx = -w.z * Sxy + w.y * Sxz, WSxy = -w.z * Syy + w.y * Syz, WSxz = -w.z * Syz + w.y * Szz;
+  float WSyx = w.z * Sxx - w.x * Sxz, WSyy = w.z * Sxy - w.x * Syz, WSyz = w.z * Sxz - w.x * Szz;
+  float WSzx = -w.y * Sxx + w.x * Sxy, WSzy = -w.y * Sxy + w.x * Syy, WSzz = -w.y * Sxz + w.x * Syz;
+  float SWxx = Sxy * w.z - Sxz * w.y, SWxy = -Sxx * w.z + Sxz * w.x, SWxz = Sxx * w.y - Sxy * w.x;
+  float SWyx = Syy * w.z - Syz * w.y, SWyy = -Sxy * w.z + Syz * w.x, SWyz = Sxy * w.y - Syy * w.x;
+  float SWzx = Syz * w.z - Szz * w.y, SWzy = -Sxz * w.z + Szz * w.x, SWzz = Sxz * w.y - Syz * w.x;
+  Sym jau = {WSxx - SWxx, WSyy - SWyy, WSzz - SWzz, 0.5f * ((WSxy - SWxy) + (WSyx - SWyx)),
+             0.5f * ((WSxz - SWxz) + (WSzx - SWzx)), 0.5f * ((WSyz - SWyz) + (WSzy - SWzy))};
+  float I1 = strace(s);
+  if (I1 > -4.0f) { sig = Sym{-1.f, -1.f, -2.f, 0.f, 0.f, 0.f}; return; }
+  Sym dev = s;
+  float third = I1 / 3.0f;
+  dev.xx = dev.xx - third; dev.yy = dev.yy - third; dev.zz = dev.zz - third;
+  float trs = strace(s), trd = strace(D), trsd = sddot(s, D), nd = sqrtf(sddot(D, D));
+  float p = -I1 / 3.0f;
+  float e = (1.0f + e0) * rho0 / rho - 1.0f;
+  e = fmaxf(e, emin);
+  e = fminf(e, emax);
+  float ec = ec0 * expf(-lam * powf(p / pref, xi));
+  float fd = powf((e - emin) / (ec - emin), alpha);
+  Mcrt = e - ec;
+  Sym sd = sadd(s, dev);
+  Sym rate = sadd(sadd(sadd(sadd(jau, smul(D, hc.C1 * trs)), smul(s, hc.C2 * trd)), smul(s, hc.C3 * trsd / trs)),
+                  smul(sd, fd * hc.C4 * nd));
+  Sym ns = sadd(sig, smul(rate, dt));
+  float sph = strace(ns) / 3.0f;
+  if (sph > 0.f) { sig = Sym{-1.0f + coh, -1.0f + coh, -2.0f + coh, 0.f, 0.f, 0.f}; return; }
+  Sym nd6 = ns;
+  nd6.xx = nd6.xx - sph; nd6.yy = nd6.yy - sph; nd6.zz = nd6.zz - sph;
+  const float fa = hc.fa, fb = hc.fb, fc = hc.fc;
+  float u = 1.0f + fa + fb / 3.0f;
+  float f2 = fd * fd * fc * fc;
+  float A_ = f2 / 3.0f - u * u;
+  float B_ = 2.0f * f2 * (u * u + 2.0f * fa + 3.0f) - 4.0f * fb * u;
+  float C_ = 4.0f / 3.0f * f2 * fb * fb - 4.0f * fb * fb;
+  float k2 = (-B_ - sqrtf(B_ * B_ - 4.0f * A_ * C_)) / (2.0f * A_);
+  float kphi = 3.0f / sqrtf(k2);
+  float tau = sqrtf(0.5f * sddot(nd6, nd6));
+  if (tau - (-sph * kphi) > 0.f) {
+    float ratio = tau < 1e-10f ? 1.0f : (-kphi * sph) / tau;
+    nd6 = smul(nd6, ratio);
+  }
+  sig = Sym{nd6.xx + sph + coh, nd6.yy + sph + coh, nd6.zz + sph + coh, nd6.xy, nd6.xz, nd6.yz};
+}
+
+// --------------------------------------------------------- predictor ----
+// LoquatSph::Predictor + cuPredictorMaterial, predictor_corrector.cu:52-103:
+// the *_ini snapshot and the half-step update in one pass.
+__global__ void __launch_bounds__(256) k_predictor(const DevParams* __restrict__ P, Arrays A, Buffers B) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P->np) return;
+  const int npm = P->npm, npb = P->npb;
+  float4 p = A.pos[i], v = A.vel[i];
+  B.pos_ini[i] = p;
+  B.vel_ini[i] = v;
+  if (npm == 0 || i >= npb + npm) return;
+  float4 sa = A.str_a[i], na = A.stn_a[i];
+  float2 sb = A.str_b[i], nb = A.stn_b[i];
+  B.str_ini_a[i] = sa; B.str_ini_b[i] = sb;
+  B.stn_ini_a[i] = na; B.stn_ini_b[i] = nb;
+  if (i < npb) return;
+  int4 cd = A.codes[i];
+  if (cd.y != LQ_NONFIX) return;
+  const float hdt = 0.5f * P->dt;
+  float4 a = B.accad[i];
+  v = make_float4(fmaf(a.x, hdt, v.x), fmaf(a.y, hdt, v.y), fmaf(a.z, hdt, v.z), fmaf(a.w, hdt, v.w));
+  p = make_float4(fmaf(v.x, hdt, p.x), fmaf(v.y, hdt, p.y), fmaf(v.z, hdt, p.z), p.w);
+  const float* mp = P->model_par[cd.w];
+  float rho = fmaxf(v.w, mp[13]);
+  rho = fminf(rho, mp[14]);
+  v.w = rho;
+  Sym D = ldsym(B.acestn_a, B.acestn_b, i);
+  Sym stn = {na.x, na.y, na.z, na.w, nb.x, nb.y};
+  stsym(A.stn_a, A.stn_b, i, sadd(stn, smul(D, hdt)));
+  Sym sig = {sa.x, sa.y, sa.z, sa.w, sb.x, sb.y};
+  float Mcrt = 0.f;
+  if (cd.z == LQ_HYPO_CS) hypo_cs(P, cd.w, hdt, rho, ld3(B.spin[i]), D, sig, Mcrt);
+  stsym(A.str_a, A.str_b, i, sig);
+  A.pos[i] = p;
+  A.vel[i] = v;
+  float4 id = A.idm[i];
+  id.x = 100.f;
+  A.idm[i] = id;
+}
+
+// --------------------------------------------------------- corrector ----
+__device__ __forceinline__ int rebin(float p, float two_h) { return int(p >= two_h) - int(p < 0.0f); }
+
+// cuCorrectorBound, predictor_corrector.cu:109-137.
+__global__ void __launch_bounds__(256) k_corrector_bound(DevParams* __restrict__ P, Arrays A, Buffers B) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P->npb) return;
+  int4 cd = A.codes[i];
+  if (cd.y != LQ_NONFIX) return;
+  const DevMotion& m = P->mv[cd.w];
+  const float two_h = P->two_h, dt = P->dt;
+  float3 vel = m.trans_vel;
+  int4 ci = A.cellpos[i];
+  if (m.is_rotat && m.pattern != LQ_CIRCLE) {
+    float4 pp = A.pos[i];
+    float3 pa = make_float3(pp.x + float(ci.x) * two_h, pp.y + float(ci.y) * two_h, pp.z + float(ci.z) * two_h);
+    vel = f3add(vel, f3cross(m.rotat_vel, f3sub(pa, m.pos_rp)));
+    float3 ang = f3mul(m.rotat_vel, dt);  // GetRodriguesMatrix, functions.h:126-131
+    float th = f3norm(ang);
+    if (th > 0.f) {
+      float3 k = f3div(ang, th);
+      float sn = sinf(th), cc = 1.0f - cosf(th);
+      float Kxx = -(k.z * k.z + k.y * k.y), Kyy = -(k.z * k.z + k.x * k.x), Kzz = -(k.y * k.y + k.x * k.x);
+      float Rxx = 1.0f + Kxx * cc, Rxy = -k.z * sn + k.x * k.y * cc, Rxz = k.y * sn + k.x * k.z * cc;
+      float Ryx = k.z * sn + k.x * k.y * cc, Ryy = 1.0f + Kyy * cc, Ryz = -k.x * sn + k.y * k.z * cc;
+      float Rzx = -k.y * sn + k.x * k.z * cc, Rzy = k.x * sn + k.y * k.z * cc, Rzz = 1.0f + Kzz * cc;
+      float4 n = A.normal[i];
+      A.normal[i] = make_float4(Rxx * n.x + Rxy * n.y + Rxz * n.z, Ryx * n.x + Ryy * n.y + Ryz * n.z,
+                                Rzx * n.x + Rzy * n.y + Rzz * n.z, n.w);
+    }
+  }
+  float4 pini = B.pos_ini[i];
+  float3 p = f3add(ld3(pini), f3mul(vel, dt));
+  int dx = rebin(p.x, two_h), dy = rebin(p.y, two_h), dz = rebin(p.z, two_h);
+  p.x = p.x - float(dx) * two_h; p.y = p.y - float(dy) * two_h; p.z = p.z - float(dz) * two_h;
+  A.pos[i] = make_float4(p.x, p.y, p.z, pini.w);
+  ci.x += dx; ci.y += dy; ci.z += dz;
+  A.cellpos[i] = ci;
+  if (!in_grid(P, ci.x, ci.y, ci.z)) set_err(P, DERR_DOMAIN);
+}
+
+// cuCorrectorMaterial, predictor_corrector.cu:139-199.
+__global__ void __launch_bounds__(256) k_corrector_material(const DevParams* __restrict__ P, Arrays A, Buffers B) {
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= P->npm) return;
+  const int i = P->npb + idx;
+  int4 cd = A.codes[i];
+  if (cd.y != LQ_NONFIX) return;
+  const float dt = P->dt, hdt = 0.5f * dt, two_h = P->two_h;
+  float4 vini = B.vel_ini[i], a = B.accad[i], pini = B.pos_ini[i];
+  float4 v = make_float4(fmaf(a.x, dt, vini.x), fmaf(a.y, dt, vini.y), fmaf(a.z, dt, vini.z), fmaf(a.w, dt, vini.w));
+  float3 p = make_float3(fmaf(v.x + vini.x, hdt, pini.x), fmaf(v.y + vini.y, hdt, pini.y), fmaf(v.z + vini.z, hdt, pini.z));
+  const float* mp = P->model_par[cd.w];
+  float rho = fmaxf(v.w, mp[13]);
+  rho = fminf(rho, mp[14]);
+  v.w = rho;
+  int4 ci = A.cellpos[i];
+  int dx = rebin(p.x, two_h), dy = rebin(p.y, two_h), dz = rebin(p.z, two_h);
+  p.x = p.x - float(dx) * two_h; p.y = p.y - float(dy) * two_h; p.z = p.z - float(dz) * two_h;
+  ci.x += dx; ci.y += dy; ci.z += dz;
+  Sym D = ldsym(B.acestn_a, B.acestn_b, i);
+  Sym stn0 = ldsym(B.stn_ini_a, B.stn_ini_b, i);
+  stsym(A.stn_a, A.stn_b, i, sadd(stn0, smul(D, dt)));
+  Sym sig = ldsym(B.str_ini_a, B.str_ini_b, i);
+  float Mcrt = 0.f;
+  if (cd.z == LQ_HYPO_CS) hypo_cs(P, cd.w, dt, rho, ld3(B.spin[i]), D, sig, Mcrt);
+  stsym(A.str_a, A.str_b, i, sig);
+  float4 id = A.idm[i];
+  id.x = 100.f;
+  id.w = Mcrt;
+  if (!in_grid(P, ci.x, ci.y, ci.z)) { ci.w = P->ncell_delete_m; id.x = 999.f; }
+  A.idm[i] = id;
+  A.cellpos[i] = ci;
+  A.pos[i] = make_float4(p.x, p.y, p.z, pini.w);
+  A.vel[i] = v;
+  float4 po = A.pos_ori[i];
+  if (P->time <= P->dt) po = make_float4(0.f, 0.f, 0.f, 0.f);
+  po = make_float4(fmaf(v.x, dt, po.x), fmaf(v.y, dt, po.y), fmaf(v.z, dt, po.z), po.w);
+  A.pos_ori[i] = po;
+  B.accad[i] = po;
+}
+
+// -------------------------------------------------------- wall force ----
+// cuGetValuesAtPoint, utilities.cu:578-640, as a two-stage FP64 reduction.
+__global__ void __launch_bounds__(256) k_wall_force_partial(const DevParams* __restrict__ P, Arrays A, float bx,
+                                                            float bz0, float bz1, double* __restrict__ partial) {
+  __shared__ double sh[256];
+  double acc = 0.0;
+  const float two_h = P->two_h;
+  for (int i = P->npb + blockIdx.x * blockDim.x + threadIdx.x; i < P->np; i += gridDim.x * blockDim.x) {
+    float4 p = A.pos[i];
+    int4 c = A.cellpos[i];
+    float xa = p.x + float(c.x) * two_h;
+    float za = p.z + float(c.z) * two_h;
+    if (xa <= bx && za >= bz0 && za <= bz1) acc += 0.001 * double(A.cf[i].x);
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+__global__ void k_reduce_partials(const double* __restrict__ partial, int n, double* __restrict__ out) {
+  __shared__ double sh[256];
+  double acc = 0.0;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) acc += partial[k];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+// ------------------------------------------------- neighbour lists ------
+__global__ void k_neighbor_list(const DevParams* __restrict__ P, Arrays A, const int* __restrict__ cbeg,
+                                const int* __restrict__ cend, int max_nb, int* __restrict__ counts,
+                                int* __restrict__ ids) {
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= P->npm) return;
+  const int i = P->npb + idx;
+  const float3 pi = ld3(A.pos[i]);
+  const int4 ci = A.cellpos[i];
+  int n = 0;
+  for (int cz = -1; cz <= 1; cz++)
+    for (int cy = -1; cy <= 1; cy++)
+      for (int cx = -1; cx <= 1; cx++) {
+        int jx = ci.x + cx, jy = ci.y + cy, jz = ci.z + cz;
+        if (!in_grid(P, jx, jy, jz)) continue;
+        int cb = lin_key(P, jx, jy, jz);
+        for (int pass = 0; pass < 2; pass++) {
+          int c = pass ? cb + P->ncell_material : cb;
+          for (int j = cbeg[c], e = cend[c]; j < e; j++) {
+            float3 d;
+            float r = pair_r(pi, A.pos[j], cx, cy, cz, P->two_h, d);
+            if (!(r * P->i_h < 2.0f)) continue;
+            if (n < max_nb) ids[(size_t)idx * max_nb + n] = A.orig_id[j];
+            n++;
+          }
+        }
+      }
+  counts[idx] = n;
+}
+
+// --------------------------------------------------------- launchers ----
+static inline int blocks(long long n, int b) { return int((n + b - 1) / b); }
+
+void launch_abs2rel(hipStream_t s, const DevParams* P, int np, Arrays A, float scale) {
+  if (np > 0) hipLaunchKernelGGL(k_abs2rel, dim3(blocks(np, 256)), dim3(256), 0, s, P, A, scale);
+}
+void launch_rel2abs(hipStream_t s, const DevParams* P, int np, Arrays A, float4* out) {
+  if (np > 0) hipLaunchKernelGGL(k_rel2abs, dim3(blocks(np, 256)), dim3(256), 0, s, P, A, out);
+}
+void launch_step_calc(hipStream_t s, DevParams* P) { hipLaunchKernelGGL(k_step_calc, dim3(1), dim3(1), 0, s, P); }
+void launch_advance(hipStream_t s, DevParams* P) { hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, s, P); }
+void launch_rigid_body_update(hipStream_t s, DevParams* P) {
+  hipLaunchKernelGGL(k_rigid_body_update, dim3(1), dim3(1), 0, s, P);
+}
+void launch_adami(hipStream_t s, const DevParams* P, int npb, Arrays A, const int* cb, const int* ce) {
+  if (npb > 0) hipLaunchKernelGGL(k_adami, dim3(blocks(npb, 256)), dim3(256), 0, s, P, A, cb, ce);
+}
+void launch_corner(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const int* cb, const int* ce) {
+  if (npm_max > 0) hipLaunchKernelGGL(k_corner, dim3(blocks(npm_max, 256)), dim3(256), 0, s, P, A, cb, ce);
+}
+void launch_shepard(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const int* cb, const int* ce,
+                    float4* oa, float2* ob) {
+  if (npm_max <= 0) return;
+  hipLaunchKernelGGL(k_shepard, dim3(blocks(npm_max, 256)), dim3(256), 0, s, P, A, cb, ce, oa, ob);
+  hipLaunchKernelGGL(k_copy_soil_str, dim3(blocks(npm_max, 256)), dim3(256), 0, s, P, A, (const float4*)oa,
+                     (const float2*)ob);
+}
+void launch_reset_deferred(hipStream_t s, DevParams* P) {
+  hipLaunchKernelGGL(k_reset_deferred, dim3(1), dim3(1), 0, s, P);
+}
+void launch_interaction(hipStream_t s, DevParams* P, int npm_max, Arrays A, const Buffers& B, const int* cb,
+                        const int* ce, int variant) {
+  if (npm_max <= 0) return;
+  if (variant == LQ_INTERACTION_REFERENCE_ORDER)
+    hipLaunchKernelGGL(k_interaction_ref, dim3(blocks(npm_max, 256)), dim3(256), 0, s, P, A, B, cb, ce);
+  else
+    hipLaunchKernelGGL(k_interaction_nl, dim3(blocks(npm_max, NL_WAVE * NL_WAVES)), dim3(NL_WAVE * NL_WAVES), 0, s,
+                       P, A, B, cb, ce);
+}
+void launch_contact(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const Buffers& B, const int* cb,
+                    const int* ce) {
+  if (npm_max <= 0) return;
+  int g = std::min(blocks(npm_max, 256), 1024);
+  hipLaunchKernelGGL(k_contact_redistribute, dim3(g), dim3(256), 0, s, P, A, B, cb, ce);
+  hipLaunchKernelGGL(k_contact_commit, dim3(g), dim3(256), 0, s, P, A, B);
+}
+void launch_predictor(hipStream_t s, const DevParams* P, int np, Arrays A, const Buffers& B) {
+  if (np > 0) hipLaunchKernelGGL(k_predictor, dim3(blocks(np, 256)), dim3(256), 0, s, P, A, B);
+}
+void launch_corrector(hipStream_t s, DevParams* P, int npb, int npm_max, int n_motion, Arrays A, const Buffers& B) {
+  if (n_motion > 0 && npb > 0)
+    hipLaunchKernelGGL(k_corrector_bound, dim3(blocks(npb, 256)), dim3(256), 0, s, P, A, B);
+  if (npm_max > 0)
+    hipLaunchKernelGGL(k_corrector_material, dim3(blocks(npm_max, 256)), dim3(256), 0, s, P, A, B);
+}
+void launch_wall_force(hipStream_t s, const DevParams* P, int np, Arrays A, const float box[3], double* partial,
+                       int nblocks, double* out) {
+  hipLaunchKernelGGL(k_wall_force_partial, dim3(nblocks), dim3(256), 0, s, P, A, box[0], box[1], box[2], partial);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, s, (const double*)partial, nblocks, out);
+}
+void launch_neighbor_list(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const int* cb, const int* ce,
+                          int max_nb, int* counts, int* ids) {
+  if (npm_max > 0)
+    hipLaunchKernelGGL(k_neighbor_list, dim3(blocks(npm_max, 256)), dim3(256), 0, s, P, A, cb, ce, max_nb, counts,
+                       ids);
+}
+
+}  // namespace lq

@@ -1,0 +1,195 @@
+"""Python handle over the C ABI (libretainwall.so) for tests and benchmarks.
+
+Mirrors the reference's LoquatSph step pieces (loquat.h:97-120) one method per
+C entry point; every call raises on a non-zero status with lq_last_error().
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+
+
+class LqError(RuntimeError):
+    pass
+
+
+class Context:
+    def __init__(self, params: _abi.Params, device: int = 0, lib=None):
+        self.lib = lib or _abi.load_library()
+        self.params = params
+        h = C.c_void_p()
+        st = self.lib.lq_create(C.byref(params), device, C.byref(h))
+        self.h = h
+        if st:
+            msg = self.lib.lq_last_error(h).decode() if h.value else ""
+            if h.value:
+                self.lib.lq_destroy(h)
+            self.h = None
+            raise LqError(f"lq_create failed ({st}): {msg}")
+
+    # -------------------------------------------------------------- util --
+    def _chk(self, st, what):
+        if st:
+            raise LqError(f"{what} failed ({st}): {self.lib.lq_last_error(self.h).decode()}")
+
+    def close(self):
+        if self.h is not None:
+            self.lib.lq_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---------------------------------------------------------- transfer --
+    def upload(self, case):
+        self._keep = [np.ascontiguousarray(case.codes, np.int32), np.ascontiguousarray(case.pospres, np.float32),
+                      np.ascontiguousarray(case.velrhop, np.float32), np.ascontiguousarray(case.idmassidx, np.float32),
+                      np.ascontiguousarray(case.str, np.float32), np.ascontiguousarray(case.stn, np.float32),
+                      np.ascontiguousarray(case.normal, np.float32)]
+        hs = _abi.HostState(*[a.ctypes.data for a in self._keep])
+        self._chk(self.lib.lq_upload(self.h, C.byref(hs)), "lq_upload")
+
+    def set_motion(self, tables, n=None):
+        n = len(tables) if n is None else n
+        self._chk(self.lib.lq_set_motion_tables(self.h, n, tables), "lq_set_motion_tables")
+
+    def get_motion(self, n):
+        t = (_abi.MotionTable * max(n, 1))()
+        self._chk(self.lib.lq_get_motion_tables(self.h, n, t), "lq_get_motion_tables")
+        return t
+
+    def field_shape(self, name):
+        fid, dt, w, rows = _abi.FIELDS[name]
+        n = self.params.np if rows == "np" else self.params.ncell
+        return (n, w) if w > 1 else (n,), np.dtype(dt), fid
+
+    def get(self, name):
+        shape, dt, fid = self.field_shape(name)
+        out = np.empty(shape, dt)
+        self._chk(self.lib.lq_get_field(self.h, fid, out.ctypes.data, out.nbytes), f"lq_get_field({name})")
+        return out
+
+    def set(self, name, value):
+        shape, dt, fid = self.field_shape(name)
+        a = np.ascontiguousarray(np.asarray(value, dt).reshape(shape))
+        self._chk(self.lib.lq_set_field(self.h, fid, a.ctypes.data, a.nbytes), f"lq_set_field({name})")
+
+    def snapshot(self, names=None):
+        names = names or [k for k in _abi.FIELDS if k not in ("part_cel", "part_idx")]
+        return {k: self.get(k) for k in names}
+
+    def get_params(self):
+        p = _abi.Params()
+        self._chk(self.lib.lq_get_params(self.h, C.byref(p)), "lq_get_params")
+        return p
+
+    def time(self):
+        t = C.c_double()
+        s = C.c_int64()
+        self._chk(self.lib.lq_get_time(self.h, C.byref(t), C.byref(s)), "lq_get_time")
+        return t.value, s.value
+
+    def set_time(self, t, step):
+        self._chk(self.lib.lq_set_time(self.h, t, step), "lq_set_time")
+
+    # -------------------------------------------------------- step pieces --
+    def step_calculation(self):
+        self._chk(self.lib.lq_step_calculation(self.h), "lq_step_calculation")
+
+    def neighbor_search(self):
+        self._chk(self.lib.lq_neighbor_search(self.h), "lq_neighbor_search")
+
+    def sort_arrays_primary(self):
+        self._chk(self.lib.lq_sort_arrays_primary(self.h), "lq_sort_arrays_primary")
+
+    def sort_arrays_all(self):
+        self._chk(self.lib.lq_sort_arrays_all(self.h), "lq_sort_arrays_all")
+
+    def rigid_body_update(self):
+        self._chk(self.lib.lq_rigid_body_update(self.h), "lq_rigid_body_update")
+
+    def adami(self):
+        self._chk(self.lib.lq_adami_boundary(self.h), "lq_adami_boundary")
+
+    def find_corner(self):
+        self._chk(self.lib.lq_find_corner_particle(self.h), "lq_find_corner_particle")
+
+    def shepard(self):
+        self._chk(self.lib.lq_stress_regularization(self.h), "lq_stress_regularization")
+
+    def interaction(self):
+        self._chk(self.lib.lq_particle_interaction(self.h), "lq_particle_interaction")
+
+    def predictor(self):
+        self._chk(self.lib.lq_predictor(self.h), "lq_predictor")
+
+    def corrector(self):
+        self._chk(self.lib.lq_corrector(self.h), "lq_corrector")
+
+    def step(self, n=1, graph=False):
+        self._chk(self.lib.lq_step(self.h, n, 1 if graph else 0), "lq_step")
+
+    def sync(self):
+        self._chk(self.lib.lq_synchronize(self.h), "lq_synchronize")
+
+    def counts(self):
+        a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
+        self._chk(self.lib.lq_get_counts(self.h, C.byref(a), C.byref(b), C.byref(c)), "lq_get_counts")
+        return a.value, b.value, c.value
+
+    def wall_force(self, box):
+        b = (C.c_float * 3)(*box)
+        F = C.c_double()
+        self._chk(self.lib.lq_wall_force(self.h, b, C.byref(F)), "lq_wall_force")
+        return F.value
+
+    def positions_absolute(self):
+        out = np.empty((self.params.np, 4), np.float32)
+        self._chk(self.lib.lq_positions_absolute(self.h, out.ctypes.data, out.nbytes), "lq_positions_absolute")
+        return out
+
+    def enable_timing(self, on=True):
+        self._chk(self.lib.lq_enable_timing(self.h, 1 if on else 0), "lq_enable_timing")
+
+    def kernel_times(self):
+        n = len(_abi.KERNEL_CLASSES)
+        ms = (C.c_double * n)()
+        cnt = (C.c_int64 * n)()
+        self._chk(self.lib.lq_kernel_times(self.h, ms, cnt, n), "lq_kernel_times")
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(_abi.KERNEL_CLASSES)}
+
+    def neighbor_lists(self, max_nb=160):
+        npm = self.counts()[1]
+        counts = np.zeros(npm, np.int32)
+        ids = np.zeros((npm, max_nb), np.int32)
+        self._chk(self.lib.lq_neighbor_lists(self.h, max_nb, counts.ctypes.data, ids.ctypes.data, ids.nbytes),
+                  "lq_neighbor_lists")
+        return counts, ids
+
+
+def open_case(case, key_mode=_abi.KEY_LINEAR, variant=_abi.INTERACTION_NEIGHBOR_LIST, device=0, initial_sort=True):
+    """Create a context, upload a case and run the initial search/sort
+    (loquat.cu:528-531: GridConfig, Abs2Rel, NeighborSearch, SortArraysPrimary)."""
+    from . import cases
+    lib = _abi.load_library()
+    p = cases.make_params(case, key_mode, variant, lib=lib)
+    ctx = Context(p, device, lib)
+    ctx.upload(case)
+    ctx.set_motion(cases.motion_tables(case), len(case.motion))
+    if initial_sort:
+        ctx.neighbor_search()
+        ctx.sort_arrays_primary()
+        ctx.sync()
+    return ctx
