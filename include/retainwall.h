@@ -213,6 +213,10 @@ int lq_adami_boundary(lq_ctx* ctx);
 int lq_find_corner_particle(lq_ctx* ctx);
 /* LoquatSph::StressRegularization, field_smoothing.cu:74-101. */
 int lq_stress_regularization(lq_ctx* ctx);
+/* LoquatSph::StressSavingRegularization, field_smoothing.cu:103-112: the
+ * Shepard-smoothed stress written only to dst f32[np][6] for output (soil
+ * rows smoothed, other rows copied); the state is not changed.  Synchronises. */
+int lq_stress_saving_regularization(lq_ctx* ctx, float* dst, size_t bytes);
 /* ParticleInteractionMaterial, particle_interaction.cu:260-305, with the
  * wall-contact force resolved in two deterministic passes. */
 int lq_particle_interaction(lq_ctx* ctx);

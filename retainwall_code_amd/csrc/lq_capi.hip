@@ -795,6 +795,28 @@ extern "C" int lq_positions_absolute(lq_ctx* c, float* dst, size_t bytes) {
   return LQ_OK;
 }
 
+extern "C" int lq_stress_saving_regularization(lq_ctx* c, float* dst, size_t bytes) {
+  NEED_UPLOAD(c);
+  const size_t n = size_t(c->par.np);
+  if (!dst || bytes != n * 24) return fail(c, LQ_ERR_ARG, "lq_stress_saving_regularization: size");
+  hipSetDevice(c->device);
+  Buffers& B = c->B;
+  HIPCHECK(c, hipMemcpyAsync(B.str_tmp_a, B.cur.str_a, n * 16, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(B.str_tmp_b, B.cur.str_b, n * 8, hipMemcpyDeviceToDevice, c->stream));
+  launch_shepard(c->stream, c->dP, c->npm0, B.cur, B.cell_beg, B.cell_end, B.str_tmp_a, B.str_tmp_b, false);
+  HIPCHECK(c, hipGetLastError());
+  std::vector<float4> ta;
+  std::vector<float2> tb;
+  int st = 0;
+  if ((st = d2h(c, ta, B.str_tmp_a, n)) || (st = d2h(c, tb, B.str_tmp_b, n))) return st;
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  for (size_t i = 0; i < n; i++) {
+    dst[6 * i] = ta[i].x; dst[6 * i + 1] = ta[i].y; dst[6 * i + 2] = ta[i].z; dst[6 * i + 3] = ta[i].w;
+    dst[6 * i + 4] = tb[i].x; dst[6 * i + 5] = tb[i].y;
+  }
+  return LQ_OK;
+}
+
 extern "C" int lq_enable_timing(lq_ctx* c, int enable) {
   if (!c) return LQ_ERR_ARG;
   resolve_timing(c);

@@ -876,10 +876,10 @@ void launch_corner(hipStream_t s, const DevParams* P, int npm_max, Arrays A, con
   if (npm_max > 0) hipLaunchKernelGGL(k_corner, dim3(blocks(npm_max, 256)), dim3(256), 0, s, P, A, cb, ce);
 }
 void launch_shepard(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const int* cb, const int* ce,
-                    float4* oa, float2* ob) {
+                    float4* oa, float2* ob, bool copy_back) {
   if (npm_max <= 0) return;
   hipLaunchKernelGGL(k_shepard, dim3(blocks(npm_max, 256)), dim3(256), 0, s, P, A, cb, ce, oa, ob);
-  hipLaunchKernelGGL(k_copy_soil_str, dim3(blocks(npm_max, 256)), dim3(256), 0, s, P, A, (const float4*)oa,
+  if (copy_back) hipLaunchKernelGGL(k_copy_soil_str, dim3(blocks(npm_max, 256)), dim3(256), 0, s, P, A, (const float4*)oa,
                      (const float2*)ob);
 }
 void launch_reset_deferred(hipStream_t s, DevParams* P) {

@@ -239,13 +239,16 @@ def test_particle_deletion(oracle_mod):
     (predictor_corrector.cu:186-190) and dropped from npm by the next search
     (loquat_grid.cu:165-180)."""
     case = cases.config("tiny")
-    # the topmost soil particle gets a large upward velocity: it leaves the
-    # top of the grid within one step
+    # the topmost soil particle is lifted into the last cell row of the grid
+    # (above the free surface, no neighbours) with a speed that carries it
+    # half a cell past the top; re-binning moves it one cell, out of the grid
     soil = np.arange(case.npb, case.np)
     top = soil[np.argmax(case.pospres[soil, 2])]
     p = cases.make_params(case)
-    zmax = p.cell_num[2] * 2 * p.h
-    case.velrhop[top, 2] = float((zmax - case.pospres[top, 2]) / case.params["dt"]) * 1.5
+    two_h = 2 * np.float32(p.h)
+    zmax = p.cell_num[2] * two_h
+    case.pospres[top, 2] = zmax - np.float32(0.2) * two_h
+    case.velrhop[top, 2] = np.float32(0.5 * two_h / case.params["dt"])
     say(f"[delete] case=tiny particle={top}")
     ctx = sim.open_case(case)
     o32 = oracle_from(ctx, oracle_mod, np.float32, case)
