@@ -72,7 +72,7 @@ def test_cli_runs_case_and_matches_api(tmp_path):
         ctx.step(k - done)
         done = k
         F.append(ctx.wall_force(caseio_box(ctx)))
-    np.testing.assert_allclose(th[:, 1], F, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(th[:, 1], F, rtol=1e-5)  # written with 6 significant digits
     ctx.close()
 
 
