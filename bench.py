@@ -79,6 +79,8 @@ def main():
     ap.add_argument("--case", default="c2")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--graph", action="store_true", help="time with hipGraph replay (no per-kernel events)")
+    ap.add_argument("--probe-copy", type=int, default=0, metavar="BYTES",
+                    help="only run the STREAM copy probe over BYTES (HBM bandwidth / counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on exactly one GPU; multi-GPU runs are not part of "
@@ -87,6 +89,14 @@ def main():
         sys.exit(2)
 
     from retainwall_code_amd import cases, sim
+
+    if args.probe_copy:
+        ctx = sim.open_case(cases.config("tiny"))
+        ms = ctx.stream_copy(args.probe_copy, reps=20)
+        print(json.dumps({"probe": "stream_copy", "bytes_read": args.probe_copy, "bytes_written": args.probe_copy,
+                          "ms_per_launch": ms, "GBps": 2 * args.probe_copy / (ms * 1e-3) / 1e9}), flush=True)
+        ctx.close()
+        return
 
     case = cases.config(args.case)
     print(f"[bench] case={args.case} np={case.np} npb={case.npb} npm={case.npm}", file=sys.stderr, flush=True)

@@ -256,6 +256,10 @@ enum lq_kernel_class {
   LQ_K_SHEPARD = 8, LQ_K_CLASS_COUNT = 9
 };
 int lq_enable_timing(lq_ctx* ctx, int enable);
+/* STREAM-style copy of `bytes` between two device scratch buffers, `reps`
+ * launches, average milliseconds per launch (HBM bandwidth probe and the
+ * known-size copy used to calibrate hardware byte counters). */
+int lq_stream_copy(lq_ctx* ctx, size_t bytes, int reps, double* ms_per_launch);
 int lq_kernel_times(lq_ctx* ctx, double* ms, int64_t* launches, int n);
 
 /* Neighbour list of every soil slot (debug/parity): for soil slot s the

@@ -123,7 +123,7 @@ EXPORTS = [
     "lq_sort_arrays_primary", "lq_sort_arrays_all", "lq_rigid_body_update", "lq_adami_boundary",
     "lq_find_corner_particle", "lq_stress_regularization", "lq_particle_interaction",
     "lq_predictor", "lq_corrector", "lq_step", "lq_wall_force", "lq_positions_absolute",
-    "lq_get_counts", "lq_synchronize", "lq_stress_saving_regularization", "lq_enable_timing", "lq_kernel_times", "lq_neighbor_lists",
+    "lq_get_counts", "lq_synchronize", "lq_stress_saving_regularization", "lq_stream_copy", "lq_enable_timing", "lq_kernel_times", "lq_neighbor_lists",
 ]
 
 
@@ -157,6 +157,7 @@ def _proto(lib):
     lib.lq_stress_saving_regularization.argtypes = [vp, vp, C.c_size_t]
     lib.lq_get_counts.argtypes = [vp, P(i32), P(i32), P(i32)]
     lib.lq_enable_timing.argtypes = [vp, C.c_int]
+    lib.lq_stream_copy.argtypes = [vp, C.c_size_t, C.c_int, P(C.c_double)]
     lib.lq_kernel_times.argtypes = [vp, P(C.c_double), P(C.c_int64), C.c_int]
     lib.lq_neighbor_lists.argtypes = [vp, C.c_int, vp, vp, C.c_size_t]
 

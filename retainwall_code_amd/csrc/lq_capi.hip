@@ -817,6 +817,33 @@ extern "C" int lq_stress_saving_regularization(lq_ctx* c, float* dst, size_t byt
   return LQ_OK;
 }
 
+extern "C" int lq_stream_copy(lq_ctx* c, size_t bytes, int reps, double* ms) {
+  if (!c || bytes < 16 || reps <= 0 || !ms) return fail(c, LQ_ERR_ARG, "lq_stream_copy: args");
+  hipSetDevice(c->device);
+  const size_t n = bytes / 16;
+  float4 *a = nullptr, *b = nullptr;
+  HIPCHECK(c, hipMalloc(&a, n * 16));
+  hipError_t e = hipMalloc(&b, n * 16);
+  if (e != hipSuccess) { hipFree(a); return fail(c, LQ_ERR_HIP, "hipMalloc"); }
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  launch_stream_copy(c->stream, a, b, n);  // warm-up
+  hipEventRecord(e0, c->stream);
+  for (int r = 0; r < reps; r++) launch_stream_copy(c->stream, a, b, n);
+  hipEventRecord(e1, c->stream);
+  e = hipEventSynchronize(e1);
+  float t = 0.f;
+  hipEventElapsedTime(&t, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(a);
+  hipFree(b);
+  if (e != hipSuccess) return fail(c, LQ_ERR_HIP, "lq_stream_copy");
+  *ms = double(t) / reps;
+  return LQ_OK;
+}
+
 extern "C" int lq_enable_timing(lq_ctx* c, int enable) {
   if (!c) return LQ_ERR_ARG;
   resolve_timing(c);

@@ -25,6 +25,7 @@ void launch_wall_force(hipStream_t s, const DevParams* P, int np, Arrays A, cons
 void launch_neighbor_list(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const int* cb, const int* ce,
                           int max_nb, int* counts, int* ids);
 
+void launch_stream_copy(hipStream_t s, const float4* a, float4* b, size_t n);
 void launch_fill_int(hipStream_t s, int* a, int n, int v);
 int radix_tiles(int n);
 void launch_neighbor_search(hipStream_t s, DevParams* P, int np, int nbits, int ncell, Arrays A, Buffers& B,

@@ -855,6 +855,11 @@ __global__ void k_neighbor_list(const DevParams* __restrict__ P, Arrays A, const
   counts[idx] = n;
 }
 
+// ------------------------------------------------------- stream copy ----
+__global__ void __launch_bounds__(256) k_stream_copy(const float4* __restrict__ a, float4* __restrict__ b, size_t n) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) b[i] = a[i];
+}
+
 // --------------------------------------------------------- launchers ----
 static inline int blocks(long long n, int b) { return int((n + b - 1) / b); }
 
@@ -914,6 +919,10 @@ void launch_wall_force(hipStream_t s, const DevParams* P, int np, Arrays A, cons
                        int nblocks, double* out) {
   hipLaunchKernelGGL(k_wall_force_partial, dim3(nblocks), dim3(256), 0, s, P, A, box[0], box[1], box[2], partial);
   hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, s, (const double*)partial, nblocks, out);
+}
+void launch_stream_copy(hipStream_t s, const float4* a, float4* b, size_t n) {
+  int g = int(std::min<size_t>((n + 255) / 256, 8192));
+  hipLaunchKernelGGL(k_stream_copy, dim3(g), dim3(256), 0, s, a, b, n);
 }
 void launch_neighbor_list(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const int* cb, const int* ce,
                           int max_nb, int* counts, int* ids) {

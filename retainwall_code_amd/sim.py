@@ -163,6 +163,11 @@ class Context:
     def enable_timing(self, on=True):
         self._chk(self.lib.lq_enable_timing(self.h, 1 if on else 0), "lq_enable_timing")
 
+    def stream_copy(self, nbytes, reps=20):
+        ms = C.c_double()
+        self._chk(self.lib.lq_stream_copy(self.h, nbytes, reps, C.byref(ms)), "lq_stream_copy")
+        return ms.value
+
     def kernel_times(self):
         n = len(_abi.KERNEL_CLASSES)
         ms = (C.c_double * n)()
