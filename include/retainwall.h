@@ -47,11 +47,16 @@ enum lq_shape { LQ_CUBOID = 0, LQ_CIRCLE = 1 };      /* types.h:49 */
  * bit for bit.  MORTON interleaves the cell coordinates instead (extension). */
 enum lq_key_mode { LQ_KEY_LINEAR = 0, LQ_KEY_MORTON = 1 };
 
-/* Interaction kernel variant.  REFERENCE_ORDER walks the 27 cells per thread
- * exactly as particle_interaction.cu:117-140; NEIGHBOR_LIST (default) first
- * compacts each particle's neighbours in LDS (same order) and then evaluates
- * the pair terms with all lanes busy. */
-enum lq_interaction_variant { LQ_INTERACTION_NEIGHBOR_LIST = 0, LQ_INTERACTION_REFERENCE_ORDER = 1 };
+/* Interaction kernel variant.  All three produce bit-identical results.
+ * TILE (default) stages each cell-column tile's neighbourhood in LDS and
+ * scans it from there; REFERENCE_ORDER walks the 27 cells per thread exactly
+ * as particle_interaction.cu:117-140; NEIGHBOR_LIST scans global memory per
+ * lane and compacts hits into LDS lists before evaluating them. */
+enum lq_interaction_variant {
+  LQ_INTERACTION_TILE = 0,
+  LQ_INTERACTION_REFERENCE_ORDER = 1,
+  LQ_INTERACTION_NEIGHBOR_LIST = 2
+};
 
 /* RigidMotion, types.h:66-76 (the per-step imposed motion of one body). */
 typedef struct lq_rigid_motion {
@@ -261,6 +266,14 @@ int lq_enable_timing(lq_ctx* ctx, int enable);
  * known-size copy used to calibrate hardware byte counters). */
 int lq_stream_copy(lq_ctx* ctx, size_t bytes, int reps, double* ms_per_launch);
 int lq_kernel_times(lq_ctx* ctx, double* ms, int64_t* launches, int n);
+
+/* Tile shape of the TILE interaction variant: shape index (0: 1 wave,
+ * 1: 2 waves, 2: 4 waves per workgroup) and tz = cells per tile along z.
+ * lq_autotune times every shape over a few tile depths on the current state
+ * (contact arrays are restored afterwards), keeps the fastest and reports it.
+ * No reference counterpart (the reference has one fixed 128-thread launch). */
+int lq_set_tile(lq_ctx* ctx, int shape, int tz);
+int lq_autotune(lq_ctx* ctx, int reps, int* shape, int* tz, double* ms_per_call);
 
 /* Neighbour list of every soil slot (debug/parity): for soil slot s the
  * original ids (.dat line) of all particles j with q_ij < 2, in traversal

@@ -31,6 +31,7 @@ struct lq_ctx {
   bool uploaded = false;
   int nbits = 0;
   int npm0 = 0;
+  int tile_shape = 1, tile_tz = 7;
   long long host_step = 0;
   // instrumentation
   bool timing = false;
@@ -229,6 +230,7 @@ static void fill_static(const lq_params& p, DevParams& d) {
 static int alloc_arrays(lq_ctx* c, Arrays& A, size_t n) {
   int st = 0;
   st |= dalloc(c, &A.codes, n); st |= dalloc(c, &A.cellpos, n); st |= dalloc(c, &A.orig_id, n);
+  st |= dalloc(c, &A.pres, n);
   st |= dalloc(c, &A.pos, n); st |= dalloc(c, &A.pos_ori, n); st |= dalloc(c, &A.vel, n);
   st |= dalloc(c, &A.idm, n); st |= dalloc(c, &A.normal, n);
   st |= dalloc(c, &A.str_a, n); st |= dalloc(c, &A.stn_a, n); st |= dalloc(c, &A.str_b, n); st |= dalloc(c, &A.stn_b, n);
@@ -326,6 +328,7 @@ extern "C" int lq_upload(lq_ctx* c, const lq_host_state* h) {
   const int np = c->par.np, npb = c->par.npb;
   std::vector<int4> codes(np);
   std::vector<int> oid(np);
+  std::vector<float> pres(np);
   std::vector<float4> pos(np), vel(np), idm(np), nrm(np), sa(np), na(np);
   std::vector<float2> sb(np), nb(np);
   for (int i = 0; i < np; i++) {
@@ -343,16 +346,18 @@ extern "C" int lq_upload(lq_ctx* c, const lq_host_state* h) {
     nrm[i] = i < npb ? make_float4(h->normal[4 * i], h->normal[4 * i + 1], h->normal[4 * i + 2], h->normal[4 * i + 3])
                      : make_float4(0.f, 0.f, 0.f, 0.f);
     oid[i] = i;
+    pres[i] = h->pospres[4 * i + 3];
   }
   Arrays& A = c->B.cur;
   int st = 0;
   st |= h2d(c, A.codes, codes); st |= h2d(c, A.pos, pos); st |= h2d(c, A.pos_ori, pos);
   st |= h2d(c, A.vel, vel); st |= h2d(c, A.idm, idm); st |= h2d(c, A.normal, nrm);
   st |= h2d(c, A.str_a, sa); st |= h2d(c, A.str_b, sb); st |= h2d(c, A.stn_a, na); st |= h2d(c, A.stn_b, nb);
-  st |= h2d(c, A.orig_id, oid);
+  st |= h2d(c, A.orig_id, oid); st |= h2d(c, A.pres, pres);
   if (st) return st;
   const float scale = 0.5f * c->par.i_h;
   launch_abs2rel(c->stream, c->dP, np, A, scale);
+  launch_set_massflag(c->stream, c->dP, np, A);
   HIPCHECK(c, hipGetLastError());
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   c->uploaded = true;
@@ -531,6 +536,13 @@ extern "C" int lq_get_field(lq_ctx* c, int f, void* dst, size_t bytes) {
     HIPCHECK(c, hipStreamSynchronize(c->stream));
   } else if (v.width == 4) {
     HIPCHECK(c, hipMemcpyAsync(dst, f4_ptr(c, f), n * 16, hipMemcpyDeviceToHost, c->stream));
+    if (f == LQ_F_POS) {  // w on the device is the massflag; the host sees p
+      std::vector<float> pr;
+      if ((st = d2h(c, pr, c->B.cur.pres, n))) return st;
+      HIPCHECK(c, hipStreamSynchronize(c->stream));
+      float* o = (float*)dst;
+      for (size_t i = 0; i < n; i++) o[4 * i + 3] = pr[i];
+    }
     HIPCHECK(c, hipStreamSynchronize(c->stream));
   } else if (v.width == 6) {
     float4* a; float2* b;
@@ -572,6 +584,13 @@ extern "C" int lq_set_field(lq_ctx* c, int f, const void* src, size_t bytes) {
     HIPCHECK(c, hipMemcpyAsync(i1_ptr(c, f), src, n * 4, hipMemcpyHostToDevice, c->stream));
   } else if (v.width == 4) {
     HIPCHECK(c, hipMemcpyAsync(f4_ptr(c, f), src, n * 16, hipMemcpyHostToDevice, c->stream));
+    if (f == LQ_F_POS) {
+      const float* in = (const float*)src;
+      std::vector<float> pr(n);
+      for (size_t i = 0; i < n; i++) pr[i] = in[4 * i + 3];
+      HIPCHECK(c, hipMemcpyAsync(c->B.cur.pres, pr.data(), n * 4, hipMemcpyHostToDevice, c->stream));
+      HIPCHECK(c, hipStreamSynchronize(c->stream));
+    }
   } else if (v.width == 6) {
     float4* a; float2* b;
     sym_ptrs(c, f, &a, &b);
@@ -597,6 +616,7 @@ extern "C" int lq_set_field(lq_ctx* c, int f, const void* src, size_t bytes) {
     HIPCHECK(c, hipMemcpyAsync(c->B.cur.cw, f2.data(), n * 16, hipMemcpyHostToDevice, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
   }
+  if (f == LQ_F_POS || f == LQ_F_IDM || f == LQ_F_CODES) launch_set_massflag(c->stream, c->dP, c->par.np, c->B.cur);
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   return LQ_OK;
 }
@@ -647,12 +667,21 @@ static int enq_shepard(lq_ctx* c) {
   launch_shepard(c->stream, c->dP, c->npm0, c->B.cur, c->B.cell_beg, c->B.cell_end, c->B.str_tmp_a, c->B.str_tmp_b);
   return 0;
 }
+static void enq_interaction_kernel(lq_ctx* c) {
+  if (c->par.interaction_variant == LQ_INTERACTION_TILE) {
+    if (c->npm0 > 0)
+      launch_interaction_tile(c->stream, c->dP, c->par.cell_num[0], c->par.cell_num[1], c->par.cell_num[2], c->B.cur,
+                              c->B, c->B.cell_beg, c->B.cell_end, c->tile_shape, c->tile_tz);
+  } else {
+    launch_interaction(c->stream, c->dP, c->npm0, c->B.cur, c->B, c->B.cell_beg, c->B.cell_end,
+                       c->par.interaction_variant);
+  }
+}
 static int enq_interaction(lq_ctx* c) {
   launch_reset_deferred(c->stream, c->dP);
   {
     Section s(c, LQ_K_INTERACTION);
-    launch_interaction(c->stream, c->dP, c->npm0, c->B.cur, c->B, c->B.cell_beg, c->B.cell_end,
-                       c->par.interaction_variant);
+    enq_interaction_kernel(c);
   }
   {
     Section s(c, LQ_K_CONTACT);
@@ -860,6 +889,76 @@ extern "C" int lq_kernel_times(lq_ctx* c, double* ms, int64_t* launches, int n) 
     if (ms) ms[k] = c->ms[k];
     if (launches) launches[k] = c->launches[k];
   }
+  return LQ_OK;
+}
+
+extern "C" int lq_set_tile(lq_ctx* c, int shape, int tz) {
+  if (!c) return LQ_ERR_ARG;
+  if (shape < 0 || shape >= interaction_tile_shapes() || tz < 1 || tz > interaction_tile_maxtz(shape))
+    return fail(c, LQ_ERR_ARG, "lq_set_tile: shape/tz out of range");
+  c->tile_shape = shape;
+  c->tile_tz = tz;
+  c->layout_version++;  // re-capture graphs
+  return LQ_OK;
+}
+
+// Times the interaction kernel for each tile shape over a few depths on the
+// current state and keeps the fastest.  The launches mutate the slip flag and
+// contact forces, so cvar/cf are saved to the alternate buffers and restored.
+extern "C" int lq_autotune(lq_ctx* c, int reps, int* shape_out, int* tz_out, double* ms_out) {
+  NEED_UPLOAD(c);
+  if (reps <= 0) return fail(c, LQ_ERR_ARG, "lq_autotune: reps");
+  hipSetDevice(c->device);
+  const size_t n = size_t(c->par.np);
+  Arrays& A = c->B.cur;
+  Arrays& T = c->B.alt;
+  HIPCHECK(c, hipMemcpyAsync(T.cvar, A.cvar, n * 16, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(T.cf, A.cf, n * 16, hipMemcpyDeviceToDevice, c->stream));
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  const int save_shape = c->tile_shape, save_tz = c->tile_tz, save_var = c->par.interaction_variant;
+  c->par.interaction_variant = LQ_INTERACTION_TILE;
+  double best = 1e30;
+  int bs = save_shape, bt = save_tz;
+  for (int sh = 0; sh < interaction_tile_shapes(); sh++) {
+    const int waves = interaction_tile_waves(sh);
+    const int base = std::max(1, (waves * 64 * 9) / (10 * 16));  // ~90% of the lanes at ~16 soil per cell
+    for (int tz = std::max(1, base - 2); tz <= std::min(interaction_tile_maxtz(sh), base + 2); tz++) {
+      c->tile_shape = sh;
+      c->tile_tz = tz;
+      launch_reset_deferred(c->stream, c->dP);
+      enq_interaction_kernel(c);  // warm-up
+      hipEventRecord(e0, c->stream);
+      for (int r = 0; r < reps; r++) {
+        launch_reset_deferred(c->stream, c->dP);
+        enq_interaction_kernel(c);
+      }
+      hipEventRecord(e1, c->stream);
+      hipError_t e = hipEventSynchronize(e1);
+      if (e != hipSuccess) {
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        return fail(c, LQ_ERR_HIP, std::string("lq_autotune: ") + hipGetErrorString(e));
+      }
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, e0, e1);
+      const double per = double(ms) / reps;
+      if (per < best) { best = per; bs = sh; bt = tz; }
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  HIPCHECK(c, hipMemcpyAsync(A.cvar, T.cvar, n * 16, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(A.cf, T.cf, n * 16, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  c->par.interaction_variant = save_var;
+  c->tile_shape = bs;
+  c->tile_tz = bt;
+  c->layout_version++;
+  if (shape_out) *shape_out = bs;
+  if (tz_out) *tz_out = bt;
+  if (ms_out) *ms_out = best;
   return LQ_OK;
 }
 

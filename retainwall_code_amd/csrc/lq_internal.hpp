@@ -1,7 +1,9 @@
 // lq_internal.hpp -- device-side data model of the gfx950 SPH step.
 //
 // Layout in HBM (structure of arrays, one row per sorted particle slot):
-//   float4 pos      relative x y z (to the 2h cell), p
+//   float4 pos      relative x y z (to the 2h cell), massflag (mass of a soil
+//                   particle; negative for contact-wall codes)
+//   float  pres     the .dat "p" column (carried for output only)
 //   float4 vel      vx vy vz rho
 //   float4 idm      id mass idx M_crt
 //   int4   codes    block id, Fix/NonFix, model, motion id
@@ -64,6 +66,7 @@ enum { DERR_NONE = 0, DERR_DELETE = 1, DERR_DOMAIN = 2 };
 struct Arrays {
   int4 *codes, *cellpos;
   int* orig_id;
+  float* pres;
   float4 *pos, *pos_ori, *vel, *idm, *normal;
   float4 *str_a, *stn_a;
   float2 *str_b, *stn_b;

@@ -5,6 +5,7 @@
 namespace lq {
 
 void launch_abs2rel(hipStream_t s, const DevParams* P, int np, Arrays A, float scale);
+void launch_set_massflag(hipStream_t s, const DevParams* P, int np, Arrays A);
 void launch_rel2abs(hipStream_t s, const DevParams* P, int np, Arrays A, float4* out);
 void launch_step_calc(hipStream_t s, DevParams* P);
 void launch_advance(hipStream_t s, DevParams* P);
@@ -26,6 +27,11 @@ void launch_neighbor_list(hipStream_t s, const DevParams* P, int npm_max, Arrays
                           int max_nb, int* counts, int* ids);
 
 void launch_stream_copy(hipStream_t s, const float4* a, float4* b, size_t n);
+int interaction_tile_shapes();
+int interaction_tile_maxtz(int shape);
+int interaction_tile_waves(int shape);
+void launch_interaction_tile(hipStream_t s, DevParams* P, int cnx, int cny, int cnz, Arrays A, const Buffers& B,
+                             const int* cb, const int* ce, int shape, int tz);
 void launch_fill_int(hipStream_t s, int* a, int n, int v);
 int radix_tiles(int n);
 void launch_neighbor_search(hipStream_t s, DevParams* P, int np, int nbits, int ncell, Arrays A, Buffers& B,

@@ -4,55 +4,9 @@
 // out (fmaf) so that each operation sequence is fixed and reproducible.
 #include "lq_internal.hpp"
 #include "lq_launch.hpp"
+#include "lq_device.hpp"
 
 namespace lq {
-
-__device__ __forceinline__ float3 ld3(const float4 v) { return make_float3(v.x, v.y, v.z); }
-__device__ __forceinline__ float3 f3add(float3 a, float3 b) { return make_float3(a.x + b.x, a.y + b.y, a.z + b.z); }
-__device__ __forceinline__ float3 f3sub(float3 a, float3 b) { return make_float3(a.x - b.x, a.y - b.y, a.z - b.z); }
-__device__ __forceinline__ float3 f3mul(float3 a, float s) { return make_float3(a.x * s, a.y * s, a.z * s); }
-__device__ __forceinline__ float3 f3div(float3 a, float s) { return make_float3(a.x / s, a.y / s, a.z / s); }
-__device__ __forceinline__ float f3dot(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ float f3norm(float3 a) { return sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); }
-__device__ __forceinline__ float3 f3cross(float3 a, float3 b) {  // functions.h:100
-  return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
-}
-
-struct Sym { float xx, yy, zz, xy, xz, yz; };
-__device__ __forceinline__ Sym ldsym(const float4* a, const float2* b, int i) {
-  float4 u = a[i]; float2 v = b[i];
-  return {u.x, u.y, u.z, u.w, v.x, v.y};
-}
-__device__ __forceinline__ void stsym(float4* a, float2* b, int i, Sym s) {
-  a[i] = make_float4(s.xx, s.yy, s.zz, s.xy);
-  b[i] = make_float2(s.xz, s.yz);
-}
-__device__ __forceinline__ Sym sadd(Sym a, Sym b) {
-  return {a.xx + b.xx, a.yy + b.yy, a.zz + b.zz, a.xy + b.xy, a.xz + b.xz, a.yz + b.yz};
-}
-__device__ __forceinline__ Sym smul(Sym a, float s) {
-  return {a.xx * s, a.yy * s, a.zz * s, a.xy * s, a.xz * s, a.yz * s};
-}
-__device__ __forceinline__ Sym sdiv(Sym a, float s) {
-  return {a.xx / s, a.yy / s, a.zz / s, a.xy / s, a.xz / s, a.yz / s};
-}
-__device__ __forceinline__ float strace(Sym a) { return a.xx + a.yy + a.zz; }
-__device__ __forceinline__ float sddot(Sym a, Sym b) {  // functions.h:95
-  return a.xx * b.xx + a.yy * b.yy + a.zz * b.zz + 2.0f * a.xy * b.xy + 2.0f * a.xz * b.xz + 2.0f * a.yz * b.yz;
-}
-__device__ __forceinline__ float3 smatvec(Sym s, float3 v) {  // functions.h:97
-  return make_float3(s.xx * v.x + s.xy * v.y + s.xz * v.z, s.xy * v.x + s.yy * v.y + s.yz * v.z,
-                     s.xz * v.x + s.yz * v.y + s.zz * v.z);
-}
-
-// Candidate geometry, particle_interaction.cu:30-33:
-// d = pos_i - pos_j - off*2h, r = |d| (r^2 = fma(dz,dz,fma(dy,dy,dx*dx))).
-__device__ __forceinline__ float pair_r(float3 pi, float4 pj, int ox, int oy, int oz, float two_h, float3& d) {
-  d.x = (pi.x - pj.x) - float(ox) * two_h;
-  d.y = (pi.y - pj.y) - float(oy) * two_h;
-  d.z = (pi.z - pj.z) - float(oz) * two_h;
-  return sqrtf(fmaf(d.z, d.z, fmaf(d.y, d.y, d.x * d.x)));
-}
 
 // ------------------------------------------------------------ frames ----
 // cuPositionFromAbsolute2Relative, loquat_grid.cu:253-261.
@@ -65,13 +19,27 @@ __global__ void k_abs2rel(const DevParams* __restrict__ P, Arrays A, float scale
   A.pos[i] = make_float4(p.x - float(cx) * P->two_h, p.y - float(cy) * P->two_h, p.z - float(cz) * P->two_h, p.w);
 }
 
+// pos.w carries the neighbour data the interaction needs with the position:
+// the mass of a soil particle (idmassidx.y) and, in the sign, whether the
+// particle's code marks a contact wall (codes.x >= 10).
+__global__ void k_set_massflag(const DevParams* __restrict__ P, Arrays A) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P->np) return;
+  float4 p = A.pos[i];
+  const float4 id = A.idm[i];
+  const float m = id.x >= 100.0f ? id.y : 1.0f;
+  p.w = A.codes[i].x >= 10 ? -m : m;
+  A.pos[i] = p;
+}
+
 // cuPositionFromRelative2Absolute, loquat_grid.cu:273-279.
 __global__ void k_rel2abs(const DevParams* __restrict__ P, Arrays A, float4* __restrict__ out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P->np) return;
   float4 p = A.pos[i];
   int4 c = A.cellpos[i];
-  out[i] = make_float4(p.x + float(c.x) * P->two_h, p.y + float(c.y) * P->two_h, p.z + float(c.z) * P->two_h, p.w);
+  out[i] = make_float4(p.x + float(c.x) * P->two_h, p.y + float(c.y) * P->two_h, p.z + float(c.z) * P->two_h,
+                       A.pres[i]);
 }
 
 // ------------------------------------------------------ step scalars ----
@@ -305,104 +273,6 @@ __global__ void k_copy_soil_str(const DevParams* __restrict__ P, Arrays A, const
   A.str_b[i] = ob[i];
 }
 
-// ------------------------------------------------------- interaction ----
-// Per-pair terms of cuParticleInteractionMaterialLoopSolution,
-// particle_interaction.cu:26-115, accumulated into the per-particle sums.
-struct IAcc {
-  float accden;
-  float3 as, av, aa;
-  float gv[9];  // xx xy xz yx yy yz zx zy zz
-};
-
-__device__ __forceinline__ void pair_terms(const DevParams* __restrict__ P, IAcc& S, float3 d, float r, float q,
-                                           float3 vi, float rhoi, Sym sri, float artpi, int shift, bool bound,
-                                           float4 vj, float mj_soil, Sym sj, int codej) {
-  float ker = kW(P->knorm, q);
-  float dwdr = kG(P->kgrad, q);
-  float3 g = f3mul(d, dwdr);
-  float rhoj = vj.w;
-  float mj = bound ? P->vol_def * rhoj : mj_soil;
-  float3 dv = f3sub(vi, ld3(vj));
-  if (codej >= 10 && shift == 1) dv = make_float3(0.f, 0.f, 0.f);
-  float volj = mj / rhoj;
-  float3 dvv = f3mul(dv, volj);
-  S.gv[0] = fmaf(-dvv.x, g.x, S.gv[0]); S.gv[1] = fmaf(-dvv.x, g.y, S.gv[1]); S.gv[2] = fmaf(-dvv.x, g.z, S.gv[2]);
-  S.gv[3] = fmaf(-dvv.y, g.x, S.gv[3]); S.gv[4] = fmaf(-dvv.y, g.y, S.gv[4]); S.gv[5] = fmaf(-dvv.y, g.z, S.gv[5]);
-  S.gv[6] = fmaf(-dvv.z, g.x, S.gv[6]); S.gv[7] = fmaf(-dvv.z, g.y, S.gv[7]); S.gv[8] = fmaf(-dvv.z, g.z, S.gv[8]);
-  if (codej >= 10) return;
-  float dvg = fmaf(dv.z, g.z, fmaf(dv.y, g.y, dv.x * g.x));
-  S.accden = fmaf(rhoi * dvg, volj, S.accden);
-  float rhoj2 = rhoj * rhoj;
-  Sym srj = sdiv(sj, rhoj2);
-  Sym st = sadd(sri, srj);
-  float3 f = smatvec(st, g);
-  S.as.x = fmaf(f.x, mj, S.as.x); S.as.y = fmaf(f.y, mj, S.as.y); S.as.z = fmaf(f.z, mj, S.as.z);
-  if (P->viscosity_type == 1) {
-    float visc = fmaf(dv.z, d.z, fmaf(dv.y, d.y, dv.x * d.x));
-    if (visc < 0.f) {
-      float phi = P->h * visc / fmaf(r, r, P->eta);
-      float artv = P->alpha_cs * phi / (0.5f * (rhoi + rhoj));
-      float s = mj * artv;
-      S.av.x = fmaf(g.x, s, S.av.x); S.av.y = fmaf(g.y, s, S.av.y); S.av.z = fmaf(g.z, s, S.av.z);
-    }
-  }
-  if (P->art_p != 0.f) {
-    float trj = strace(srj);
-    float artp = artpi + (trj > 0.f ? trj * P->art_p : 0.f);
-    if (artp > 0.f) {
-      artp = artp * powf(ker / P->kernel_dr, P->art_exp);
-      float s = mj * artp;
-      S.aa.x = fmaf(-g.x, s, S.aa.x); S.aa.y = fmaf(-g.y, s, S.aa.y); S.aa.z = fmaf(-g.z, s, S.aa.z);
-    }
-  }
-}
-
-// Epilogue of cuParticleInteractionMaterial (particle_interaction.cu:284-295)
-// with pass A of the wall contact (LoopContact, :174-237).
-__device__ __forceinline__ void interaction_epilogue(DevParams* __restrict__ P, Arrays A, const Buffers& B, int i,
-                                                     const IAcc& S, float3 vi) {
-  float3 damp = f3mul(vi, P->damping);
-  float3 s3 = f3add(f3add(S.as, S.av), S.aa);
-  float mi = A.idm[i].y;
-  const float3 g = make_float3(P->gx, P->gy, P->gz);
-  float3 acc0 = f3add(f3sub(s3, damp), g);
-  const float dt = P->dt;
-  float3 vp = make_float3(fmaf(acc0.x, dt, vi.x), fmaf(acc0.y, dt, vi.y), fmaf(acc0.z, dt, vi.z));
-  float4 cvi = A.cvar[i];
-  float3 nb = ld3(A.cn[i]);
-  float3 vb = ld3(A.cw[i]);
-  float3 dvc = f3sub(vb, vp);
-  float3 ast = make_float3(0.f, 0.f, 0.f);
-  float dvn = f3dot(dvc, nb);
-  bool deferred = false;
-  if (cvi.y <= P->h * 2.0f && dvn > 0.f) {
-    float3 dvn3 = f3mul(nb, dvn);
-    float3 fn = f3div(f3mul(dvn3, mi), dt);
-    float3 ft = f3div(f3mul(f3sub(dvc, dvn3), mi), dt);
-    float afn = f3norm(fn), aft = f3norm(ft);
-    if (aft > P->mu * afn) { ft = f3mul(ft, P->mu * afn / aft); cvi.z = 1.f; }
-    else cvi.z = 0.f;
-    A.cvar[i] = cvi;
-    float3 fr = f3add(ft, fn);
-    if (cvi.y > P->dr) {
-      deferred = true;
-    } else {
-      A.cf[i] = make_float4(fr.x, fr.y, fr.z, 0.f);
-      ast = f3mul(fr, 1.0f / mi);
-    }
-  }
-  if (deferred) {
-    int k = atomicAdd(&P->n_deferred, 1);
-    B.deferred[k] = i;
-    B.s3[i] = make_float4(s3.x, s3.y, s3.z, mi);
-  }
-  float3 acc = f3add(f3sub(f3add(s3, ast), damp), g);
-  B.accad[i] = make_float4(acc.x, acc.y, acc.z, S.accden);
-  B.acestn_a[i] = make_float4(S.gv[0], S.gv[4], S.gv[8], 0.5f * (S.gv[1] + S.gv[3]));
-  B.acestn_b[i] = make_float2(0.5f * (S.gv[2] + S.gv[6]), 0.5f * (S.gv[5] + S.gv[7]));
-  B.spin[i] = make_float4(0.5f * (S.gv[3] - S.gv[1]), 0.5f * (S.gv[6] - S.gv[2]), 0.5f * (S.gv[7] - S.gv[5]), 0.f);
-}
-
 // Reference-order variant: each thread walks its 27 cells (cz, cy, cx;
 // boundary range then soil range) exactly as particle_interaction.cu:117-140.
 __global__ void __launch_bounds__(256) k_interaction_ref(DevParams* __restrict__ P, Arrays A, Buffers B,
@@ -437,11 +307,12 @@ __global__ void __launch_bounds__(256) k_interaction_ref(DevParams* __restrict__
           int c = bound ? cb : cb + P->ncell_material;
           for (int j = cbeg[c], e = cend[c]; j < e; j++) {
             float3 d;
-            float r = pair_r(pi, A.pos[j], cx, cy, cz, two_h, d);
+            const float4 pj = A.pos[j];
+            float r = pair_r(pi, pj, cx, cy, cz, two_h, d);
             float q = r * i_h;
             if (!(q < 2.0f)) continue;
-            pair_terms(P, S, d, r, q, vi, rhoi, sri, artpi, shift, bound, A.vel[j], A.idm[j].y,
-                       ldsym(A.str_a, A.str_b, j), A.codes[j].x);
+            pair_terms(P, S, d, r, q, vi, rhoi, sri, artpi, shift, bound, A.vel[j], pj.w,
+                       ldsym(A.str_a, A.str_b, j));
           }
         }
       }
@@ -532,10 +403,11 @@ __global__ void __launch_bounds__(NL_WAVE * NL_WAVES) k_interaction_nl(DevParams
         bool bound = (cur & 1) == 0;
         int cz = cell / 9 - 1, cy = (cell / 3) % 3 - 1, cx = cell % 3 - 1;
         float3 d;
-        float r = pair_r(pi, A.pos[jj], cx, cy, cz, two_h, d);
+        const float4 pj = A.pos[jj];
+        float r = pair_r(pi, pj, cx, cy, cz, two_h, d);
         float q = r * i_h;
-        pair_terms(P, S, d, r, q, vi, rhoi, sri, artpi, shift, bound, A.vel[jj], A.idm[jj].y,
-                   ldsym(A.str_a, A.str_b, jj), A.codes[jj].x);
+        pair_terms(P, S, d, r, q, vi, rhoi, sri, artpi, shift, bound, A.vel[jj], pj.w,
+                   ldsym(A.str_a, A.str_b, jj));
       }
     }
     bool more = (j < jend) || (cstep < 54);
@@ -608,63 +480,6 @@ __global__ void k_contact_commit(const DevParams* __restrict__ P, Arrays A, Buff
     cv.x = f.w;
     A.cvar[i] = cv;
   }
-}
-
-// ------------------------------------------------------------ HypoCS ----
-// cuMatModHypo (critical state), constitutive_model.h:209-299.
-__device__ __forceinline__ void hypo_cs(const DevParams* __restrict__ P, int model, float dt, float rho, float3 w,
-                                        Sym D, Sym& sig, float& Mcrt) {
-  const float* mp = P->model_par[model];
-  const HypoConst& hc = P->hypo[model];
-  const float coh = mp[3], pref = mp[4], ec0 = mp[5], lam = mp[6], xi = mp[7], alpha = mp[8];
-  const float emin = mp[9], emax = mp[10], e0 = mp[11], rho0 = mp[12];
-  Sym s = sig;
-  if (coh > 0.f) { s.xx = s.xx - coh; s.yy = s.yy - coh; s.zz = s.zz - coh; }
-  const float Sxx = s.xx, Syy = s.yy, Szz = s.zz, Sxy = s.xy, Sxz = s.xz, Syz = s.yz;
-  // W S - S W with W = SkewTensor2FullTensor(w), symmetric part
-  float WSxx = -w.z * Sxy + w.y * Sxz, WSxy = -w.z * Syy + w.y * Syz, WSxz = -w.z * Syz + w.y * Szz;
-  float WSyx = w.z * Sxx - w.x * Sxz, WSyy = w.z * Sxy - w.x * Syz, WSyz = w.z * Sxz - w.x * Szz;
-  float WSzx = -w.y * Sxx + w.x * Sxy, WSzy = -w.y * Sxy + w.x * Syy, WSzz = -w.y * Sxz + w.x * Syz;
-  float SWxx = Sxy * w.z - Sxz * w.y, SWxy = -Sxx * w.z + Sxz * w.x, SWxz = Sxx * w.y - Sxy * w.x;
-  float SWyx = Syy * w.z - Syz * w.y, SWyy = -Sxy * w.z + Syz * w.x, SWyz = Sxy * w.y - Syy * w.x;
-  float SWzx = Syz * w.z - Szz * w.y, SWzy = -Sxz * w.z + Szz * w.x, SWzz = Sxz * w.y - Syz * w.x;
-  Sym jau = {WSxx - SWxx, WSyy - SWyy, WSzz - SWzz, 0.5f * ((WSxy - SWxy) + (WSyx - SWyx)),
-             0.5f * ((WSxz - SWxz) + (WSzx - SWzx)), 0.5f * ((WSyz - SWyz) + (WSzy - SWzy))};
-  float I1 = strace(s);
-  if (I1 > -4.0f) { sig = Sym{-1.f, -1.f, -2.f, 0.f, 0.f, 0.f}; return; }
-  Sym dev = s;
-  float third = I1 / 3.0f;
-  dev.xx = dev.xx - third; dev.yy = dev.yy - third; dev.zz = dev.zz - third;
-  float trs = strace(s), trd = strace(D), trsd = sddot(s, D), nd = sqrtf(sddot(D, D));
-  float p = -I1 / 3.0f;
-  float e = (1.0f + e0) * rho0 / rho - 1.0f;
-  e = fmaxf(e, emin);
-  e = fminf(e, emax);
-  float ec = ec0 * expf(-lam * powf(p / pref, xi));
-  float fd = powf((e - emin) / (ec - emin), alpha);
-  Mcrt = e - ec;
-  Sym sd = sadd(s, dev);
-  Sym rate = sadd(sadd(sadd(sadd(jau, smul(D, hc.C1 * trs)), smul(s, hc.C2 * trd)), smul(s, hc.C3 * trsd / trs)),
-                  smul(sd, fd * hc.C4 * nd));
-  Sym ns = sadd(sig, smul(rate, dt));
-  float sph = strace(ns) / 3.0f;
-  if (sph > 0.f) { sig = Sym{-1.0f + coh, -1.0f + coh, -2.0f + coh, 0.f, 0.f, 0.f}; return; }
-  Sym nd6 = ns;
-  nd6.xx = nd6.xx - sph; nd6.yy = nd6.yy - sph; nd6.zz = nd6.zz - sph;
-  const float fa = hc.fa, fb = hc.fb, fc = hc.fc;
-  float u = 1.0f + fa + fb / 3.0f;
-  float f2 = fd * fd * fc * fc;
-  float A_ = f2 / 3.0f - u * u;
-  float B_ = 2.0f * f2 * (u * u + 2.0f * fa + 3.0f) - 4.0f * fb * u;
-  float C_ = 4.0f / 3.0f * f2 * fb * fb - 4.0f * fb * fb;
-  float k2 = (-B_ - sqrtf(B_ * B_ - 4.0f * A_ * C_)) / (2.0f * A_);
-  float kphi = 3.0f / sqrtf(k2);
-  float tau = sqrtf(0.5f * sddot(nd6, nd6));
-  if (tau - (-sph * kphi) > 0.f) {
-    float ratio = tau < 1e-10f ? 1.0f : (-kphi * sph) / tau;
-    nd6 = smul(nd6, ratio);
-  }
-  sig = Sym{nd6.xx + sph + coh, nd6.yy + sph + coh, nd6.zz + sph + coh, nd6.xy, nd6.xz, nd6.yz};
 }
 
 // --------------------------------------------------------- predictor ----
@@ -866,6 +681,9 @@ static inline int blocks(long long n, int b) { return int((n + b - 1) / b); }
 void launch_abs2rel(hipStream_t s, const DevParams* P, int np, Arrays A, float scale) {
   if (np > 0) hipLaunchKernelGGL(k_abs2rel, dim3(blocks(np, 256)), dim3(256), 0, s, P, A, scale);
 }
+void launch_set_massflag(hipStream_t s, const DevParams* P, int np, Arrays A) {
+  if (np > 0) hipLaunchKernelGGL(k_set_massflag, dim3(blocks(np, 256)), dim3(256), 0, s, P, A);
+}
 void launch_rel2abs(hipStream_t s, const DevParams* P, int np, Arrays A, float4* out) {
   if (np > 0) hipLaunchKernelGGL(k_rel2abs, dim3(blocks(np, 256)), dim3(256), 0, s, P, A, out);
 }
@@ -895,7 +713,7 @@ void launch_interaction(hipStream_t s, DevParams* P, int npm_max, Arrays A, cons
   if (npm_max <= 0) return;
   if (variant == LQ_INTERACTION_REFERENCE_ORDER)
     hipLaunchKernelGGL(k_interaction_ref, dim3(blocks(npm_max, 256)), dim3(256), 0, s, P, A, B, cb, ce);
-  else
+  else if (variant == LQ_INTERACTION_NEIGHBOR_LIST)
     hipLaunchKernelGGL(k_interaction_nl, dim3(blocks(npm_max, NL_WAVE * NL_WAVES)), dim3(NL_WAVE * NL_WAVES), 0, s,
                        P, A, B, cb, ce);
 }

@@ -175,6 +175,14 @@ class Context:
         self._chk(self.lib.lq_kernel_times(self.h, ms, cnt, n), "lq_kernel_times")
         return {k: (ms[i], cnt[i]) for i, k in enumerate(_abi.KERNEL_CLASSES)}
 
+    def set_tile(self, shape, tz):
+        self._chk(self.lib.lq_set_tile(self.h, shape, tz), "lq_set_tile")
+
+    def autotune(self, reps=3):
+        sh, tz, ms = C.c_int(), C.c_int(), C.c_double()
+        self._chk(self.lib.lq_autotune(self.h, reps, C.byref(sh), C.byref(tz), C.byref(ms)), "lq_autotune")
+        return sh.value, tz.value, ms.value
+
     def neighbor_lists(self, max_nb=160):
         npm = self.counts()[1]
         counts = np.zeros(npm, np.int32)
@@ -184,7 +192,7 @@ class Context:
         return counts, ids
 
 
-def open_case(case, key_mode=_abi.KEY_LINEAR, variant=_abi.INTERACTION_NEIGHBOR_LIST, device=0, initial_sort=True):
+def open_case(case, key_mode=_abi.KEY_LINEAR, variant=_abi.INTERACTION_TILE, device=0, initial_sort=True):
     """Create a context, upload a case and run the initial search/sort
     (loquat.cu:528-531: GridConfig, Abs2Rel, NeighborSearch, SortArraysPrimary)."""
     from . import cases

@@ -113,7 +113,7 @@ def _cmp(tag, g, o32, o64, keys, rows, state_keys=()):
     return out
 
 
-@pytest.mark.parametrize("variant", [_abi.INTERACTION_NEIGHBOR_LIST, _abi.INTERACTION_REFERENCE_ORDER])
+@pytest.mark.parametrize("variant", [_abi.INTERACTION_TILE, _abi.INTERACTION_REFERENCE_ORDER])
 def test_step_pieces(variant, oracle_mod):
     case = cases.config("small")
     say(f"[pieces] case=small variant={variant}")
@@ -154,18 +154,48 @@ def test_step_pieces(variant, oracle_mod):
     ctx.close()
 
 
+# (variant, tile shape, tile depth): shape 0 at depth 6 overflows its LDS
+# capacity on most tiles and exercises the global-memory fallback.
+VARIANTS = [(_abi.INTERACTION_REFERENCE_ORDER, None, None), (_abi.INTERACTION_NEIGHBOR_LIST, None, None),
+            (_abi.INTERACTION_TILE, 0, 3), (_abi.INTERACTION_TILE, 0, 6), (_abi.INTERACTION_TILE, 1, 7),
+            (_abi.INTERACTION_TILE, 2, 13), (_abi.INTERACTION_TILE, 2, 1)]
+
+
 def test_interaction_variants_identical():
     case = cases.config("small")
-    say("[variants] case=small")
     outs = []
-    for v in (_abi.INTERACTION_NEIGHBOR_LIST, _abi.INTERACTION_REFERENCE_ORDER):
+    for v, shape, tz in VARIANTS:
+        say(f"[variants] case=small variant={v} shape={shape} tz={tz}")
         ctx = sim.open_case(case, variant=v)
+        if shape is not None:
+            ctx.set_tile(shape, tz)
         ctx.step(3)
         ctx.sync()
-        outs.append(ctx.snapshot(["accad", "acestn", "spin", "vel", "str", "cmag"]))
+        outs.append(ctx.snapshot(["accad", "acestn", "spin", "vel", "str", "cmag", "cvar"]))
         ctx.close()
-    for k in outs[0]:
-        np.testing.assert_array_equal(bits(outs[0][k]), bits(outs[1][k]), err_msg=k)
+    for o in outs[1:]:
+        for k in outs[0]:
+            np.testing.assert_array_equal(bits(outs[0][k]), bits(o[k]), err_msg=k)
+
+
+def test_autotune_keeps_state():
+    case = cases.config("small")
+    say("[autotune] case=small")
+    a = sim.open_case(case)
+    b = sim.open_case(case)
+    a.step(2)
+    b.step(2)
+    shape, tz, ms = b.autotune(reps=2)
+    say(f"  autotune -> shape={shape} tz={tz} ms={ms:.4f}")
+    assert 0 <= shape <= 2 and tz >= 1 and ms > 0
+    a.step(2)
+    b.step(2)
+    a.sync()
+    b.sync()
+    for k in ["vel", "str", "cmag", "cvar", "pos"]:
+        np.testing.assert_array_equal(bits(a.get(k)), bits(b.get(k)), err_msg=k)
+    a.close()
+    b.close()
 
 
 @pytest.mark.parametrize("name", ["tiny", "small"])
