@@ -536,7 +536,7 @@ extern "C" int lq_get_field(lq_ctx* c, int f, void* dst, size_t bytes) {
     HIPCHECK(c, hipStreamSynchronize(c->stream));
   } else if (v.width == 4) {
     HIPCHECK(c, hipMemcpyAsync(dst, f4_ptr(c, f), n * 16, hipMemcpyDeviceToHost, c->stream));
-    if (f == LQ_F_POS) {  // w on the device is the massflag; the host sees p
+    if (f == LQ_F_POS || f == LQ_F_POS_INI) {  // w on the device is the massflag; the host sees p
       std::vector<float> pr;
       if ((st = d2h(c, pr, c->B.cur.pres, n))) return st;
       HIPCHECK(c, hipStreamSynchronize(c->stream));
@@ -617,6 +617,8 @@ extern "C" int lq_set_field(lq_ctx* c, int f, const void* src, size_t bytes) {
     HIPCHECK(c, hipStreamSynchronize(c->stream));
   }
   if (f == LQ_F_POS || f == LQ_F_IDM || f == LQ_F_CODES) launch_set_massflag(c->stream, c->dP, c->par.np, c->B.cur);
+  if (f == LQ_F_POS_INI || f == LQ_F_POS || f == LQ_F_IDM || f == LQ_F_CODES)
+    launch_copy_w(c->stream, c->dP, c->par.np, c->B.cur.pos, c->B.pos_ini);
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   return LQ_OK;
 }

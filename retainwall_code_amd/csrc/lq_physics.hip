@@ -32,6 +32,14 @@ __global__ void k_set_massflag(const DevParams* __restrict__ P, Arrays A) {
   A.pos[i] = p;
 }
 
+__global__ void k_copy_w(const DevParams* __restrict__ P, const float4* __restrict__ src, float4* __restrict__ dst) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P->np) return;
+  float4 d = dst[i];
+  d.w = src[i].w;
+  dst[i] = d;
+}
+
 // cuPositionFromRelative2Absolute, loquat_grid.cu:273-279.
 __global__ void k_rel2abs(const DevParams* __restrict__ P, Arrays A, float4* __restrict__ out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -683,6 +691,9 @@ void launch_abs2rel(hipStream_t s, const DevParams* P, int np, Arrays A, float s
 }
 void launch_set_massflag(hipStream_t s, const DevParams* P, int np, Arrays A) {
   if (np > 0) hipLaunchKernelGGL(k_set_massflag, dim3(blocks(np, 256)), dim3(256), 0, s, P, A);
+}
+void launch_copy_w(hipStream_t s, const DevParams* P, int np, const float4* src, float4* dst) {
+  if (np > 0) hipLaunchKernelGGL(k_copy_w, dim3(blocks(np, 256)), dim3(256), 0, s, P, src, dst);
 }
 void launch_rel2abs(hipStream_t s, const DevParams* P, int np, Arrays A, float4* out) {
   if (np > 0) hipLaunchKernelGGL(k_rel2abs, dim3(blocks(np, 256)), dim3(256), 0, s, P, A, out);
