@@ -23,6 +23,120 @@
 
 namespace lq {
 
+// Scan + evaluate for the tile's home particles.  STAGED reads candidate
+// positions from the LDS copy; otherwise from global memory (tile overflow).
+template <bool STAGED, int WAVES, int NLCAP, int NZW>
+__device__ __forceinline__ void tile_home(DevParams* __restrict__ P, const Arrays& A, const Buffers& B,
+                                          const int* __restrict__ cbeg, const int* __restrict__ cend,
+                                          const float4* __restrict__ s_pos, const int (*s_lo)[NZW],
+                                          const int (*s_hi)[NZW], const int* s_rs, const int* s_ro,
+                                          unsigned (*s_list)[NLCAP][64], int hb, int he, int zw0, int nzw) {
+  constexpr int WG = 64 * WAVES;
+  const int t = threadIdx.x;
+  const int lane = t & 63, wv = t >> 6;
+  const int ncm = P->ncell_material;
+  const float two_h = P->two_h, i_h = P->i_h;
+  const int H = he - hb;
+  for (int base = 0; base < H; base += WG) {
+    const int li = base + t;
+    const bool active = li < H;
+    const int i = hb + (active ? li : 0);
+    const float3 pi = ld3(A.pos[i]);
+    const int4 ci = A.cellpos[i];
+    const float4 vri = A.vel[i];
+    const float3 vi = ld3(vri);
+    const float rhoi = vri.w;
+    const float rhoi2 = rhoi * rhoi;
+    const Sym sri = sdiv(ldsym(A.str_a, A.str_b, i), rhoi2);
+    const float tri = strace(sri);
+    const float artpi = tri > 0.f ? tri * P->art_p : 0.f;
+    const int shift = int(A.cvar[i].z);
+    const int zoff = ci.z - zw0;
+    IAcc S;
+    S.accden = 0.f;
+    S.as = S.av = S.aa = make_float3(0.f, 0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < 9; k++) S.gv[k] = 0.f;
+    int cstep = active ? 0 : 54;
+    int j = 0, jend = 0;
+    int n = 0;
+    for (;;) {
+      while (n < NLCAP) {
+        if (j >= jend) {
+          bool found = false;
+          while (cstep < 54) {
+            const int cur = cstep++;
+            const int cell = cur >> 1, bsel = cur & 1;
+            const int cz = cell / 9 - 1, cy = (cell / 3) % 3 - 1, cx = cell % 3 - 1;
+            if constexpr (STAGED) {
+              const int zi = zoff + cz;
+              if (zi < 0 || zi >= nzw) continue;
+              const int r = ((cy + 1) * 3 + (cx + 1)) * 2 + bsel;
+              const int lo = s_lo[r][zi], hi = s_hi[r][zi];
+              if (lo >= hi) continue;
+              j = lo;
+              jend = hi;
+            } else {
+              const int jx = ci.x + cx, jy = ci.y + cy, jz = ci.z + cz;
+              if (!in_grid(P, jx, jy, jz)) continue;
+              const int c = lin_key(P, jx, jy, jz) + (bsel ? ncm : 0);
+              const int bb = cbeg[c];
+              if (bb < 0) continue;
+              j = bb;
+              jend = cend[c];
+            }
+            found = true;
+            break;
+          }
+          if (!found) break;
+        }
+        const int cur = cstep - 1;
+        const int cell = cur >> 1;
+        const int cz = cell / 9 - 1, cy = (cell / 3) % 3 - 1, cx = cell % 3 - 1;
+        float4 pj;
+        if constexpr (STAGED) pj = s_pos[j];
+        else pj = A.pos[j];
+        float3 d;
+        const float r = pair_r(pi, pj, cx, cy, cz, two_h, d);
+        if (r * i_h < 2.0f) {
+          s_list[wv][n][lane] = (unsigned(j) << 6) | unsigned(cur);
+          n++;
+        }
+        j++;
+      }
+      int nmax = n;
+      for (int o = 32; o > 0; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o));
+      for (int k = 0; k < nmax; k++) {
+        if (k < n) {
+          const unsigned e = s_list[wv][k][lane];
+          const int jj = int(e >> 6), cur = int(e & 63u);
+          const int cell = cur >> 1, bsel = cur & 1;
+          const int cz = cell / 9 - 1, cy = (cell / 3) % 3 - 1, cx = cell % 3 - 1;
+          int slot;
+          float4 pj;
+          if constexpr (STAGED) {
+            const int r = ((cy + 1) * 3 + (cx + 1)) * 2 + bsel;
+            pj = s_pos[jj];
+            slot = s_rs[r] + (jj - s_ro[r]);
+          } else {
+            pj = A.pos[jj];
+            slot = jj;
+          }
+          float3 d;
+          const float r = pair_r(pi, pj, cx, cy, cz, two_h, d);
+          const float q = r * i_h;
+          pair_terms(P, S, d, r, q, vi, rhoi, sri, artpi, shift, bsel == 0, A.vel[slot], pj.w,
+                     ldsym(A.str_a, A.str_b, slot));
+        }
+      }
+      const bool more = (j < jend) || (cstep < 54);
+      n = 0;
+      if (__ballot(more) == 0ull) break;
+    }
+    if (active) interaction_epilogue(P, A, B, i, S, vi);
+  }
+}
+
 template <int WAVES, int CAP, int MAXTZ, int NLCAP>
 __global__ void __launch_bounds__(64 * WAVES) k_interaction_tile(DevParams* __restrict__ P, Arrays A, Buffers B,
                                                                  const int* __restrict__ cbeg,
@@ -120,103 +234,10 @@ __global__ void __launch_bounds__(64 * WAVES) k_interaction_tile(DevParams* __re
   }
   __syncthreads();
 
-  const float two_h = P->two_h, i_h = P->i_h;
-  const int H = he - hb;
-  for (int base = 0; base < H; base += WG) {
-    const int li = base + t;
-    const bool active = li < H;
-    const int i = hb + (active ? li : 0);
-    const float3 pi = ld3(A.pos[i]);
-    const int4 ci = A.cellpos[i];
-    const float4 vri = A.vel[i];
-    const float3 vi = ld3(vri);
-    const float rhoi = vri.w;
-    const float rhoi2 = rhoi * rhoi;
-    const Sym sri = sdiv(ldsym(A.str_a, A.str_b, i), rhoi2);
-    const float tri = strace(sri);
-    const float artpi = tri > 0.f ? tri * P->art_p : 0.f;
-    const int shift = int(A.cvar[i].z);
-    const int zoff = ci.z - zw0;
-    IAcc S;
-    S.accden = 0.f;
-    S.as = S.av = S.aa = make_float3(0.f, 0.f, 0.f);
-#pragma unroll
-    for (int k = 0; k < 9; k++) S.gv[k] = 0.f;
-    int cstep = active ? 0 : 54;
-    int j = 0, jend = 0;
-    int n = 0;
-    for (;;) {
-      while (n < NLCAP) {
-        if (j >= jend) {
-          bool found = false;
-          while (cstep < 54) {
-            const int cur = cstep++;
-            const int cell = cur >> 1, bsel = cur & 1;
-            const int cz = cell / 9 - 1, cy = (cell / 3) % 3 - 1, cx = cell % 3 - 1;
-            if (staged) {
-              const int zi = zoff + cz;
-              if (zi < 0 || zi >= nzw) continue;
-              const int r = ((cy + 1) * 3 + (cx + 1)) * 2 + bsel;
-              const int lo = s_lo[r][zi], hi = s_hi[r][zi];
-              if (lo >= hi) continue;
-              j = lo;
-              jend = hi;
-            } else {
-              const int jx = ci.x + cx, jy = ci.y + cy, jz = ci.z + cz;
-              if (!in_grid(P, jx, jy, jz)) continue;
-              const int c = lin_key(P, jx, jy, jz) + (bsel ? ncm : 0);
-              const int bb = cbeg[c];
-              if (bb < 0) continue;
-              j = bb;
-              jend = cend[c];
-            }
-            found = true;
-            break;
-          }
-          if (!found) break;
-        }
-        const int cur = cstep - 1;
-        const int cell = cur >> 1;
-        const int cz = cell / 9 - 1, cy = (cell / 3) % 3 - 1, cx = cell % 3 - 1;
-        const float4 pj = staged ? s_pos[j] : A.pos[j];
-        float3 d;
-        const float r = pair_r(pi, pj, cx, cy, cz, two_h, d);
-        if (r * i_h < 2.0f) {
-          s_list[wv][n][lane] = (unsigned(j) << 6) | unsigned(cur);
-          n++;
-        }
-        j++;
-      }
-      int nmax = n;
-      for (int o = 32; o > 0; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o));
-      for (int k = 0; k < nmax; k++) {
-        if (k < n) {
-          const unsigned e = s_list[wv][k][lane];
-          const int jj = int(e >> 6), cur = int(e & 63u);
-          const int cell = cur >> 1, bsel = cur & 1;
-          const int cz = cell / 9 - 1, cy = (cell / 3) % 3 - 1, cx = cell % 3 - 1;
-          int slot = jj;
-          float4 pj;
-          if (staged) {
-            const int r = ((cy + 1) * 3 + (cx + 1)) * 2 + bsel;
-            pj = s_pos[jj];
-            slot = s_rs[r] + (jj - s_ro[r]);
-          } else {
-            pj = A.pos[jj];
-          }
-          float3 d;
-          const float r = pair_r(pi, pj, cx, cy, cz, two_h, d);
-          const float q = r * i_h;
-          pair_terms(P, S, d, r, q, vi, rhoi, sri, artpi, shift, bsel == 0, A.vel[slot], pj.w,
-                     ldsym(A.str_a, A.str_b, slot));
-        }
-      }
-      const bool more = (j < jend) || (cstep < 54);
-      n = 0;
-      if (__ballot(more) == 0ull) break;
-    }
-    if (active) interaction_epilogue(P, A, B, i, S, vi);
-  }
+  if (staged)
+    tile_home<true, WAVES, NLCAP, NZW>(P, A, B, cbeg, cend, s_pos, s_lo, s_hi, s_rs, s_ro, s_list, hb, he, zw0, nzw);
+  else
+    tile_home<false, WAVES, NLCAP, NZW>(P, A, B, cbeg, cend, s_pos, s_lo, s_hi, s_rs, s_ro, s_list, hb, he, zw0, nzw);
 }
 
 // Tile shapes the autotuner may choose from.
