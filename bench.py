@@ -79,6 +79,9 @@ def main():
     ap.add_argument("--case", default="c2")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--graph", action="store_true", help="time with hipGraph replay (no per-kernel events)")
+    ap.add_argument("--variant", default="tile", choices=["tile", "reference", "neighbor_list"],
+                    help="interaction kernel variant (all bit-identical)")
+    ap.add_argument("--tile", default="", metavar="SHAPE,TZ", help="fix the tile shape/depth instead of autotuning")
     ap.add_argument("--probe-copy", type=int, default=0, metavar="BYTES",
                     help="only run the STREAM copy probe over BYTES (HBM bandwidth / counter calibration)")
     args = ap.parse_args()
@@ -100,11 +103,20 @@ def main():
 
     case = cases.config(args.case)
     print(f"[bench] case={args.case} np={case.np} npb={case.npb} npm={case.npm}", file=sys.stderr, flush=True)
-    ctx = sim.open_case(case)
+    from retainwall_code_amd import _abi
+    variant = {"tile": _abi.INTERACTION_TILE, "reference": _abi.INTERACTION_REFERENCE_ORDER,
+               "neighbor_list": _abi.INTERACTION_NEIGHBOR_LIST}[args.variant]
+    ctx = sim.open_case(case, variant=variant)
     p = ctx.params
     ctx.step(args.warmup, graph=args.graph)
     ctx.sync()
-    tile_shape, tile_tz, tune_ms = ctx.autotune(reps=3)  # tile shape/depth for this case (untimed)
+    tile_shape = tile_tz = None
+    if args.variant == "tile":
+        if args.tile:
+            tile_shape, tile_tz = (int(v) for v in args.tile.split(","))
+            ctx.set_tile(tile_shape, tile_tz)
+        else:
+            tile_shape, tile_tz, _ = ctx.autotune(reps=3)  # tile shape/depth for this case (untimed)
     # interaction work per call (outside the timed region)
     soil = slice(case.npb, case.np)
     cellpos = ctx.get("cellpos")
@@ -137,7 +149,8 @@ def main():
             traffic = json.load(open(pmc)).get("hbm_bytes_per_launch")
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "k_interaction_tile", "avg_launch_ms": round(avg_ms, 4),
+                "kernel": {"tile": "k_interaction_tile", "reference": "k_interaction_ref",
+                           "neighbor_list": "k_interaction_nl"}[args.variant], "avg_launch_ms": round(avg_ms, 4),
                 "algorithmic_bytes_per_launch": bytes_call,
                 "valu_tflops": round(flops_call / (avg_ms * 1e-3) / 1e12, 2),
                 "valu_frac": round(flops_call / (avg_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
@@ -153,7 +166,7 @@ def main():
                                f"0.01 rad/s (active), jitter 0.05dx, void ratio U(0.65,0.75)",
                    "soil_particles": case.npm, "boundary_particles": case.npb, "cells": int(p.ncell),
                    "dt": float(p.dt) if p.dt else case.params["dt"], "graph": bool(args.graph),
-                   "interaction_variant": "tile", "tile_shape": tile_shape, "tile_tz": tile_tz,
+                   "interaction_variant": args.variant, "tile_shape": tile_shape, "tile_tz": tile_tz,
                    "key_mode": "linear"},
         "roofline": roof,
         "cpu_baseline": cpu,
