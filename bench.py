@@ -23,10 +23,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
 FP32_PEAK_TFLOPS = 157.3     # MI355X vector FP32
-# Compulsory bytes per interaction call: 168 B per soil particle (pos, vel,
-# rho, mass, stress, flags, contact state in; acceleration, drho/dt, D, spin,
-# contact force/slip/K out), 56 B per boundary particle, 8 B per cell.
-B_SOIL, B_BND, B_CELL = 168, 56, 8
+# Compulsory bytes per interaction call: 176 B per soil particle (pos, vel,
+# rho, sigma/rho^2 and m/rho, flags, contact state in; acceleration, drho/dt,
+# D, spin, contact force/slip/K out), 64 B per boundary particle (pos, vel,
+# sigma/rho^2 and m/rho), 8 B per cell (begin/end).
+B_SOIL, B_BND, B_CELL = 176, 64, 8
 # Flops of the interaction kernel counted from its source (FMA = 2):
 # 15 per candidate distance test, 110 per neighbour pair evaluation.
 FLOP_CAND, FLOP_PAIR = 15, 110
@@ -79,8 +80,9 @@ def main():
     ap.add_argument("--case", default="c2")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--graph", action="store_true", help="time with hipGraph replay (no per-kernel events)")
-    ap.add_argument("--variant", default="tile", choices=["tile", "reference", "neighbor_list"],
-                    help="interaction kernel variant (all bit-identical)")
+    ap.add_argument("--variant", default="auto", choices=["auto", "tile", "reference", "neighbor_list"],
+                    help="interaction kernel variant (all bit-identical); auto times each and keeps the fastest")
+    ap.add_argument("--remap", type=int, default=0, help="XCD block renumbering for reference/neighbor_list")
     ap.add_argument("--tile", default="", metavar="SHAPE,TZ", help="fix the tile shape/depth instead of autotuning")
     ap.add_argument("--probe-copy", type=int, default=0, metavar="BYTES",
                     help="only run the STREAM copy probe over BYTES (HBM bandwidth / counter calibration)")
@@ -104,19 +106,26 @@ def main():
     case = cases.config(args.case)
     print(f"[bench] case={args.case} np={case.np} npb={case.npb} npm={case.npm}", file=sys.stderr, flush=True)
     from retainwall_code_amd import _abi
-    variant = {"tile": _abi.INTERACTION_TILE, "reference": _abi.INTERACTION_REFERENCE_ORDER,
-               "neighbor_list": _abi.INTERACTION_NEIGHBOR_LIST}[args.variant]
+    names = {_abi.INTERACTION_TILE: "tile", _abi.INTERACTION_REFERENCE_ORDER: "reference",
+             _abi.INTERACTION_NEIGHBOR_LIST: "neighbor_list"}
+    ids = {v: k for k, v in names.items()}
+    variant = ids.get(args.variant, _abi.INTERACTION_REFERENCE_ORDER)
     ctx = sim.open_case(case, variant=variant)
     p = ctx.params
     ctx.step(args.warmup, graph=args.graph)
     ctx.sync()
     tile_shape = tile_tz = None
-    if args.variant == "tile":
-        if args.tile:
-            tile_shape, tile_tz = (int(v) for v in args.tile.split(","))
-            ctx.set_tile(tile_shape, tile_tz)
-        else:
-            tile_shape, tile_tz, _ = ctx.autotune(reps=3)  # tile shape/depth for this case (untimed)
+    remap = args.remap
+    if args.variant == "tile" and args.tile:
+        tile_shape, tile_tz = (int(v) for v in args.tile.split(","))
+        ctx.set_tile(tile_shape, tile_tz)
+    elif args.variant == "auto":
+        variant, tile_shape, tile_tz, remap, _ = ctx.autotune(reps=3)  # untimed: pick the fastest kernel
+    else:
+        ctx.set_interaction(variant, remap)
+    if variant != _abi.INTERACTION_TILE:
+        tile_shape = tile_tz = None
+    vname = names[variant]
     # interaction work per call (outside the timed region)
     soil = slice(case.npb, case.np)
     cellpos = ctx.get("cellpos")
@@ -150,7 +159,7 @@ def main():
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": {"tile": "k_interaction_tile", "reference": "k_interaction_ref",
-                           "neighbor_list": "k_interaction_nl"}[args.variant], "avg_launch_ms": round(avg_ms, 4),
+                           "neighbor_list": "k_interaction_nl"}[vname], "avg_launch_ms": round(avg_ms, 4),
                 "algorithmic_bytes_per_launch": bytes_call,
                 "valu_tflops": round(flops_call / (avg_ms * 1e-3) / 1e12, 2),
                 "valu_frac": round(flops_call / (avg_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
@@ -166,7 +175,8 @@ def main():
                                f"0.01 rad/s (active), jitter 0.05dx, void ratio U(0.65,0.75)",
                    "soil_particles": case.npm, "boundary_particles": case.npb, "cells": int(p.ncell),
                    "dt": float(p.dt) if p.dt else case.params["dt"], "graph": bool(args.graph),
-                   "interaction_variant": args.variant, "tile_shape": tile_shape, "tile_tz": tile_tz,
+                   "interaction_variant": vname, "tile_shape": tile_shape, "tile_tz": tile_tz,
+                   "xcd_remap": remap if variant != _abi.INTERACTION_TILE else None,
                    "key_mode": "linear"},
         "roofline": roof,
         "cpu_baseline": cpu,

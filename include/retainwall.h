@@ -48,13 +48,14 @@ enum lq_shape { LQ_CUBOID = 0, LQ_CIRCLE = 1 };      /* types.h:49 */
 enum lq_key_mode { LQ_KEY_LINEAR = 0, LQ_KEY_MORTON = 1 };
 
 /* Interaction kernel variant.  All three produce bit-identical results.
- * TILE (default) stages each cell-column tile's neighbourhood in LDS and
- * scans it from there; REFERENCE_ORDER walks the 27 cells per thread exactly
- * as particle_interaction.cu:117-140; NEIGHBOR_LIST scans global memory per
- * lane and compacts hits into LDS lists before evaluating them. */
+ * REFERENCE_ORDER (default) walks the 27 cells per thread exactly as
+ * particle_interaction.cu:117-140; TILE stages each cell-column tile's
+ * neighbourhood in LDS and scans it from there; NEIGHBOR_LIST scans global
+ * memory per lane and compacts hits into LDS lists before evaluating them.
+ * lq_autotune picks the fastest for the current state. */
 enum lq_interaction_variant {
-  LQ_INTERACTION_TILE = 0,
-  LQ_INTERACTION_REFERENCE_ORDER = 1,
+  LQ_INTERACTION_REFERENCE_ORDER = 0,
+  LQ_INTERACTION_TILE = 1,
   LQ_INTERACTION_NEIGHBOR_LIST = 2
 };
 
@@ -269,11 +270,25 @@ int lq_kernel_times(lq_ctx* ctx, double* ms, int64_t* launches, int n);
 
 /* Tile shape of the TILE interaction variant: shape index (0: 1 wave,
  * 1: 2 waves, 2: 4 waves per workgroup) and tz = cells per tile along z.
- * lq_autotune times every shape over a few tile depths on the current state
- * (contact arrays are restored afterwards), keeps the fastest and reports it.
- * No reference counterpart (the reference has one fixed 128-thread launch). */
+ * Contact arrays are restored after tuning.  No reference counterpart (the
+ * reference has one fixed 128-thread launch). */
 int lq_set_tile(lq_ctx* ctx, int shape, int tz);
-int lq_autotune(lq_ctx* ctx, int reps, int* shape, int* tz, double* ms_per_call);
+/* Switch the interaction variant of a live context; xcd_remap != 0 makes
+ * REFERENCE_ORDER and NEIGHBOR_LIST hand each of the 8 XCDs one contiguous
+ * run of particle blocks (hardware places block b on XCD b % 8).  No
+ * reference counterpart. */
+int lq_set_interaction(lq_ctx* ctx, int variant, int xcd_remap);
+/* lq_autotune times every interaction variant (reference order, neighbour
+ * lists, each tile shape over a few depths) on the current state, makes the
+ * fastest the active one and reports it (shape/tz meaningful for TILE,
+ * xcd_remap for REFERENCE_ORDER and NEIGHBOR_LIST). */
+int lq_autotune(lq_ctx* ctx, int reps, int* variant, int* shape, int* tz, int* xcd_remap, double* ms_per_call);
+
+/* Squared pair cut-off used by every neighbour kernel: the largest float
+ * r2_cut with fl(fl(sqrt(r2_cut)) * i_h) < 2, so r^2 <= r2_cut is exactly
+ * the reference's q < 2 test (particle_interaction.cu:33-35,
+ * adami_boundary.cu:60-62).  Host-only, no device needed. */
+int lq_interaction_cutoff_sq(float i_h, float* r2_cut);
 
 /* Neighbour list of every soil slot (debug/parity): for soil slot s the
  * original ids (.dat line) of all particles j with q_ij < 2, in traversal

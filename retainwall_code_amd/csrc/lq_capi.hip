@@ -32,6 +32,7 @@ struct lq_ctx {
   int nbits = 0;
   int npm0 = 0;
   int tile_shape = 1, tile_tz = 7;
+  int xcd_remap = 0;  // renumber interaction blocks per XCD (lq_set_interaction / lq_autotune)
   long long host_step = 0;
   // instrumentation
   bool timing = false;
@@ -172,6 +173,27 @@ extern "C" int lq_initialize_parameters(lq_params* p) {
   return LQ_OK;
 }
 
+// Largest float r2 with fl(fl(sqrt(r2)) * i_h) < 2: the kernels keep a
+// candidate pair when r^2 <= r2_cut, which is the reference's q < 2 test
+// (particle_interaction.cu:33-35) bit for bit, because correctly rounded
+// sqrt and multiply are monotone.  Non-negative floats order like their bit
+// patterns, so the cut is found by bisection over [0, +inf).
+extern "C" int lq_interaction_cutoff_sq(float i_h, float* r2_cut) {
+  if (!r2_cut || !(i_h > 0.f) || !std::isfinite(i_h)) return LQ_ERR_ARG;
+  auto keep = [i_h](uint32_t u) {
+    float r2;
+    std::memcpy(&r2, &u, sizeof r2);
+    return std::sqrt(r2) * i_h < 2.0f;
+  };
+  uint32_t lo = 0u, hi = 0x7f800000u;  // keep(lo), !keep(hi)
+  while (hi - lo > 1u) {
+    const uint32_t mid = lo + (hi - lo) / 2u;
+    if (keep(mid)) lo = mid; else hi = mid;
+  }
+  std::memcpy(r2_cut, &lo, sizeof lo);
+  return LQ_OK;
+}
+
 static void fill_static(const lq_params& p, DevParams& d) {
   std::memset(&d, 0, sizeof(d));
   d.np = p.np; d.npb = p.npb;
@@ -181,6 +203,7 @@ static void fill_static(const lq_params& p, DevParams& d) {
   d.n_motion = p.n_motion; d.n_model = p.n_model;
   d.h = p.h; d.i_h = p.i_h; d.dr = p.dr; d.vol_def = p.vol_def; d.bt = p.bound_thickness; d.eta = p.eta;
   d.two_h = 2.0f * p.h;
+  lq_interaction_cutoff_sq(p.i_h, &d.r2cut);
   d.gx = p.gravity[0]; d.gy = p.gravity[1]; d.gz = p.gravity[2];
   d.damping = p.damping;
   d.knorm = p.kernel_normalization_par; d.kgrad = p.kernel_gradient_normalization_par;
@@ -274,6 +297,7 @@ extern "C" int lq_create(const lq_params* par, int device, lq_ctx** out) {
   st |= dalloc(c, &B.acestn_a, n); st |= dalloc(c, &B.acestn_b, n);
   st |= dalloc(c, &B.s3, n); st |= dalloc(c, &B.fdef, n); st |= dalloc(c, &B.deferred, n);
   st |= dalloc(c, &B.str_tmp_a, n); st |= dalloc(c, &B.str_tmp_b, n);
+  st |= dalloc(c, &B.sr_a, n); st |= dalloc(c, &B.sr_b, n);
   st |= dalloc(c, &B.keys, n); st |= dalloc(c, &B.vals, n);
   st |= dalloc(c, &B.keys_alt, n); st |= dalloc(c, &B.vals_alt, n);
   st |= dalloc(c, &B.cell_beg, size_t(par->ncell)); st |= dalloc(c, &B.cell_end, size_t(par->ncell));
@@ -676,11 +700,12 @@ static void enq_interaction_kernel(lq_ctx* c) {
                               c->B, c->B.cell_beg, c->B.cell_end, c->tile_shape, c->tile_tz);
   } else {
     launch_interaction(c->stream, c->dP, c->npm0, c->B.cur, c->B, c->B.cell_beg, c->B.cell_end,
-                       c->par.interaction_variant);
+                       c->par.interaction_variant, c->xcd_remap);
   }
 }
 static int enq_interaction(lq_ctx* c) {
   launch_reset_deferred(c->stream, c->dP);
+  launch_pair_prep(c->stream, c->dP, c->par.npb + c->npm0, c->B.cur, c->B);
   {
     Section s(c, LQ_K_INTERACTION);
     enq_interaction_kernel(c);
@@ -904,10 +929,24 @@ extern "C" int lq_set_tile(lq_ctx* c, int shape, int tz) {
   return LQ_OK;
 }
 
-// Times the interaction kernel for each tile shape over a few depths on the
-// current state and keeps the fastest.  The launches mutate the slip flag and
-// contact forces, so cvar/cf are saved to the alternate buffers and restored.
-extern "C" int lq_autotune(lq_ctx* c, int reps, int* shape_out, int* tz_out, double* ms_out) {
+// Selects the interaction kernel: variant and, for REFERENCE_ORDER and
+// NEIGHBOR_LIST, whether blocks are renumbered per XCD.
+extern "C" int lq_set_interaction(lq_ctx* c, int variant, int xcd_remap) {
+  if (!c) return LQ_ERR_ARG;
+  if (variant < LQ_INTERACTION_REFERENCE_ORDER || variant > LQ_INTERACTION_NEIGHBOR_LIST)
+    return fail(c, LQ_ERR_ARG, "lq_set_interaction: variant out of range");
+  c->par.interaction_variant = variant;
+  c->xcd_remap = xcd_remap ? 1 : 0;
+  c->layout_version++;  // re-capture graphs
+  return LQ_OK;
+}
+
+// Times every interaction variant on the current state -- reference order,
+// neighbour lists, and each tile shape over a few depths -- and keeps the
+// fastest.  The launches mutate the slip flag and contact forces, so
+// cvar/cf are saved to the alternate buffers and restored.
+extern "C" int lq_autotune(lq_ctx* c, int reps, int* variant_out, int* shape_out, int* tz_out, int* remap_out,
+                           double* ms_out) {
   NEED_UPLOAD(c);
   if (reps <= 0) return fail(c, LQ_ERR_ARG, "lq_autotune: reps");
   hipSetDevice(c->device);
@@ -916,50 +955,62 @@ extern "C" int lq_autotune(lq_ctx* c, int reps, int* shape_out, int* tz_out, dou
   Arrays& T = c->B.alt;
   HIPCHECK(c, hipMemcpyAsync(T.cvar, A.cvar, n * 16, hipMemcpyDeviceToDevice, c->stream));
   HIPCHECK(c, hipMemcpyAsync(T.cf, A.cf, n * 16, hipMemcpyDeviceToDevice, c->stream));
-  hipEvent_t e0, e1;
-  hipEventCreate(&e0);
-  hipEventCreate(&e1);
-  const int save_shape = c->tile_shape, save_tz = c->tile_tz, save_var = c->par.interaction_variant;
-  c->par.interaction_variant = LQ_INTERACTION_TILE;
-  double best = 1e30;
-  int bs = save_shape, bt = save_tz;
+  launch_pair_prep(c->stream, c->dP, c->par.npb + c->npm0, A, c->B);
+  struct Cand { int variant, shape, tz, remap; };
+  std::vector<Cand> cands;
+  for (int remap = 0; remap < 2; remap++) {
+    cands.push_back({LQ_INTERACTION_REFERENCE_ORDER, c->tile_shape, c->tile_tz, remap});
+    cands.push_back({LQ_INTERACTION_NEIGHBOR_LIST, c->tile_shape, c->tile_tz, remap});
+  }
   for (int sh = 0; sh < interaction_tile_shapes(); sh++) {
     const int waves = interaction_tile_waves(sh);
     const int base = std::max(1, (waves * 64 * 9) / (10 * 16));  // ~90% of the lanes at ~16 soil per cell
-    for (int tz = std::max(1, base - 2); tz <= std::min(interaction_tile_maxtz(sh), base + 2); tz++) {
-      c->tile_shape = sh;
-      c->tile_tz = tz;
+    for (int tz = std::max(1, base - 2); tz <= std::min(interaction_tile_maxtz(sh), base + 2); tz++)
+      cands.push_back({LQ_INTERACTION_TILE, sh, tz, c->xcd_remap});
+  }
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  double best = 1e30;
+  Cand win = {c->par.interaction_variant, c->tile_shape, c->tile_tz, c->xcd_remap};
+  for (const Cand& k : cands) {
+    c->par.interaction_variant = k.variant;
+    c->tile_shape = k.shape;
+    c->tile_tz = k.tz;
+    c->xcd_remap = k.remap;
+    launch_reset_deferred(c->stream, c->dP);
+    enq_interaction_kernel(c);  // warm-up
+    hipEventRecord(e0, c->stream);
+    for (int r = 0; r < reps; r++) {
       launch_reset_deferred(c->stream, c->dP);
-      enq_interaction_kernel(c);  // warm-up
-      hipEventRecord(e0, c->stream);
-      for (int r = 0; r < reps; r++) {
-        launch_reset_deferred(c->stream, c->dP);
-        enq_interaction_kernel(c);
-      }
-      hipEventRecord(e1, c->stream);
-      hipError_t e = hipEventSynchronize(e1);
-      if (e != hipSuccess) {
-        hipEventDestroy(e0);
-        hipEventDestroy(e1);
-        return fail(c, LQ_ERR_HIP, std::string("lq_autotune: ") + hipGetErrorString(e));
-      }
-      float ms = 0.f;
-      hipEventElapsedTime(&ms, e0, e1);
-      const double per = double(ms) / reps;
-      if (per < best) { best = per; bs = sh; bt = tz; }
+      enq_interaction_kernel(c);
     }
+    hipEventRecord(e1, c->stream);
+    hipError_t e = hipEventSynchronize(e1);
+    if (e != hipSuccess) {
+      hipEventDestroy(e0);
+      hipEventDestroy(e1);
+      return fail(c, LQ_ERR_HIP, std::string("lq_autotune: ") + hipGetErrorString(e));
+    }
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    const double per = double(ms) / reps;
+    if (per < best) { best = per; win = k; }
   }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   HIPCHECK(c, hipMemcpyAsync(A.cvar, T.cvar, n * 16, hipMemcpyDeviceToDevice, c->stream));
   HIPCHECK(c, hipMemcpyAsync(A.cf, T.cf, n * 16, hipMemcpyDeviceToDevice, c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
-  c->par.interaction_variant = save_var;
-  c->tile_shape = bs;
-  c->tile_tz = bt;
+  c->par.interaction_variant = win.variant;
+  c->tile_shape = win.shape;
+  c->tile_tz = win.tz;
+  c->xcd_remap = win.remap;
   c->layout_version++;
-  if (shape_out) *shape_out = bs;
-  if (tz_out) *tz_out = bt;
+  if (variant_out) *variant_out = win.variant;
+  if (shape_out) *shape_out = win.shape;
+  if (tz_out) *tz_out = win.tz;
+  if (remap_out) *remap_out = win.remap;
   if (ms_out) *ms_out = best;
   return LQ_OK;
 }

@@ -48,12 +48,27 @@ __device__ __forceinline__ float3 smatvec(Sym s, float3 v) {  // functions.h:97
 }
 
 // Candidate geometry, particle_interaction.cu:30-33:
-// d = pos_i - pos_j - off*2h, r = |d| (r^2 = fma(dz,dz,fma(dy,dy,dx*dx))).
-__device__ __forceinline__ float pair_r(float3 pi, float4 pj, int ox, int oy, int oz, float two_h, float3& d) {
+// d = pos_i - pos_j - off*2h and r^2 = fma(dz,dz,fma(dy,dy,dx*dx)); r = sqrt(r^2).
+// The reference keeps a pair when q = fl(fl(sqrt(r^2)) * (1/h)) < 2.  With
+// correctly rounded sqrt and multiply that test is monotone in r^2, so it is
+// exactly r^2 <= r2cut for the largest float r2cut that passes
+// (lq_interaction_cutoff_sq bisects the float bit patterns for it); the
+// square root is then taken only for kept pairs.
+__device__ __forceinline__ float pair_r2(float3 pi, float4 pj, int ox, int oy, int oz, float two_h, float3& d) {
   d.x = (pi.x - pj.x) - float(ox) * two_h;
   d.y = (pi.y - pj.y) - float(oy) * two_h;
   d.z = (pi.z - pj.z) - float(oz) * two_h;
-  return sqrtf(fmaf(d.z, d.z, fmaf(d.y, d.y, d.x * d.x)));
+  return fmaf(d.z, d.z, fmaf(d.y, d.y, d.x * d.x));
+}
+
+// Hardware places workgroup b on XCD b % 8.  Renumber the 1-D grid so each
+// XCD runs one contiguous run of logical blocks: consecutive particles are
+// cell neighbours, so their candidate data then share that XCD's L2.
+__device__ __forceinline__ int xcd_block(int remap) {
+  const int nb = int(gridDim.x), b = int(blockIdx.x);
+  if (!remap) return b;
+  const int x = b & 7, q = nb >> 3, r = nb & 7;
+  return x * q + min(x, r) + (b >> 3);
 }
 
 // ------------------------------------------------------- interaction ----
@@ -67,9 +82,11 @@ struct IAcc {
 
 __device__ __forceinline__ void pair_terms(const DevParams* __restrict__ P, IAcc& S, float3 d, float r, float q,
                                            float3 vi, float rhoi, Sym sri, float artpi, int shift, bool bound,
-                                           float4 vj, float wj, Sym sj) {
+                                           float4 vj, float wj, float4 sra, float4 srb) {
   // wj = pos.w of the neighbour: |wj| is a soil particle's mass, and wj < 0
-  // marks a contact-wall code (codes.x >= 10).
+  // marks a contact-wall code (codes.x >= 10).  sra/srb are the neighbour's
+  // per-particle terms from k_pair_prep: sigma/rho^2 (xx yy zz xy | xz yz),
+  // m/rho and trace(sigma/rho^2), each rounded exactly as the inline form.
   const bool contactj = wj < 0.f;
   float ker = kW(P->knorm, q);
   float dwdr = kG(P->kgrad, q);
@@ -78,7 +95,7 @@ __device__ __forceinline__ void pair_terms(const DevParams* __restrict__ P, IAcc
   float mj = bound ? P->vol_def * rhoj : fabsf(wj);
   float3 dv = f3sub(vi, ld3(vj));
   if (contactj && shift == 1) dv = make_float3(0.f, 0.f, 0.f);
-  float volj = mj / rhoj;
+  const float volj = srb.z;
   float3 dvv = f3mul(dv, volj);
   S.gv[0] = fmaf(-dvv.x, g.x, S.gv[0]); S.gv[1] = fmaf(-dvv.x, g.y, S.gv[1]); S.gv[2] = fmaf(-dvv.x, g.z, S.gv[2]);
   S.gv[3] = fmaf(-dvv.y, g.x, S.gv[3]); S.gv[4] = fmaf(-dvv.y, g.y, S.gv[4]); S.gv[5] = fmaf(-dvv.y, g.z, S.gv[5]);
@@ -86,8 +103,7 @@ __device__ __forceinline__ void pair_terms(const DevParams* __restrict__ P, IAcc
   if (contactj) return;
   float dvg = fmaf(dv.z, g.z, fmaf(dv.y, g.y, dv.x * g.x));
   S.accden = fmaf(rhoi * dvg, volj, S.accden);
-  float rhoj2 = rhoj * rhoj;
-  Sym srj = sdiv(sj, rhoj2);
+  const Sym srj = {sra.x, sra.y, sra.z, sra.w, srb.x, srb.y};
   Sym st = sadd(sri, srj);
   float3 f = smatvec(st, g);
   S.as.x = fmaf(f.x, mj, S.as.x); S.as.y = fmaf(f.y, mj, S.as.y); S.as.z = fmaf(f.z, mj, S.as.z);
@@ -101,7 +117,7 @@ __device__ __forceinline__ void pair_terms(const DevParams* __restrict__ P, IAcc
     }
   }
   if (P->art_p != 0.f) {
-    float trj = strace(srj);
+    float trj = srb.w;
     float artp = artpi + (trj > 0.f ? trj * P->art_p : 0.f);
     if (artp > 0.f) {
       artp = artp * powf(ker / P->kernel_dr, P->art_exp);

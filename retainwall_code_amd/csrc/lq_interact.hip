@@ -35,7 +35,7 @@ __device__ __forceinline__ void tile_home(DevParams* __restrict__ P, const Array
   const int t = threadIdx.x;
   const int lane = t & 63, wv = t >> 6;
   const int ncm = P->ncell_material;
-  const float two_h = P->two_h, i_h = P->i_h;
+  const float two_h = P->two_h, i_h = P->i_h, r2cut = P->r2cut;
   const int H = he - hb;
   for (int base = 0; base < H; base += WG) {
     const int li = base + t;
@@ -46,9 +46,9 @@ __device__ __forceinline__ void tile_home(DevParams* __restrict__ P, const Array
     const float4 vri = A.vel[i];
     const float3 vi = ld3(vri);
     const float rhoi = vri.w;
-    const float rhoi2 = rhoi * rhoi;
-    const Sym sri = sdiv(ldsym(A.str_a, A.str_b, i), rhoi2);
-    const float tri = strace(sri);
+    const float4 sria = B.sr_a[i], srib = B.sr_b[i];
+    const Sym sri = {sria.x, sria.y, sria.z, sria.w, srib.x, srib.y};
+    const float tri = srib.w;
     const float artpi = tri > 0.f ? tri * P->art_p : 0.f;
     const int shift = int(A.cvar[i].z);
     const int zoff = ci.z - zw0;
@@ -97,8 +97,7 @@ __device__ __forceinline__ void tile_home(DevParams* __restrict__ P, const Array
         if constexpr (STAGED) pj = s_pos[j];
         else pj = A.pos[j];
         float3 d;
-        const float r = pair_r(pi, pj, cx, cy, cz, two_h, d);
-        if (r * i_h < 2.0f) {
+        if (pair_r2(pi, pj, cx, cy, cz, two_h, d) <= r2cut) {
           s_list[wv][n][lane] = (unsigned(j) << 6) | unsigned(cur);
           n++;
         }
@@ -123,10 +122,10 @@ __device__ __forceinline__ void tile_home(DevParams* __restrict__ P, const Array
             slot = jj;
           }
           float3 d;
-          const float r = pair_r(pi, pj, cx, cy, cz, two_h, d);
+          const float r = sqrtf(pair_r2(pi, pj, cx, cy, cz, two_h, d));
           const float q = r * i_h;
-          pair_terms(P, S, d, r, q, vi, rhoi, sri, artpi, shift, bsel == 0, A.vel[slot], pj.w,
-                     ldsym(A.str_a, A.str_b, slot));
+          pair_terms(P, S, d, r, q, vi, rhoi, sri, artpi, shift, bsel == 0, A.vel[slot], pj.w, B.sr_a[slot],
+                     B.sr_b[slot]);
         }
       }
       const bool more = (j < jend) || (cstep < 54);

@@ -45,7 +45,7 @@ struct DevParams {
   // ---- static (set at create) ----
   int np, npb, cnx, cny, cnz, ncell, ncell_material, ncell_delete_m;
   int key_mode, shepard, viscosity_type, n_motion, n_model, nbits, pad0, pad1;
-  float h, i_h, dr, vol_def, bt, eta, two_h, two_h_sq_lim;
+  float h, i_h, dr, vol_def, bt, eta, two_h, r2cut;
   float gx, gy, gz, damping, knorm, kgrad, kernel_zero, kernel_dr;
   float art_exp, alpha_cs, art_p, cfl, cs, band, half_dr, mu;
   float model_par[LQ_MODEL_NUMBER_MAX][LQ_MATERIAL_SIZE_MAX];
@@ -85,6 +85,8 @@ struct Buffers {
   int* deferred;     // compact list of deferred soil slots
   float4* str_tmp_a; // Shepard output scratch
   float2* str_tmp_b;
+  float4* sr_a;      // per-slot sigma/rho^2 (xx yy zz xy), written by k_pair_prep
+  float4* sr_b;      // per-slot sigma/rho^2 (xz yz), m/rho, trace
   // grid / sort
   int *keys, *vals, *keys_alt, *vals_alt, *part_cel, *part_idx;
   int *cell_beg, *cell_end;

@@ -16,8 +16,9 @@ void launch_corner(hipStream_t s, const DevParams* P, int npm_max, Arrays A, con
 void launch_shepard(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const int* cb, const int* ce,
                     float4* oa, float2* ob, bool copy_back = true);
 void launch_reset_deferred(hipStream_t s, DevParams* P);
+void launch_pair_prep(hipStream_t s, const DevParams* P, int n, Arrays A, const Buffers& B);
 void launch_interaction(hipStream_t s, DevParams* P, int npm_max, Arrays A, const Buffers& B, const int* cb,
-                        const int* ce, int variant);
+                        const int* ce, int variant, int xcd_remap);
 void launch_contact(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const Buffers& B, const int* cb,
                     const int* ce);
 void launch_predictor(hipStream_t s, const DevParams* P, int np, Arrays A, const Buffers& B);

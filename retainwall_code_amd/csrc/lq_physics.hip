@@ -99,13 +99,13 @@ __global__ void k_rigid_body_update(DevParams* P) {
 // ------------------------------------------------------------- Adami ----
 // cuAdamiBoundaryMaterial + loops, adami_boundary.cu:26-158.
 __global__ void __launch_bounds__(256) k_adami(const DevParams* __restrict__ P, Arrays A,
-                                               const int* __restrict__ cbeg, const int* __restrict__ cend) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+                                               const int* __restrict__ cbeg, const int* __restrict__ cend, int remap) {
+  int i = xcd_block(remap) * blockDim.x + threadIdx.x;
   if (i >= P->npb) return;
   const float3 pi = ld3(A.pos[i]);
   const int4 ci = A.cellpos[i];
   const bool contact = A.codes[i].x >= 10;
-  const float two_h = P->two_h, i_h = P->i_h;
+  const float two_h = P->two_h, i_h = P->i_h, r2cut = P->r2cut;
   float3 n = make_float3(0.f, 0.f, 0.f);
   Sym rs = {0, 0, 0, 0, 0, 0};
   float3 rpos = make_float3(0.f, 0.f, 0.f);
@@ -121,9 +121,10 @@ __global__ void __launch_bounds__(256) k_adami(const DevParams* __restrict__ P, 
         if (contact) {
           for (int j = cbeg[cb], e = cend[cb]; j < e; j++) {
             float3 d;
-            float r = pair_r(pi, A.pos[j], cx, cy, cz, two_h, d);
+            const float r2 = pair_r2(pi, A.pos[j], cx, cy, cz, two_h, d);
+            if (!(r2 <= r2cut)) continue;
+            float r = sqrtf(r2);
             float q = r * i_h;
-            if (!(q < 2.0f)) continue;
             float dwdr = kG(P->kgrad, q);
             float3 g = f3mul(d, dwdr);
             n = f3sub(n, f3mul(g, P->vol_def));
@@ -132,9 +133,10 @@ __global__ void __launch_bounds__(256) k_adami(const DevParams* __restrict__ P, 
         int cm = cb + P->ncell_material;
         for (int j = cbeg[cm], e = cend[cm]; j < e; j++) {
           float3 d;
-          float r = pair_r(pi, A.pos[j], cx, cy, cz, two_h, d);
+          const float r2 = pair_r2(pi, A.pos[j], cx, cy, cz, two_h, d);
+          if (!(r2 <= r2cut)) continue;
+          float r = sqrtf(r2);
           float q = r * i_h;
-          if (!(q < 2.0f)) continue;
           float ker = kW(P->knorm, q);
           float4 idmj = A.idm[j];
           float4 vj = A.vel[j];
@@ -190,13 +192,13 @@ __global__ void __launch_bounds__(256) k_adami(const DevParams* __restrict__ P, 
 // ------------------------------------------------------------ corner ----
 // cuFindCornerParticleMaterial + loop, adami_boundary.cu:209-292.
 __global__ void __launch_bounds__(256) k_corner(const DevParams* __restrict__ P, Arrays A,
-                                                const int* __restrict__ cbeg, const int* __restrict__ cend) {
-  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+                                                const int* __restrict__ cbeg, const int* __restrict__ cend, int remap) {
+  int idx = xcd_block(remap) * blockDim.x + threadIdx.x;
   if (idx >= P->npm) return;
   const int i = P->npb + idx;
   const float3 pi = ld3(A.pos[i]);
   const int4 ci = A.cellpos[i];
-  const float two_h = P->two_h, i_h = P->i_h;
+  const float two_h = P->two_h, i_h = P->i_h, r2cut = P->r2cut;
   float4 cv = A.cvar[i];
   float4 cn = A.cn[i];
   float4 cw = A.cw[i];
@@ -211,9 +213,10 @@ __global__ void __launch_bounds__(256) k_corner(const DevParams* __restrict__ P,
         int c = lin_key(P, jx, jy, jz);
         for (int j = cbeg[c], e = cend[c]; j < e; j++) {
           float3 d;
-          float r = pair_r(pi, A.pos[j], cx, cy, cz, two_h, d);
+          const float r2 = pair_r2(pi, A.pos[j], cx, cy, cz, two_h, d);
+          if (!(r2 <= r2cut)) continue;
+          float r = sqrtf(r2);
           float q = r * i_h;
-          if (!(q < 2.0f)) continue;
           int cj = A.codes[j].x;
           if (cj < 10) continue;
           float dwdr = kG(P->kgrad, q);
@@ -248,6 +251,7 @@ __global__ void __launch_bounds__(256) k_shepard(const DevParams* __restrict__ P
   const int i = P->npb + idx;
   const float3 pi = ld3(A.pos[i]);
   const int4 ci = A.cellpos[i];
+  const float r2cut = P->r2cut;
   Sym acc = {0, 0, 0, 0, 0, 0};
   float sk = 0.f;
   for (int cz = -1; cz <= 1; cz++)
@@ -258,9 +262,10 @@ __global__ void __launch_bounds__(256) k_shepard(const DevParams* __restrict__ P
         int c = lin_key(P, jx, jy, jz) + P->ncell_material;
         for (int j = cbeg[c], e = cend[c]; j < e; j++) {
           float3 d;
-          float r = pair_r(pi, A.pos[j], cx, cy, cz, P->two_h, d);
+          const float r2 = pair_r2(pi, A.pos[j], cx, cy, cz, P->two_h, d);
+          if (!(r2 <= r2cut)) continue;
+          float r = sqrtf(r2);
           float q = r * P->i_h;
-          if (!(q < 2.0f)) continue;
           float ker = kW(P->knorm, q);
           float vol = A.idm[j].y / A.vel[j].w;
           float wv = ker * vol;
@@ -281,11 +286,27 @@ __global__ void k_copy_soil_str(const DevParams* __restrict__ P, Arrays A, const
   A.str_b[i] = ob[i];
 }
 
+// Per-particle terms of the pair sum that depend on the neighbour alone
+// (particle_interaction.cu:52-60): sigma/rho^2, its trace and m/rho, with m
+// the Adami volume mass vol_def*rho for wall slots and the particle mass for
+// soil.  Computed once per slot here instead of once per pair; the values
+// are rounded exactly as the inline expressions.
+__global__ void __launch_bounds__(256) k_pair_prep(const DevParams* __restrict__ P, Arrays A, Buffers B, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float rho = A.vel[i].w;
+  const float m = i < P->npb ? P->vol_def * rho : fabsf(A.pos[i].w);
+  const Sym sr = sdiv(ldsym(A.str_a, A.str_b, i), rho * rho);
+  B.sr_a[i] = make_float4(sr.xx, sr.yy, sr.zz, sr.xy);
+  B.sr_b[i] = make_float4(sr.xz, sr.yz, m / rho, strace(sr));
+}
+
 // Reference-order variant: each thread walks its 27 cells (cz, cy, cx;
 // boundary range then soil range) exactly as particle_interaction.cu:117-140.
 __global__ void __launch_bounds__(256) k_interaction_ref(DevParams* __restrict__ P, Arrays A, Buffers B,
-                                                         const int* __restrict__ cbeg, const int* __restrict__ cend) {
-  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+                                                         const int* __restrict__ cbeg, const int* __restrict__ cend,
+                                                         int remap) {
+  int idx = xcd_block(remap) * blockDim.x + threadIdx.x;
   if (idx >= P->npm) return;
   const int i = P->npb + idx;
   const float3 pi = ld3(A.pos[i]);
@@ -293,12 +314,12 @@ __global__ void __launch_bounds__(256) k_interaction_ref(DevParams* __restrict__
   const float4 vri = A.vel[i];
   const float3 vi = ld3(vri);
   const float rhoi = vri.w;
-  const float rhoi2 = rhoi * rhoi;
-  const Sym sri = sdiv(ldsym(A.str_a, A.str_b, i), rhoi2);
-  const float tri = strace(sri);
+  const float4 sria = B.sr_a[i], srib = B.sr_b[i];
+  const Sym sri = {sria.x, sria.y, sria.z, sria.w, srib.x, srib.y};
+  const float tri = srib.w;
   const float artpi = tri > 0.f ? tri * P->art_p : 0.f;
   const int shift = int(A.cvar[i].z);
-  const float two_h = P->two_h, i_h = P->i_h;
+  const float two_h = P->two_h, i_h = P->i_h, r2cut = P->r2cut;
   IAcc S;
   S.accden = 0.f;
   S.as = S.av = S.aa = make_float3(0.f, 0.f, 0.f);
@@ -316,11 +337,11 @@ __global__ void __launch_bounds__(256) k_interaction_ref(DevParams* __restrict__
           for (int j = cbeg[c], e = cend[c]; j < e; j++) {
             float3 d;
             const float4 pj = A.pos[j];
-            float r = pair_r(pi, pj, cx, cy, cz, two_h, d);
+            const float r2 = pair_r2(pi, pj, cx, cy, cz, two_h, d);
+            if (!(r2 <= r2cut)) continue;
+            float r = sqrtf(r2);
             float q = r * i_h;
-            if (!(q < 2.0f)) continue;
-            pair_terms(P, S, d, r, q, vi, rhoi, sri, artpi, shift, bound, A.vel[j], pj.w,
-                       ldsym(A.str_a, A.str_b, j));
+            pair_terms(P, S, d, r, q, vi, rhoi, sri, artpi, shift, bound, A.vel[j], pj.w, B.sr_a[j], B.sr_b[j]);
           }
         }
       }
@@ -338,11 +359,11 @@ constexpr int NL_CAP = 24;
 
 __global__ void __launch_bounds__(NL_WAVE * NL_WAVES) k_interaction_nl(DevParams* __restrict__ P, Arrays A, Buffers B,
                                                                       const int* __restrict__ cbeg,
-                                                                      const int* __restrict__ cend) {
+                                                                      const int* __restrict__ cend, int remap) {
   __shared__ int s_list[NL_WAVES][NL_CAP][NL_WAVE];
   const int lane = threadIdx.x & (NL_WAVE - 1);
   const int wv = threadIdx.x / NL_WAVE;
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int idx = xcd_block(remap) * blockDim.x + threadIdx.x;
   const bool active = idx < P->npm;
   const int i = P->npb + (active ? idx : 0);
   const float3 pi = ld3(A.pos[i]);
@@ -350,12 +371,12 @@ __global__ void __launch_bounds__(NL_WAVE * NL_WAVES) k_interaction_nl(DevParams
   const float4 vri = A.vel[i];
   const float3 vi = ld3(vri);
   const float rhoi = vri.w;
-  const float rhoi2 = rhoi * rhoi;
-  const Sym sri = sdiv(ldsym(A.str_a, A.str_b, i), rhoi2);
-  const float tri = strace(sri);
+  const float4 sria = B.sr_a[i], srib = B.sr_b[i];
+  const Sym sri = {sria.x, sria.y, sria.z, sria.w, srib.x, srib.y};
+  const float tri = srib.w;
   const float artpi = tri > 0.f ? tri * P->art_p : 0.f;
   const int shift = int(A.cvar[i].z);
-  const float two_h = P->two_h, i_h = P->i_h;
+  const float two_h = P->two_h, i_h = P->i_h, r2cut = P->r2cut;
   const int ncm = P->ncell_material;
   IAcc S;
   S.accden = 0.f;
@@ -392,9 +413,8 @@ __global__ void __launch_bounds__(NL_WAVE * NL_WAVES) k_interaction_nl(DevParams
       int cell = cur >> 1;
       int cz = cell / 9 - 1, cy = (cell / 3) % 3 - 1, cx = cell % 3 - 1;
       float3 d;
-      float r = pair_r(pi, A.pos[j], cx, cy, cz, two_h, d);
-      float q = r * i_h;
-      if (q < 2.0f) {
+      const float r2 = pair_r2(pi, A.pos[j], cx, cy, cz, two_h, d);
+      if (r2 <= r2cut) {
         s_list[wv][n][lane] = int((unsigned(j) << 6) | unsigned(cur));
         n++;
       }
@@ -412,10 +432,9 @@ __global__ void __launch_bounds__(NL_WAVE * NL_WAVES) k_interaction_nl(DevParams
         int cz = cell / 9 - 1, cy = (cell / 3) % 3 - 1, cx = cell % 3 - 1;
         float3 d;
         const float4 pj = A.pos[jj];
-        float r = pair_r(pi, pj, cx, cy, cz, two_h, d);
-        float q = r * i_h;
-        pair_terms(P, S, d, r, q, vi, rhoi, sri, artpi, shift, bound, A.vel[jj], pj.w,
-                   ldsym(A.str_a, A.str_b, jj));
+        const float r = sqrtf(pair_r2(pi, pj, cx, cy, cz, two_h, d));
+        const float q = r * i_h;
+        pair_terms(P, S, d, r, q, vi, rhoi, sri, artpi, shift, bound, A.vel[jj], pj.w, B.sr_a[jj], B.sr_b[jj]);
       }
     }
     bool more = (j < jend) || (cstep < 54);
@@ -668,8 +687,8 @@ __global__ void k_neighbor_list(const DevParams* __restrict__ P, Arrays A, const
           int c = pass ? cb + P->ncell_material : cb;
           for (int j = cbeg[c], e = cend[c]; j < e; j++) {
             float3 d;
-            float r = pair_r(pi, A.pos[j], cx, cy, cz, P->two_h, d);
-            if (!(r * P->i_h < 2.0f)) continue;
+            const float r2 = pair_r2(pi, A.pos[j], cx, cy, cz, P->two_h, d);
+            if (!(r2 <= P->r2cut)) continue;
             if (n < max_nb) ids[(size_t)idx * max_nb + n] = A.orig_id[j];
             n++;
           }
@@ -704,10 +723,10 @@ void launch_rigid_body_update(hipStream_t s, DevParams* P) {
   hipLaunchKernelGGL(k_rigid_body_update, dim3(1), dim3(1), 0, s, P);
 }
 void launch_adami(hipStream_t s, const DevParams* P, int npb, Arrays A, const int* cb, const int* ce) {
-  if (npb > 0) hipLaunchKernelGGL(k_adami, dim3(blocks(npb, 256)), dim3(256), 0, s, P, A, cb, ce);
+  if (npb > 0) hipLaunchKernelGGL(k_adami, dim3(blocks(npb, 256)), dim3(256), 0, s, P, A, cb, ce, 1);
 }
 void launch_corner(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const int* cb, const int* ce) {
-  if (npm_max > 0) hipLaunchKernelGGL(k_corner, dim3(blocks(npm_max, 256)), dim3(256), 0, s, P, A, cb, ce);
+  if (npm_max > 0) hipLaunchKernelGGL(k_corner, dim3(blocks(npm_max, 256)), dim3(256), 0, s, P, A, cb, ce, 1);
 }
 void launch_shepard(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const int* cb, const int* ce,
                     float4* oa, float2* ob, bool copy_back) {
@@ -719,14 +738,18 @@ void launch_shepard(hipStream_t s, const DevParams* P, int npm_max, Arrays A, co
 void launch_reset_deferred(hipStream_t s, DevParams* P) {
   hipLaunchKernelGGL(k_reset_deferred, dim3(1), dim3(1), 0, s, P);
 }
+void launch_pair_prep(hipStream_t s, const DevParams* P, int n, Arrays A, const Buffers& B) {
+  if (n > 0) hipLaunchKernelGGL(k_pair_prep, dim3(blocks(n, 256)), dim3(256), 0, s, P, A, B, n);
+}
 void launch_interaction(hipStream_t s, DevParams* P, int npm_max, Arrays A, const Buffers& B, const int* cb,
-                        const int* ce, int variant) {
+                        const int* ce, int variant, int xcd_remap) {
   if (npm_max <= 0) return;
   if (variant == LQ_INTERACTION_REFERENCE_ORDER)
-    hipLaunchKernelGGL(k_interaction_ref, dim3(blocks(npm_max, 256)), dim3(256), 0, s, P, A, B, cb, ce);
+    hipLaunchKernelGGL(k_interaction_ref, dim3(blocks(npm_max, 256)), dim3(256), 0, s, P, A, B, cb, ce,
+                       xcd_remap);
   else if (variant == LQ_INTERACTION_NEIGHBOR_LIST)
     hipLaunchKernelGGL(k_interaction_nl, dim3(blocks(npm_max, NL_WAVE * NL_WAVES)), dim3(NL_WAVE * NL_WAVES), 0, s,
-                       P, A, B, cb, ce);
+                       P, A, B, cb, ce, xcd_remap);
 }
 void launch_contact(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const Buffers& B, const int* cb,
                     const int* ce) {

@@ -178,10 +178,16 @@ class Context:
     def set_tile(self, shape, tz):
         self._chk(self.lib.lq_set_tile(self.h, shape, tz), "lq_set_tile")
 
+    def set_interaction(self, variant, xcd_remap=0):
+        self._chk(self.lib.lq_set_interaction(self.h, variant, xcd_remap), "lq_set_interaction")
+
     def autotune(self, reps=3):
-        sh, tz, ms = C.c_int(), C.c_int(), C.c_double()
-        self._chk(self.lib.lq_autotune(self.h, reps, C.byref(sh), C.byref(tz), C.byref(ms)), "lq_autotune")
-        return sh.value, tz.value, ms.value
+        """Time every interaction variant on the current state, keep the
+        fastest; returns (variant, tile_shape, tile_tz, xcd_remap, ms_per_call)."""
+        v, sh, tz, rm, ms = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        self._chk(self.lib.lq_autotune(self.h, reps, C.byref(v), C.byref(sh), C.byref(tz), C.byref(rm),
+                                       C.byref(ms)), "lq_autotune")
+        return v.value, sh.value, tz.value, rm.value, ms.value
 
     def neighbor_lists(self, max_nb=160):
         npm = self.counts()[1]
@@ -192,7 +198,7 @@ class Context:
         return counts, ids
 
 
-def open_case(case, key_mode=_abi.KEY_LINEAR, variant=_abi.INTERACTION_TILE, device=0, initial_sort=True):
+def open_case(case, key_mode=_abi.KEY_LINEAR, variant=_abi.INTERACTION_REFERENCE_ORDER, device=0, initial_sort=True):
     """Create a context, upload a case and run the initial search/sort
     (loquat.cu:528-531: GridConfig, Abs2Rel, NeighborSearch, SortArraysPrimary)."""
     from . import cases

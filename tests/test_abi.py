@@ -68,3 +68,31 @@ def test_missing_library_fails_loudly(tmp_path):
         assert "no CPU fallback" in str(e)
     else:
         raise AssertionError("expected failure")
+
+
+def _keep_q(r2, i_h):
+    """The reference's pair test q = sqrt(r^2) * (1/h) < 2 in float32
+    (particle_interaction.cu:32-35); numpy's float32 sqrt and multiply are
+    correctly rounded."""
+    r2 = np.asarray(r2, np.float32)
+    return (np.sqrt(r2) * np.float32(i_h)) < np.float32(2.0)
+
+
+def test_interaction_cutoff_equals_sqrt_test():
+    """r^2 <= r2_cut must select exactly the pairs q < 2 selects: checked on
+    every float in a window of 20001 bit patterns around the cut and on random
+    r^2 over [0, 16 h^2], for the case smoothing lengths and a few others."""
+    lib = _abi.load_library()
+    rng = np.random.default_rng(7)
+    hs = [cases.make_params(cases.config(n), lib=lib).h for n in ["tiny", "c1", "c2"]] + [1e-3, 0.0137, 0.5, 3.0]
+    for h in hs:
+        i_h = np.float32(1.0) / np.float32(h)
+        cut = C.c_float()
+        assert lib.lq_interaction_cutoff_sq(C.c_float(i_h), C.byref(cut)) == _abi.OK
+        cu = np.array([cut.value], np.float32).view(np.uint32)[0]
+        window = (np.arange(-10000, 10001, dtype=np.int64) + int(cu)).astype(np.uint32).view(np.float32)
+        np.testing.assert_array_equal(_keep_q(window, i_h), window <= np.float32(cut.value))
+        assert _keep_q(cut.value, i_h) and not _keep_q(np.nextafter(np.float32(cut.value), np.float32(np.inf)), i_h)
+        r2 = (rng.random(200000) * 16.0 / float(i_h) ** 2).astype(np.float32)
+        np.testing.assert_array_equal(_keep_q(r2, i_h), r2 <= np.float32(cut.value))
+    assert lib.lq_interaction_cutoff_sq(C.c_float(0.0), C.byref(C.c_float())) == _abi.ERR_ARG

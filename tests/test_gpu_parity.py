@@ -154,9 +154,11 @@ def test_step_pieces(variant, oracle_mod):
     ctx.close()
 
 
-# (variant, tile shape, tile depth): shape 0 at depth 6 overflows its LDS
-# capacity on most tiles and exercises the global-memory fallback.
-VARIANTS = [(_abi.INTERACTION_REFERENCE_ORDER, None, None), (_abi.INTERACTION_NEIGHBOR_LIST, None, None),
+# (variant, tile shape or XCD remap, tile depth): shape 0 at depth 6
+# overflows its LDS capacity on most tiles and exercises the global-memory
+# fallback.
+VARIANTS = [(_abi.INTERACTION_REFERENCE_ORDER, 0, None), (_abi.INTERACTION_REFERENCE_ORDER, 1, None),
+            (_abi.INTERACTION_NEIGHBOR_LIST, 0, None), (_abi.INTERACTION_NEIGHBOR_LIST, 1, None),
             (_abi.INTERACTION_TILE, 0, 3), (_abi.INTERACTION_TILE, 0, 6), (_abi.INTERACTION_TILE, 1, 7),
             (_abi.INTERACTION_TILE, 2, 13), (_abi.INTERACTION_TILE, 2, 1)]
 
@@ -167,8 +169,10 @@ def test_interaction_variants_identical():
     for v, shape, tz in VARIANTS:
         say(f"[variants] case=small variant={v} shape={shape} tz={tz}")
         ctx = sim.open_case(case, variant=v)
-        if shape is not None:
+        if tz is not None:
             ctx.set_tile(shape, tz)
+        else:
+            ctx.set_interaction(v, shape)
         ctx.step(3)
         ctx.sync()
         outs.append(ctx.snapshot(["accad", "acestn", "spin", "vel", "str", "cmag", "cvar"]))
@@ -185,9 +189,9 @@ def test_autotune_keeps_state():
     b = sim.open_case(case)
     a.step(2)
     b.step(2)
-    shape, tz, ms = b.autotune(reps=2)
-    say(f"  autotune -> shape={shape} tz={tz} ms={ms:.4f}")
-    assert 0 <= shape <= 2 and tz >= 1 and ms > 0
+    variant, shape, tz, remap, ms = b.autotune(reps=2)
+    say(f"  autotune -> variant={variant} shape={shape} tz={tz} remap={remap} ms={ms:.4f}")
+    assert variant in (0, 1, 2) and 0 <= shape <= 2 and tz >= 1 and remap in (0, 1) and ms > 0
     a.step(2)
     b.step(2)
     a.sync()
