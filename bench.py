@@ -80,7 +80,7 @@ def main():
     ap.add_argument("--case", default="c2")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--graph", action="store_true", help="time with hipGraph replay (no per-kernel events)")
-    ap.add_argument("--variant", default="auto", choices=["auto", "tile", "reference", "neighbor_list"],
+    ap.add_argument("--variant", default="auto", choices=["auto", "tile", "reference", "neighbor_list", "pair_list"],
                     help="interaction kernel variant (all bit-identical); auto times each and keeps the fastest")
     ap.add_argument("--remap", type=int, default=0, help="XCD block renumbering for reference/neighbor_list")
     ap.add_argument("--tile", default="", metavar="SHAPE,TZ", help="fix the tile shape/depth instead of autotuning")
@@ -107,7 +107,7 @@ def main():
     print(f"[bench] case={args.case} np={case.np} npb={case.npb} npm={case.npm}", file=sys.stderr, flush=True)
     from retainwall_code_amd import _abi
     names = {_abi.INTERACTION_TILE: "tile", _abi.INTERACTION_REFERENCE_ORDER: "reference",
-             _abi.INTERACTION_NEIGHBOR_LIST: "neighbor_list"}
+             _abi.INTERACTION_NEIGHBOR_LIST: "neighbor_list", _abi.INTERACTION_PAIR_LIST: "pair_list"}
     ids = {v: k for k, v in names.items()}
     variant = ids.get(args.variant, _abi.INTERACTION_REFERENCE_ORDER)
     ctx = sim.open_case(case, variant=variant)
@@ -159,7 +159,8 @@ def main():
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": {"tile": "k_interaction_tile", "reference": "k_interaction_ref",
-                           "neighbor_list": "k_interaction_nl"}[vname], "avg_launch_ms": round(avg_ms, 4),
+                           "neighbor_list": "k_interaction_nl", "pair_list": "k_interaction_plist"}[vname],
+                "avg_launch_ms": round(avg_ms, 4),
                 "algorithmic_bytes_per_launch": bytes_call,
                 "valu_tflops": round(flops_call / (avg_ms * 1e-3) / 1e12, 2),
                 "valu_frac": round(flops_call / (avg_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),

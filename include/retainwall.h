@@ -47,16 +47,21 @@ enum lq_shape { LQ_CUBOID = 0, LQ_CIRCLE = 1 };      /* types.h:49 */
  * bit for bit.  MORTON interleaves the cell coordinates instead (extension). */
 enum lq_key_mode { LQ_KEY_LINEAR = 0, LQ_KEY_MORTON = 1 };
 
-/* Interaction kernel variant.  All three produce bit-identical results.
+/* Interaction kernel variant.  All four produce bit-identical results.
  * REFERENCE_ORDER (default) walks the 27 cells per thread exactly as
  * particle_interaction.cu:117-140; TILE stages each cell-column tile's
  * neighbourhood in LDS and scans it from there; NEIGHBOR_LIST scans global
- * memory per lane and compacts hits into LDS lists before evaluating them.
- * lq_autotune picks the fastest for the current state. */
+ * memory per lane and compacts hits into LDS lists before evaluating them;
+ * PAIR_LIST scans once per step into per-slot pair lists in HBM (cut-off
+ * plus a 1% skin, traversal order kept) that both interaction calls of the
+ * step evaluate, falling back to the cell scan on the device when the
+ * predictor moved a particle further than the skin allows or a list
+ * overflowed.  lq_autotune picks the fastest for the current state. */
 enum lq_interaction_variant {
   LQ_INTERACTION_REFERENCE_ORDER = 0,
   LQ_INTERACTION_TILE = 1,
-  LQ_INTERACTION_NEIGHBOR_LIST = 2
+  LQ_INTERACTION_NEIGHBOR_LIST = 2,
+  LQ_INTERACTION_PAIR_LIST = 3
 };
 
 /* RigidMotion, types.h:66-76 (the per-step imposed motion of one body). */
@@ -259,7 +264,7 @@ int lq_synchronize(lq_ctx* ctx);
 enum lq_kernel_class {
   LQ_K_NEIGHBOR_SEARCH = 0, LQ_K_SORT_ARRAYS = 1, LQ_K_ADAMI = 2, LQ_K_CORNER = 3,
   LQ_K_INTERACTION = 4, LQ_K_CONTACT = 5, LQ_K_PREDICTOR = 6, LQ_K_CORRECTOR = 7,
-  LQ_K_SHEPARD = 8, LQ_K_CLASS_COUNT = 9
+  LQ_K_SHEPARD = 8, LQ_K_PAIR_LIST_BUILD = 9, LQ_K_CLASS_COUNT = 10
 };
 int lq_enable_timing(lq_ctx* ctx, int enable);
 /* STREAM-style copy of `bytes` between two device scratch buffers, `reps`
@@ -274,15 +279,26 @@ int lq_kernel_times(lq_ctx* ctx, double* ms, int64_t* launches, int n);
  * reference has one fixed 128-thread launch). */
 int lq_set_tile(lq_ctx* ctx, int shape, int tz);
 /* Switch the interaction variant of a live context; xcd_remap != 0 makes
- * REFERENCE_ORDER and NEIGHBOR_LIST hand each of the 8 XCDs one contiguous
- * run of particle blocks (hardware places block b on XCD b % 8).  No
- * reference counterpart. */
+ * the particle-parallel variants hand each of the 8 XCDs one contiguous run
+ * of particle blocks (hardware places block b on XCD b % 8).  No reference
+ * counterpart. */
 int lq_set_interaction(lq_ctx* ctx, int variant, int xcd_remap);
+/* PAIR_LIST capacity (entries per soil slot, 1..160; default 160) and skin
+ * (list radius (1 + skin) * 2h, 0..0.1; default 0.01).  The lists are
+ * reused in the corrector stage while no particle moved more than
+ * 0.4 * skin * 2h; otherwise, or when a slot overflows, the step falls back
+ * to the cell scan on the device.  No reference counterpart. */
+int lq_set_pair_list(lq_ctx* ctx, int cap, float skin);
+/* After a step: whether the last pair-list build overflowed, the largest
+ * particle displacement since that build and the displacement the skin
+ * allows (max_disp > disp_limit means the corrector stage used the cell
+ * scan).  Synchronises. */
+int lq_pair_list_status(lq_ctx* ctx, int* overflow, float* max_disp, float* disp_limit);
 /* lq_autotune times every interaction variant (reference order, neighbour
- * lists, each tile shape over a few depths) on the current state, makes the
- * fastest the active one and reports it (shape/tz meaningful for TILE,
- * xcd_remap for REFERENCE_ORDER and NEIGHBOR_LIST). */
-int lq_autotune(lq_ctx* ctx, int reps, int* variant, int* shape, int* tz, int* xcd_remap, double* ms_per_call);
+ * lists, pair lists, each tile shape over a few depths) on the current state
+ * as the two interaction calls of one step, makes the fastest the active one
+ * and reports it (shape/tz meaningful for TILE, xcd_remap for the others). */
+int lq_autotune(lq_ctx* ctx, int reps, int* variant, int* shape, int* tz, int* xcd_remap, double* ms_per_step);
 
 /* Squared pair cut-off used by every neighbour kernel: the largest float
  * r2_cut with fl(fl(sqrt(r2_cut)) * i_h) < 2, so r^2 <= r2_cut is exactly

@@ -21,7 +21,7 @@ RIGID, DP_EPP, MUI_DP_EPP, HYPO_BASIC, HYPO_CS = range(5)
 FIX, NONFIX = 0, 1
 CUBOID, CIRCLE = 0, 1
 KEY_LINEAR, KEY_MORTON = 0, 1
-INTERACTION_REFERENCE_ORDER, INTERACTION_TILE, INTERACTION_NEIGHBOR_LIST = 0, 1, 2
+INTERACTION_REFERENCE_ORDER, INTERACTION_TILE, INTERACTION_NEIGHBOR_LIST, INTERACTION_PAIR_LIST = 0, 1, 2, 3
 
 # lq_field
 (F_CODES, F_CELLPOS, F_POS, F_POS_ORI, F_VEL, F_IDM, F_POS_INI, F_VEL_INI, F_ACCAD, F_SPIN,
@@ -57,7 +57,7 @@ FIELDS = {
 
 # lq_kernel_class
 KERNEL_CLASSES = ["neighbor_search", "sort_arrays", "adami", "corner", "interaction",
-                  "contact", "predictor", "corrector", "shepard"]
+                  "contact", "predictor", "corrector", "shepard", "pair_list_build"]
 
 
 class RigidMotion(C.Structure):
@@ -123,7 +123,7 @@ EXPORTS = [
     "lq_sort_arrays_primary", "lq_sort_arrays_all", "lq_rigid_body_update", "lq_adami_boundary",
     "lq_find_corner_particle", "lq_stress_regularization", "lq_particle_interaction",
     "lq_predictor", "lq_corrector", "lq_step", "lq_wall_force", "lq_positions_absolute",
-    "lq_get_counts", "lq_synchronize", "lq_stress_saving_regularization", "lq_stream_copy", "lq_enable_timing", "lq_kernel_times", "lq_neighbor_lists", "lq_set_tile", "lq_set_interaction", "lq_autotune", "lq_interaction_cutoff_sq",
+    "lq_get_counts", "lq_synchronize", "lq_stress_saving_regularization", "lq_stream_copy", "lq_enable_timing", "lq_kernel_times", "lq_neighbor_lists", "lq_set_tile", "lq_set_interaction", "lq_set_pair_list", "lq_pair_list_status", "lq_autotune", "lq_interaction_cutoff_sq",
 ]
 
 
@@ -162,6 +162,8 @@ def _proto(lib):
     lib.lq_neighbor_lists.argtypes = [vp, C.c_int, vp, vp, C.c_size_t]
     lib.lq_set_tile.argtypes = [vp, C.c_int, C.c_int]
     lib.lq_set_interaction.argtypes = [vp, C.c_int, C.c_int]
+    lib.lq_set_pair_list.argtypes = [vp, C.c_int, C.c_float]
+    lib.lq_pair_list_status.argtypes = [vp, P(C.c_int), P(C.c_float), P(C.c_float)]
     lib.lq_autotune.argtypes = [vp, C.c_int, P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_double)]
     lib.lq_interaction_cutoff_sq.argtypes = [C.c_float, P(C.c_float)]
 

@@ -194,6 +194,10 @@ extern "C" int lq_interaction_cutoff_sq(float i_h, float* r2_cut) {
   return LQ_OK;
 }
 
+// Pair-list capacity per soil slot (a regular lattice at h = 1.3 dr gives
+// about 80 pairs inside 2h; overflow falls back to the cell scan).
+constexpr int kPairListCap = 160;
+
 static void fill_static(const lq_params& p, DevParams& d) {
   std::memset(&d, 0, sizeof(d));
   d.np = p.np; d.npb = p.npb;
@@ -204,6 +208,17 @@ static void fill_static(const lq_params& p, DevParams& d) {
   d.h = p.h; d.i_h = p.i_h; d.dr = p.dr; d.vol_def = p.vol_def; d.bt = p.bound_thickness; d.eta = p.eta;
   d.two_h = 2.0f * p.h;
   lq_interaction_cutoff_sq(p.i_h, &d.r2cut);
+  {
+    // List radius 1.01 * 2h.  A pair left out at the build is at least
+    // 1.01 * 2h apart (up to float rounding, ~1e-6 relative); after moves of
+    // at most 0.004 * 2h per particle it is still >= 1.002 * 2h apart, so it
+    // fails the r^2 <= r2cut test exactly as the cell scan would.
+    const float rl = d.two_h * (1.0f + 0.01f), dl = d.two_h * (0.4f * 0.01f);
+    d.r2list = rl * rl;
+    d.disp2_lim = dl * dl;
+    d.plist_cap = kPairListCap;
+    d.plist_stride = std::max(1, p.npm);
+  }
   d.gx = p.gravity[0]; d.gy = p.gravity[1]; d.gz = p.gravity[2];
   d.damping = p.damping;
   d.knorm = p.kernel_normalization_par; d.kgrad = p.kernel_gradient_normalization_par;
@@ -298,6 +313,10 @@ extern "C" int lq_create(const lq_params* par, int device, lq_ctx** out) {
   st |= dalloc(c, &B.s3, n); st |= dalloc(c, &B.fdef, n); st |= dalloc(c, &B.deferred, n);
   st |= dalloc(c, &B.str_tmp_a, n); st |= dalloc(c, &B.str_tmp_b, n);
   st |= dalloc(c, &B.sr_a, n); st |= dalloc(c, &B.sr_b, n);
+  if (par->np < (1 << 25)) {  // pair-list entries hold the slot in 25 bits
+    st |= dalloc(c, &B.plist, size_t(kPairListCap) * size_t(std::max(1, par->npm)));
+    st |= dalloc(c, &B.pcount, size_t(std::max(1, par->npm)));
+  }
   st |= dalloc(c, &B.keys, n); st |= dalloc(c, &B.vals, n);
   st |= dalloc(c, &B.keys_alt, n); st |= dalloc(c, &B.vals_alt, n);
   st |= dalloc(c, &B.cell_beg, size_t(par->ncell)); st |= dalloc(c, &B.cell_end, size_t(par->ncell));
@@ -694,7 +713,11 @@ static int enq_shepard(lq_ctx* c) {
   return 0;
 }
 static void enq_interaction_kernel(lq_ctx* c) {
-  if (c->par.interaction_variant == LQ_INTERACTION_TILE) {
+  if (c->par.interaction_variant == LQ_INTERACTION_PAIR_LIST) {
+    launch_interaction_plist(c->stream, c->dP, c->npm0, c->B.cur, c->B, c->B.cell_beg, c->B.cell_end, c->xcd_remap);
+    launch_interaction(c->stream, c->dP, c->npm0, c->B.cur, c->B, c->B.cell_beg, c->B.cell_end,
+                       LQ_INTERACTION_PAIR_LIST, c->xcd_remap);  // guarded cell-scan fallback
+  } else if (c->par.interaction_variant == LQ_INTERACTION_TILE) {
     if (c->npm0 > 0)
       launch_interaction_tile(c->stream, c->dP, c->par.cell_num[0], c->par.cell_num[1], c->par.cell_num[2], c->B.cur,
                               c->B, c->B.cell_beg, c->B.cell_end, c->tile_shape, c->tile_tz);
@@ -703,9 +726,20 @@ static void enq_interaction_kernel(lq_ctx* c) {
                        c->par.interaction_variant, c->xcd_remap);
   }
 }
-static int enq_interaction(lq_ctx* c) {
-  launch_reset_deferred(c->stream, c->dP);
+static void enq_plist_build(lq_ctx* c) {
+  launch_plist_build(c->stream, c->dP, c->npm0, c->B.cur, c->B, c->B.cell_beg, c->B.cell_end, c->xcd_remap);
+}
+// stage 0: first interaction after a sort (a pair-list variant builds its
+// lists); stage 1: the corrector-stage call of the same step (lists reused
+// when still exact).
+static int enq_interaction(lq_ctx* c, int stage) {
+  const bool build = stage == 0 && c->par.interaction_variant == LQ_INTERACTION_PAIR_LIST;
+  launch_reset_deferred(c->stream, c->dP, build ? 1 : 0);
   launch_pair_prep(c->stream, c->dP, c->par.npb + c->npm0, c->B.cur, c->B);
+  if (build) {
+    Section s(c, LQ_K_PAIR_LIST_BUILD);
+    enq_plist_build(c);
+  }
   {
     Section s(c, LQ_K_INTERACTION);
     enq_interaction_kernel(c);
@@ -718,7 +752,8 @@ static int enq_interaction(lq_ctx* c) {
 }
 static int enq_predictor(lq_ctx* c) {
   Section s(c, LQ_K_PREDICTOR);
-  launch_predictor(c->stream, c->dP, c->par.np, c->B.cur, c->B);
+  launch_predictor(c->stream, c->dP, c->par.np, c->B.cur, c->B,
+                   c->par.interaction_variant == LQ_INTERACTION_PAIR_LIST ? 1 : 0);
   return 0;
 }
 static int enq_corrector(lq_ctx* c) {
@@ -745,7 +780,7 @@ extern "C" int lq_rigid_body_update(lq_ctx* c) {
 extern "C" int lq_adami_boundary(lq_ctx* c) { NEED_UPLOAD(c); hipSetDevice(c->device); enq_adami(c); return check_launch(c); }
 extern "C" int lq_find_corner_particle(lq_ctx* c) { NEED_UPLOAD(c); hipSetDevice(c->device); enq_corner(c); return check_launch(c); }
 extern "C" int lq_stress_regularization(lq_ctx* c) { NEED_UPLOAD(c); hipSetDevice(c->device); enq_shepard(c); return check_launch(c); }
-extern "C" int lq_particle_interaction(lq_ctx* c) { NEED_UPLOAD(c); hipSetDevice(c->device); enq_interaction(c); return check_launch(c); }
+extern "C" int lq_particle_interaction(lq_ctx* c) { NEED_UPLOAD(c); hipSetDevice(c->device); enq_interaction(c, 0); return check_launch(c); }
 extern "C" int lq_predictor(lq_ctx* c) { NEED_UPLOAD(c); hipSetDevice(c->device); enq_predictor(c); return check_launch(c); }
 extern "C" int lq_corrector(lq_ctx* c) { NEED_UPLOAD(c); hipSetDevice(c->device); enq_corrector(c); return check_launch(c); }
 
@@ -762,11 +797,11 @@ static int enq_step(lq_ctx* c) {
   enq_adami(c);
   enq_corner(c);
   if (shep != 0 && c->host_step % shep == 0 && c->host_step != 0) enq_shepard(c);
-  enq_interaction(c);
+  enq_interaction(c, 0);
   enq_predictor(c);
   enq_adami(c);
   enq_corner(c);
-  enq_interaction(c);
+  enq_interaction(c, 1);
   enq_corrector(c);
   launch_advance(c->stream, c->dP);
   c->host_step++;
@@ -933,11 +968,51 @@ extern "C" int lq_set_tile(lq_ctx* c, int shape, int tz) {
 // NEIGHBOR_LIST, whether blocks are renumbered per XCD.
 extern "C" int lq_set_interaction(lq_ctx* c, int variant, int xcd_remap) {
   if (!c) return LQ_ERR_ARG;
-  if (variant < LQ_INTERACTION_REFERENCE_ORDER || variant > LQ_INTERACTION_NEIGHBOR_LIST)
+  if (variant < LQ_INTERACTION_REFERENCE_ORDER || variant > LQ_INTERACTION_PAIR_LIST)
     return fail(c, LQ_ERR_ARG, "lq_set_interaction: variant out of range");
+  if (variant == LQ_INTERACTION_PAIR_LIST && !c->B.plist)
+    return fail(c, LQ_ERR_ARG, "lq_set_interaction: pair lists need fewer than 2^25 particles");
   c->par.interaction_variant = variant;
   c->xcd_remap = xcd_remap ? 1 : 0;
   c->layout_version++;  // re-capture graphs
+  return LQ_OK;
+}
+
+// Pair-list capacity and skin: the list radius is (1 + skin) * 2h and the
+// lists stay in use while no particle moves more than 0.4 * skin * 2h.
+extern "C" int lq_set_pair_list(lq_ctx* c, int cap, float skin) {
+  if (!c) return LQ_ERR_ARG;
+  if (!c->B.plist) return fail(c, LQ_ERR_ARG, "lq_set_pair_list: pair lists need fewer than 2^25 particles");
+  if (cap < 1 || cap > kPairListCap || !(skin >= 0.f && skin <= 0.1f))
+    return fail(c, LQ_ERR_ARG, "lq_set_pair_list: cap in [1, 160], skin in [0, 0.1]");
+  hipSetDevice(c->device);
+  const float rl = c->hp.two_h * (1.0f + skin), dl = c->hp.two_h * (0.4f * skin);
+  c->hp.r2list = std::max(rl * rl, c->hp.r2cut);
+  c->hp.disp2_lim = dl * dl;
+  c->hp.plist_cap = cap;
+  const size_t off = offsetof(DevParams, r2list);
+  const size_t len = offsetof(DevParams, plist_stride) - off;  // r2list, disp2_lim, plist_cap
+  HIPCHECK(c, hipMemcpyAsync(reinterpret_cast<char*>(c->dP) + off, reinterpret_cast<const char*>(&c->hp) + off, len,
+                             hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  c->layout_version++;
+  return LQ_OK;
+}
+
+// State of the most recent pair-list build: overflow flag, the largest soil
+// displacement since the build and the displacement the skin allows
+// (synchronises).
+extern "C" int lq_pair_list_status(lq_ctx* c, int* overflow, float* max_disp, float* disp_limit) {
+  if (!c) return LQ_ERR_ARG;
+  hipSetDevice(c->device);
+  DevParams d;
+  int st = pull_params(c, d);
+  if (st) return st;
+  float d2;
+  std::memcpy(&d2, &d.max_disp2_bits, sizeof d2);
+  if (overflow) *overflow = d.plist_bad;
+  if (max_disp) *max_disp = std::sqrt(d2);
+  if (disp_limit) *disp_limit = std::sqrt(d.disp2_lim);
   return LQ_OK;
 }
 
@@ -961,6 +1036,7 @@ extern "C" int lq_autotune(lq_ctx* c, int reps, int* variant_out, int* shape_out
   for (int remap = 0; remap < 2; remap++) {
     cands.push_back({LQ_INTERACTION_REFERENCE_ORDER, c->tile_shape, c->tile_tz, remap});
     cands.push_back({LQ_INTERACTION_NEIGHBOR_LIST, c->tile_shape, c->tile_tz, remap});
+    if (c->B.plist) cands.push_back({LQ_INTERACTION_PAIR_LIST, c->tile_shape, c->tile_tz, remap});
   }
   for (int sh = 0; sh < interaction_tile_shapes(); sh++) {
     const int waves = interaction_tile_waves(sh);
@@ -978,13 +1054,17 @@ extern "C" int lq_autotune(lq_ctx* c, int reps, int* variant_out, int* shape_out
     c->tile_shape = k.shape;
     c->tile_tz = k.tz;
     c->xcd_remap = k.remap;
-    launch_reset_deferred(c->stream, c->dP);
-    enq_interaction_kernel(c);  // warm-up
-    hipEventRecord(e0, c->stream);
-    for (int r = 0; r < reps; r++) {
-      launch_reset_deferred(c->stream, c->dP);
+    auto step_calls = [&]() {  // the two interaction calls of one step
+      const bool pl = k.variant == LQ_INTERACTION_PAIR_LIST;
+      launch_reset_deferred(c->stream, c->dP, pl ? 1 : 0);
+      if (pl) enq_plist_build(c);
       enq_interaction_kernel(c);
-    }
+      launch_reset_deferred(c->stream, c->dP, 0);
+      enq_interaction_kernel(c);
+    };
+    step_calls();  // warm-up
+    hipEventRecord(e0, c->stream);
+    for (int r = 0; r < reps; r++) step_calls();
     hipEventRecord(e1, c->stream);
     hipError_t e = hipEventSynchronize(e1);
     if (e != hipSuccess) {

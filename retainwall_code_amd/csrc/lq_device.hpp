@@ -71,7 +71,25 @@ __device__ __forceinline__ int xcd_block(int remap) {
   return x * q + min(x, r) + (b >> 3);
 }
 
+// True when the pair lists built at the start of the step are still exact:
+// no soil slot overflowed its capacity and no particle moved more than the
+// skin allows since the build.
+__device__ __forceinline__ bool plist_exact(const DevParams* __restrict__ P) {
+  return P->plist_bad == 0 && __uint_as_float(P->max_disp2_bits) <= P->disp2_lim;
+}
+
 // ------------------------------------------------------- interaction ----
+// Constants of the pair sum, loaded once per thread so the inner loops hold
+// them in registers.
+struct PairConst {
+  float knorm, kgrad, vol_def, h, eta, alpha_cs, art_p, kernel_dr, art_exp;
+  int viscosity_type;
+};
+__device__ __forceinline__ PairConst pair_const(const DevParams* __restrict__ P) {
+  return {P->knorm, P->kgrad, P->vol_def, P->h, P->eta, P->alpha_cs, P->art_p, P->kernel_dr, P->art_exp,
+          P->viscosity_type};
+}
+
 // Per-pair terms of cuParticleInteractionMaterialLoopSolution,
 // particle_interaction.cu:26-115, accumulated into the per-particle sums.
 struct IAcc {
@@ -80,7 +98,7 @@ struct IAcc {
   float gv[9];  // xx xy xz yx yy yz zx zy zz
 };
 
-__device__ __forceinline__ void pair_terms(const DevParams* __restrict__ P, IAcc& S, float3 d, float r, float q,
+__device__ __forceinline__ void pair_terms(const PairConst& P, IAcc& S, float3 d, float r, float q,
                                            float3 vi, float rhoi, Sym sri, float artpi, int shift, bool bound,
                                            float4 vj, float wj, float4 sra, float4 srb) {
   // wj = pos.w of the neighbour: |wj| is a soil particle's mass, and wj < 0
@@ -88,11 +106,11 @@ __device__ __forceinline__ void pair_terms(const DevParams* __restrict__ P, IAcc
   // per-particle terms from k_pair_prep: sigma/rho^2 (xx yy zz xy | xz yz),
   // m/rho and trace(sigma/rho^2), each rounded exactly as the inline form.
   const bool contactj = wj < 0.f;
-  float ker = kW(P->knorm, q);
-  float dwdr = kG(P->kgrad, q);
+  float ker = kW(P.knorm, q);
+  float dwdr = kG(P.kgrad, q);
   float3 g = f3mul(d, dwdr);
   float rhoj = vj.w;
-  float mj = bound ? P->vol_def * rhoj : fabsf(wj);
+  float mj = bound ? P.vol_def * rhoj : fabsf(wj);
   float3 dv = f3sub(vi, ld3(vj));
   if (contactj && shift == 1) dv = make_float3(0.f, 0.f, 0.f);
   const float volj = srb.z;
@@ -107,20 +125,20 @@ __device__ __forceinline__ void pair_terms(const DevParams* __restrict__ P, IAcc
   Sym st = sadd(sri, srj);
   float3 f = smatvec(st, g);
   S.as.x = fmaf(f.x, mj, S.as.x); S.as.y = fmaf(f.y, mj, S.as.y); S.as.z = fmaf(f.z, mj, S.as.z);
-  if (P->viscosity_type == 1) {
+  if (P.viscosity_type == 1) {
     float visc = fmaf(dv.z, d.z, fmaf(dv.y, d.y, dv.x * d.x));
     if (visc < 0.f) {
-      float phi = P->h * visc / fmaf(r, r, P->eta);
-      float artv = P->alpha_cs * phi / (0.5f * (rhoi + rhoj));
+      float phi = P.h * visc / fmaf(r, r, P.eta);
+      float artv = P.alpha_cs * phi / (0.5f * (rhoi + rhoj));
       float s = mj * artv;
       S.av.x = fmaf(g.x, s, S.av.x); S.av.y = fmaf(g.y, s, S.av.y); S.av.z = fmaf(g.z, s, S.av.z);
     }
   }
-  if (P->art_p != 0.f) {
+  if (P.art_p != 0.f) {
     float trj = srb.w;
-    float artp = artpi + (trj > 0.f ? trj * P->art_p : 0.f);
+    float artp = artpi + (trj > 0.f ? trj * P.art_p : 0.f);
     if (artp > 0.f) {
-      artp = artp * powf(ker / P->kernel_dr, P->art_exp);
+      artp = artp * powf(ker / P.kernel_dr, P.art_exp);
       float s = mj * artp;
       S.aa.x = fmaf(-g.x, s, S.aa.x); S.aa.y = fmaf(-g.y, s, S.aa.y); S.aa.z = fmaf(-g.z, s, S.aa.z);
     }

@@ -52,6 +52,7 @@ __device__ __forceinline__ void tile_home(DevParams* __restrict__ P, const Array
     const float artpi = tri > 0.f ? tri * P->art_p : 0.f;
     const int shift = int(A.cvar[i].z);
     const int zoff = ci.z - zw0;
+    const PairConst K = pair_const(P);
     IAcc S;
     S.accden = 0.f;
     S.as = S.av = S.aa = make_float3(0.f, 0.f, 0.f);
@@ -124,7 +125,7 @@ __device__ __forceinline__ void tile_home(DevParams* __restrict__ P, const Array
           float3 d;
           const float r = sqrtf(pair_r2(pi, pj, cx, cy, cz, two_h, d));
           const float q = r * i_h;
-          pair_terms(P, S, d, r, q, vi, rhoi, sri, artpi, shift, bsel == 0, A.vel[slot], pj.w, B.sr_a[slot],
+          pair_terms(K, S, d, r, q, vi, rhoi, sri, artpi, shift, bsel == 0, A.vel[slot], pj.w, B.sr_a[slot],
                      B.sr_b[slot]);
         }
       }

@@ -48,11 +48,19 @@ struct DevParams {
   float h, i_h, dr, vol_def, bt, eta, two_h, r2cut;
   float gx, gy, gz, damping, knorm, kgrad, kernel_zero, kernel_dr;
   float art_exp, alpha_cs, art_p, cfl, cs, band, half_dr, mu;
+  // pair lists: squared list radius (cut-off plus skin), largest squared
+  // displacement between build and reuse that keeps them exact, capacity
+  // (entries per soil slot) and column stride (soil slots)
+  float r2list, disp2_lim;
+  int plist_cap, plist_stride;
   float model_par[LQ_MODEL_NUMBER_MAX][LQ_MATERIAL_SIZE_MAX];
   HypoConst hypo[LQ_MODEL_NUMBER_MAX];
   // ---- dynamic (device-updated) ----
   int npm, np_delete_m, err, n_deferred;
   int sort_changed, pad2, pad3, pad4;
+  unsigned max_disp2_bits;  // largest squared soil displacement since the pair-list build (float bits)
+  int plist_bad;            // a soil slot overflowed its pair-list capacity
+  int pad5, pad6;
   float time, dt;
   double time_d;
   long long step;
@@ -87,6 +95,8 @@ struct Buffers {
   float2* str_tmp_b;
   float4* sr_a;      // per-slot sigma/rho^2 (xx yy zz xy), written by k_pair_prep
   float4* sr_b;      // per-slot sigma/rho^2 (xz yz), m/rho, trace
+  int* plist;        // pair lists, entry k of soil slot s at plist[k * stride + s]
+  int* pcount;       // pair-list length per soil slot
   // grid / sort
   int *keys, *vals, *keys_alt, *vals_alt, *part_cel, *part_idx;
   int *cell_beg, *cell_end;

@@ -15,13 +15,17 @@ void launch_adami(hipStream_t s, const DevParams* P, int npb, Arrays A, const in
 void launch_corner(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const int* cb, const int* ce);
 void launch_shepard(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const int* cb, const int* ce,
                     float4* oa, float2* ob, bool copy_back = true);
-void launch_reset_deferred(hipStream_t s, DevParams* P);
+void launch_reset_deferred(hipStream_t s, DevParams* P, int list_build);
+void launch_plist_build(hipStream_t s, DevParams* P, int npm_max, Arrays A, const Buffers& B, const int* cb,
+                        const int* ce, int xcd_remap);
+void launch_interaction_plist(hipStream_t s, DevParams* P, int npm_max, Arrays A, const Buffers& B, const int* cb,
+                              const int* ce, int xcd_remap);
 void launch_pair_prep(hipStream_t s, const DevParams* P, int n, Arrays A, const Buffers& B);
 void launch_interaction(hipStream_t s, DevParams* P, int npm_max, Arrays A, const Buffers& B, const int* cb,
                         const int* ce, int variant, int xcd_remap);
 void launch_contact(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const Buffers& B, const int* cb,
                     const int* ce);
-void launch_predictor(hipStream_t s, const DevParams* P, int np, Arrays A, const Buffers& B);
+void launch_predictor(hipStream_t s, DevParams* P, int np, Arrays A, const Buffers& B, int track_disp);
 void launch_corrector(hipStream_t s, DevParams* P, int npb, int npm_max, int n_motion, Arrays A, const Buffers& B);
 void launch_wall_force(hipStream_t s, const DevParams* P, int np, Arrays A, const float box[3], double* partial,
                        int nblocks, double* out);

@@ -62,7 +62,16 @@ __global__ void k_advance(DevParams* P) {
   P->time_d += double(P->dt);
   P->step += 1;
 }
-__global__ void k_reset_deferred(DevParams* P) { P->n_deferred = 0; }
+// Clears the deferred-contact counter before each interaction; a pair-list
+// build also clears the list's overflow flag and the displacement maximum.
+__global__ void k_reset_deferred(DevParams* P, int list_build) {
+  P->n_deferred = 0;
+  if (list_build) {
+    P->plist_bad = 0;
+    P->max_disp2_bits = 0u;
+  }
+}
+
 
 // RigidBodyUpdate, adami_boundary.cu:175-203 (one thread).
 __global__ void k_rigid_body_update(DevParams* P) {
@@ -303,9 +312,12 @@ __global__ void __launch_bounds__(256) k_pair_prep(const DevParams* __restrict__
 
 // Reference-order variant: each thread walks its 27 cells (cz, cy, cx;
 // boundary range then soil range) exactly as particle_interaction.cu:117-140.
+// With guard set it is the pair-list variant's fallback and runs only when
+// the lists are not valid.
 __global__ void __launch_bounds__(256) k_interaction_ref(DevParams* __restrict__ P, Arrays A, Buffers B,
                                                          const int* __restrict__ cbeg, const int* __restrict__ cend,
-                                                         int remap) {
+                                                         int remap, int guard) {
+  if (guard && plist_exact(P)) return;
   int idx = xcd_block(remap) * blockDim.x + threadIdx.x;
   if (idx >= P->npm) return;
   const int i = P->npb + idx;
@@ -320,6 +332,7 @@ __global__ void __launch_bounds__(256) k_interaction_ref(DevParams* __restrict__
   const float artpi = tri > 0.f ? tri * P->art_p : 0.f;
   const int shift = int(A.cvar[i].z);
   const float two_h = P->two_h, i_h = P->i_h, r2cut = P->r2cut;
+  const PairConst K = pair_const(P);
   IAcc S;
   S.accden = 0.f;
   S.as = S.av = S.aa = make_float3(0.f, 0.f, 0.f);
@@ -341,28 +354,28 @@ __global__ void __launch_bounds__(256) k_interaction_ref(DevParams* __restrict__
             if (!(r2 <= r2cut)) continue;
             float r = sqrtf(r2);
             float q = r * i_h;
-            pair_terms(P, S, d, r, q, vi, rhoi, sri, artpi, shift, bound, A.vel[j], pj.w, B.sr_a[j], B.sr_b[j]);
+            pair_terms(K, S, d, r, q, vi, rhoi, sri, artpi, shift, bound, A.vel[j], pj.w, B.sr_a[j], B.sr_b[j]);
           }
         }
       }
   interaction_epilogue(P, A, B, i, S, vi);
 }
 
-// Neighbour-list variant.  Each wave keeps a per-lane list of neighbour
-// slots in LDS: lanes scan candidates in reference order, append hits
-// (pair index j and cell-offset code) and, whenever any lane's list is full
-// or the scan ends, the wave evaluates the stored pairs with every lane busy.
-// Summation order per particle is the reference order, as in k_interaction_ref.
+// Neighbour-list variant.  The wave walks the 27 cells x {wall, soil}
+// ranges in reference order in lockstep (loop indices are wave-uniform, so
+// the lanes of one home cell read the same candidate at the same time).
+// Each range is tested in chunks of NL_CAP candidates into a per-lane hit
+// mask, and the chunk's hits are then evaluated with every lane taking its
+// own next hit, so the expensive pair terms run on the hits alone instead
+// of on every candidate some lane of the wave accepts.  Each particle still
+// sums its pairs in reference order.
 constexpr int NL_WAVE = 64;
 constexpr int NL_WAVES = 4;
-constexpr int NL_CAP = 24;
+constexpr int NL_CAP = 32;
 
 __global__ void __launch_bounds__(NL_WAVE * NL_WAVES) k_interaction_nl(DevParams* __restrict__ P, Arrays A, Buffers B,
                                                                       const int* __restrict__ cbeg,
                                                                       const int* __restrict__ cend, int remap) {
-  __shared__ int s_list[NL_WAVES][NL_CAP][NL_WAVE];
-  const int lane = threadIdx.x & (NL_WAVE - 1);
-  const int wv = threadIdx.x / NL_WAVE;
   const int idx = xcd_block(remap) * blockDim.x + threadIdx.x;
   const bool active = idx < P->npm;
   const int i = P->npb + (active ? idx : 0);
@@ -383,64 +396,38 @@ __global__ void __launch_bounds__(NL_WAVE * NL_WAVES) k_interaction_nl(DevParams
   S.as = S.av = S.aa = make_float3(0.f, 0.f, 0.f);
 #pragma unroll
   for (int k = 0; k < 9; k++) S.gv[k] = 0.f;
-  // scan state: cell counter 0..53 (27 cells x {boundary, soil}) and slot
-  int cstep = active ? 0 : 54;
-  int j = 0, jend = 0;
-  int n = 0;
-  while (true) {
-    // fill phase: scan until list full or scan exhausted
-    while (n < NL_CAP) {
-      if (j >= jend) {
-        // advance to next non-empty range
-        bool found = false;
-        while (cstep < 54) {
-          int cell = cstep >> 1, bsel = cstep & 1;
-          int cz = cell / 9 - 1, cy = (cell / 3) % 3 - 1, cx = cell % 3 - 1;
-          cstep++;
-          int jx = ci.x + cx, jy = ci.y + cy, jz = ci.z + cz;
-          if (!in_grid(P, jx, jy, jz)) continue;
-          int c = lin_key(P, jx, jy, jz) + (bsel ? ncm : 0);
-          int b = cbeg[c];
-          if (b < 0) continue;
-          j = b;
-          jend = cend[c];
-          found = true;
-          break;
-        }
-        if (!found) break;
-      }
-      int cur = cstep - 1;
-      int cell = cur >> 1;
-      int cz = cell / 9 - 1, cy = (cell / 3) % 3 - 1, cx = cell % 3 - 1;
-      float3 d;
-      const float r2 = pair_r2(pi, A.pos[j], cx, cy, cz, two_h, d);
-      if (r2 <= r2cut) {
-        s_list[wv][n][lane] = int((unsigned(j) << 6) | unsigned(cur));
-        n++;
-      }
-      j++;
+#pragma unroll 1
+  for (int r = 0; r < 54; r++) {
+    const int cell = r >> 1, pass = r & 1;
+    const int cz = cell / 9 - 1, cy = (cell / 3) % 3 - 1, cx = cell % 3 - 1;
+    const int jx = ci.x + cx, jy = ci.y + cy, jz = ci.z + cz;
+    int j = 0, e = 0;
+    if (active && in_grid(P, jx, jy, jz)) {
+      const int c = lin_key(P, jx, jy, jz) + (pass ? ncm : 0);
+      j = cbeg[c];
+      e = cend[c];
     }
-    // evaluate phase
-    int nmax = n;
-    for (int o = 32; o > 0; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o));
-    for (int k = 0; k < nmax; k++) {
-      if (k < n) {
-        int e = s_list[wv][k][lane];
-        int jj = int(unsigned(e) >> 6), cur = e & 63;
-        int cell = cur >> 1;
-        bool bound = (cur & 1) == 0;
-        int cz = cell / 9 - 1, cy = (cell / 3) % 3 - 1, cx = cell % 3 - 1;
+    while (j < e) {
+      const int m = min(e - j, NL_CAP);
+      unsigned mask = 0u;
+#pragma unroll 1
+      for (int t = 0; t < m; t++) {
         float3 d;
+        if (pair_r2(pi, A.pos[j + t], cx, cy, cz, two_h, d) <= r2cut) mask |= 1u << t;
+      }
+#pragma unroll 1
+      while (mask) {
+        const int jj = j + __ffs(mask) - 1;
+        mask &= mask - 1u;
         const float4 pj = A.pos[jj];
+        float3 d;
         const float r = sqrtf(pair_r2(pi, pj, cx, cy, cz, two_h, d));
         const float q = r * i_h;
-        pair_terms(P, S, d, r, q, vi, rhoi, sri, artpi, shift, bound, A.vel[jj], pj.w, B.sr_a[jj], B.sr_b[jj]);
+        pair_terms(pair_const(P), S, d, r, q, vi, rhoi, sri, artpi, shift, pass == 0, A.vel[jj], pj.w,
+                   B.sr_a[jj], B.sr_b[jj]);
       }
+      j += m;
     }
-    bool more = (j < jend) || (cstep < 54);
-    n = 0;
-    unsigned long long anymore = __ballot(more);
-    if (anymore == 0ull) break;
   }
   if (active) interaction_epilogue(P, A, B, i, S, vi);
 }
@@ -511,25 +498,25 @@ __global__ void k_contact_commit(const DevParams* __restrict__ P, Arrays A, Buff
 
 // --------------------------------------------------------- predictor ----
 // LoquatSph::Predictor + cuPredictorMaterial, predictor_corrector.cu:52-103:
-// the *_ini snapshot and the half-step update in one pass.
-__global__ void __launch_bounds__(256) k_predictor(const DevParams* __restrict__ P, Arrays A, Buffers B) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P->np) return;
+// the *_ini snapshot and the half-step update in one pass.  Returns the
+// squared displacement of slot i (soil that moves; 0 otherwise).
+__device__ __forceinline__ float predictor_slot(const DevParams* __restrict__ P, Arrays A, const Buffers& B, int i) {
   const int npm = P->npm, npb = P->npb;
   float4 p = A.pos[i], v = A.vel[i];
   B.pos_ini[i] = p;
   B.vel_ini[i] = v;
-  if (npm == 0 || i >= npb + npm) return;
+  if (npm == 0 || i >= npb + npm) return 0.f;
   float4 sa = A.str_a[i], na = A.stn_a[i];
   float2 sb = A.str_b[i], nb = A.stn_b[i];
   B.str_ini_a[i] = sa; B.str_ini_b[i] = sb;
   B.stn_ini_a[i] = na; B.stn_ini_b[i] = nb;
-  if (i < npb) return;
+  if (i < npb) return 0.f;
   int4 cd = A.codes[i];
-  if (cd.y != LQ_NONFIX) return;
+  if (cd.y != LQ_NONFIX) return 0.f;
   const float hdt = 0.5f * P->dt;
   float4 a = B.accad[i];
   v = make_float4(fmaf(a.x, hdt, v.x), fmaf(a.y, hdt, v.y), fmaf(a.z, hdt, v.z), fmaf(a.w, hdt, v.w));
+  const float4 p0 = p;
   p = make_float4(fmaf(v.x, hdt, p.x), fmaf(v.y, hdt, p.y), fmaf(v.z, hdt, p.z), p.w);
   const float* mp = P->model_par[cd.w];
   float rho = fmaxf(v.w, mp[13]);
@@ -547,6 +534,24 @@ __global__ void __launch_bounds__(256) k_predictor(const DevParams* __restrict__
   float4 id = A.idm[i];
   id.x = 100.f;
   A.idm[i] = id;
+  const float dx = p.x - p0.x, dy = p.y - p0.y, dz = p.z - p0.z;
+  return dx * dx + dy * dy + dz * dz;
+}
+
+// With track set, the wave maximum of the squared displacement goes to
+// max_disp2_bits for the pair lists (non-negative floats order like their
+// bit patterns).  The running maximum only grows, so a wave whose maximum a
+// plain read already covers skips the atomic.
+__global__ void __launch_bounds__(256) k_predictor(DevParams* __restrict__ P, Arrays A, Buffers B, int track) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  float d2 = i < P->np ? predictor_slot(P, A, B, i) : 0.f;
+  if (!track) return;
+  for (int o = 32; o > 0; o >>= 1) d2 = fmaxf(d2, __shfl_xor(d2, o));
+  if ((threadIdx.x & 63) == 0 && d2 > 0.f) {
+    const unsigned bits = __float_as_uint(d2);
+    if (bits > __hip_atomic_load(&P->max_disp2_bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      atomicMax(&P->max_disp2_bits, bits);
+  }
 }
 
 // --------------------------------------------------------- corrector ----
@@ -735,8 +740,8 @@ void launch_shepard(hipStream_t s, const DevParams* P, int npm_max, Arrays A, co
   if (copy_back) hipLaunchKernelGGL(k_copy_soil_str, dim3(blocks(npm_max, 256)), dim3(256), 0, s, P, A, (const float4*)oa,
                      (const float2*)ob);
 }
-void launch_reset_deferred(hipStream_t s, DevParams* P) {
-  hipLaunchKernelGGL(k_reset_deferred, dim3(1), dim3(1), 0, s, P);
+void launch_reset_deferred(hipStream_t s, DevParams* P, int list_build) {
+  hipLaunchKernelGGL(k_reset_deferred, dim3(1), dim3(1), 0, s, P, list_build);
 }
 void launch_pair_prep(hipStream_t s, const DevParams* P, int n, Arrays A, const Buffers& B) {
   if (n > 0) hipLaunchKernelGGL(k_pair_prep, dim3(blocks(n, 256)), dim3(256), 0, s, P, A, B, n);
@@ -746,10 +751,13 @@ void launch_interaction(hipStream_t s, DevParams* P, int npm_max, Arrays A, cons
   if (npm_max <= 0) return;
   if (variant == LQ_INTERACTION_REFERENCE_ORDER)
     hipLaunchKernelGGL(k_interaction_ref, dim3(blocks(npm_max, 256)), dim3(256), 0, s, P, A, B, cb, ce,
-                       xcd_remap);
+                       xcd_remap, 0);
   else if (variant == LQ_INTERACTION_NEIGHBOR_LIST)
     hipLaunchKernelGGL(k_interaction_nl, dim3(blocks(npm_max, NL_WAVE * NL_WAVES)), dim3(NL_WAVE * NL_WAVES), 0, s,
                        P, A, B, cb, ce, xcd_remap);
+  else if (variant == LQ_INTERACTION_PAIR_LIST)  // fallback: runs only if the pair lists are not exact
+    hipLaunchKernelGGL(k_interaction_ref, dim3(blocks(npm_max, 256)), dim3(256), 0, s, P, A, B, cb, ce,
+                       xcd_remap, 1);
 }
 void launch_contact(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const Buffers& B, const int* cb,
                     const int* ce) {
@@ -758,8 +766,8 @@ void launch_contact(hipStream_t s, const DevParams* P, int npm_max, Arrays A, co
   hipLaunchKernelGGL(k_contact_redistribute, dim3(g), dim3(256), 0, s, P, A, B, cb, ce);
   hipLaunchKernelGGL(k_contact_commit, dim3(g), dim3(256), 0, s, P, A, B);
 }
-void launch_predictor(hipStream_t s, const DevParams* P, int np, Arrays A, const Buffers& B) {
-  if (np > 0) hipLaunchKernelGGL(k_predictor, dim3(blocks(np, 256)), dim3(256), 0, s, P, A, B);
+void launch_predictor(hipStream_t s, DevParams* P, int np, Arrays A, const Buffers& B, int track_disp) {
+  if (np > 0) hipLaunchKernelGGL(k_predictor, dim3(blocks(np, 256)), dim3(256), 0, s, P, A, B, track_disp);
 }
 void launch_corrector(hipStream_t s, DevParams* P, int npb, int npm_max, int n_motion, Arrays A, const Buffers& B) {
   if (n_motion > 0 && npb > 0)

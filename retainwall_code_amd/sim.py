@@ -181,6 +181,16 @@ class Context:
     def set_interaction(self, variant, xcd_remap=0):
         self._chk(self.lib.lq_set_interaction(self.h, variant, xcd_remap), "lq_set_interaction")
 
+    def set_pair_list(self, cap=160, skin=0.01):
+        self._chk(self.lib.lq_set_pair_list(self.h, cap, skin), "lq_set_pair_list")
+
+    def pair_list_status(self):
+        """(overflow, max displacement since the last build, allowed displacement)."""
+        ov, md, lim = C.c_int(), C.c_float(), C.c_float()
+        self._chk(self.lib.lq_pair_list_status(self.h, C.byref(ov), C.byref(md), C.byref(lim)),
+                  "lq_pair_list_status")
+        return ov.value, md.value, lim.value
+
     def autotune(self, reps=3):
         """Time every interaction variant on the current state, keep the
         fastest; returns (variant, tile_shape, tile_tz, xcd_remap, ms_per_call)."""

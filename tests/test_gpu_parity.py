@@ -159,6 +159,7 @@ def test_step_pieces(variant, oracle_mod):
 # fallback.
 VARIANTS = [(_abi.INTERACTION_REFERENCE_ORDER, 0, None), (_abi.INTERACTION_REFERENCE_ORDER, 1, None),
             (_abi.INTERACTION_NEIGHBOR_LIST, 0, None), (_abi.INTERACTION_NEIGHBOR_LIST, 1, None),
+            (_abi.INTERACTION_PAIR_LIST, 0, None), (_abi.INTERACTION_PAIR_LIST, 1, None),
             (_abi.INTERACTION_TILE, 0, 3), (_abi.INTERACTION_TILE, 0, 6), (_abi.INTERACTION_TILE, 1, 7),
             (_abi.INTERACTION_TILE, 2, 13), (_abi.INTERACTION_TILE, 2, 1)]
 
@@ -182,6 +183,40 @@ def test_interaction_variants_identical():
             np.testing.assert_array_equal(bits(outs[0][k]), bits(o[k]), err_msg=k)
 
 
+@pytest.mark.parametrize("cap,skin,expect", [(160, 0.01, "reuse"), (8, 0.01, "overflow"), (160, 0.0, "moved")])
+def test_pair_list_fallbacks(cap, skin, expect):
+    """Pair lists reused, overflowing (cell-scan fallback in both calls) or
+    outrun by the predictor's displacement (fallback in the corrector-stage
+    call): every path matches the reference-order kernel bit for bit.  The
+    failure conditions are forced from the host through the capacity/skin."""
+    case = cases.config("small")
+    say(f"[pair_list] case=small cap={cap} skin={skin}")
+    ref = sim.open_case(case, variant=_abi.INTERACTION_REFERENCE_ORDER)
+    pl = sim.open_case(case, variant=_abi.INTERACTION_PAIR_LIST)
+    pl.set_pair_list(cap, skin)
+    keys = ["accad", "acestn", "spin", "vel", "str", "cmag", "cvar", "pos"]
+    moved = []
+    for step in range(4):
+        ref.step(1)
+        pl.step(1)
+        ov, md, lim = pl.pair_list_status()
+        say(f"  step {step}: overflow={ov} max_disp={md:.3e} limit={lim:.3e}")
+        if expect == "reuse":
+            assert ov == 0 and md <= lim
+        elif expect == "overflow":
+            assert ov == 1
+        else:
+            assert ov == 0
+        moved.append(md > lim)
+        a, b = ref.snapshot(keys), pl.snapshot(keys)
+        for k in keys:
+            np.testing.assert_array_equal(bits(a[k]), bits(b[k]), err_msg=f"{k} step {step}")
+    if expect == "moved":
+        assert any(moved)  # the zero skin sent at least one corrector stage to the cell scan
+    ref.close()
+    pl.close()
+
+
 def test_autotune_keeps_state():
     case = cases.config("small")
     say("[autotune] case=small")
@@ -191,7 +226,7 @@ def test_autotune_keeps_state():
     b.step(2)
     variant, shape, tz, remap, ms = b.autotune(reps=2)
     say(f"  autotune -> variant={variant} shape={shape} tz={tz} remap={remap} ms={ms:.4f}")
-    assert variant in (0, 1, 2) and 0 <= shape <= 2 and tz >= 1 and remap in (0, 1) and ms > 0
+    assert variant in (0, 1, 2, 3) and 0 <= shape <= 2 and tz >= 1 and remap in (0, 1) and ms > 0
     a.step(2)
     b.step(2)
     a.sync()
