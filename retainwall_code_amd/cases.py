@@ -214,7 +214,7 @@ def retaining_wall(box_length=0.4, box_width=0.2, box_height=0.3, dr=0.005, hdr=
                 [mt], box, name)
 
 
-def make_params(case: Case, key_mode=_abi.KEY_LINEAR, variant=_abi.INTERACTION_REFERENCE_ORDER,
+def make_params(case: Case, key_mode=_abi.KEY_LINEAR, variant=_abi.INTERACTION_NEIGHBOR_LIST,
                 lib=None) -> _abi.Params:
     """Fill an lq_params struct from a case and initialise derived fields."""
     p = _abi.Params()

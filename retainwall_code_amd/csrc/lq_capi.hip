@@ -32,7 +32,7 @@ struct lq_ctx {
   int nbits = 0;
   int npm0 = 0;
   int tile_shape = 1, tile_tz = 7;
-  int xcd_remap = 0;  // renumber interaction blocks per XCD (lq_set_interaction / lq_autotune)
+  int xcd_remap = 1;  // renumber interaction blocks per XCD (lq_set_interaction / lq_autotune)
   long long host_step = 0;
   // instrumentation
   bool timing = false;
@@ -320,7 +320,8 @@ extern "C" int lq_create(const lq_params* par, int device, lq_ctx** out) {
   st |= dalloc(c, &B.keys, n); st |= dalloc(c, &B.vals, n);
   st |= dalloc(c, &B.keys_alt, n); st |= dalloc(c, &B.vals_alt, n);
   st |= dalloc(c, &B.cell_beg, size_t(par->ncell)); st |= dalloc(c, &B.cell_end, size_t(par->ncell));
-  st |= dalloc(c, &B.hist, size_t(256) * radix_tiles(par->np));
+  st |= dalloc(c, &B.wall_near, size_t(std::max(1, par->ncell_material)));
+  st |= dalloc(c, &B.hist, size_t(256) * radix_tiles(par->np) + size_t(radix_tiles(par->np)) + 1);  // + scan tile sums
   c->wf_blocks = std::max(1, std::min(1024, (par->np + 255) / 256));
   st |= dalloc(c, &B.partial, size_t(c->wf_blocks));
   st |= dalloc(c, &c->wf_out, 1);
@@ -704,7 +705,8 @@ static int enq_adami(lq_ctx* c) {
 }
 static int enq_corner(lq_ctx* c) {
   Section s(c, LQ_K_CORNER);
-  if (c->par.npb > 0) launch_corner(c->stream, c->dP, c->npm0, c->B.cur, c->B.cell_beg, c->B.cell_end);
+  if (c->par.npb > 0)
+    launch_corner(c->stream, c->dP, c->npm0, c->B.cur, c->B.cell_beg, c->B.cell_end, c->B.wall_near);
   return 0;
 }
 static int enq_shepard(lq_ctx* c) {

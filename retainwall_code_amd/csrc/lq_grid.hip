@@ -58,28 +58,94 @@ __global__ void __launch_bounds__(RS_THREADS) k_radix_hist(const int* __restrict
   hist[t * nblocks + blockIdx.x] = cnt[t];
 }
 
-// In-place exclusive scan of n ints by one 1024-thread block.
-__global__ void __launch_bounds__(1024) k_scan_excl(int* __restrict__ data, int n) {
-  __shared__ int sh[1024];
-  const int t = threadIdx.x;
-  const int per = (n + 1023) / 1024;
-  const int s = min(t * per, n), e = min(s + per, n);
-  int sum = 0;
-  for (int k = s; k < e; k++) sum += data[k];
-  sh[t] = sum;
+// In-place exclusive scan of the radix histogram (n ints) in three
+// launches: per-tile sums of SC_TILE consecutive ints (coalesced int4
+// loads), one block scanning the tile sums, and a tile-local scan that adds
+// the tile's offset.  sums holds ceil(n / SC_TILE) ints.
+constexpr int SC_THREADS = 256;
+constexpr int SC_TILE = SC_THREADS * 4;
+
+// Block-wide inclusive scan of one int per thread (SC_THREADS threads).
+__device__ __forceinline__ int block_incl_scan(int v, int* sh_wave) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(v, o, 64);
+    if (lane >= o) v += u;
+  }
+  if (lane == 63) sh_wave[w] = v;
   __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    int v = t >= off ? sh[t - off] : 0;
+  int add = 0;
+  for (int k = 0; k < w; k++) add += sh_wave[k];
+  __syncthreads();
+  return v + add;
+}
+
+__device__ __forceinline__ int4 load_tile4(const int* __restrict__ data, int n, int base) {
+  const int k = base + 4 * int(threadIdx.x);
+  if (k + 3 < n) return *reinterpret_cast<const int4*>(data + k);
+  return make_int4(k < n ? data[k] : 0, k + 1 < n ? data[k + 1] : 0, k + 2 < n ? data[k + 2] : 0,
+                   k + 3 < n ? data[k + 3] : 0);
+}
+
+__global__ void __launch_bounds__(SC_THREADS) k_scan_tile_sums(const int* __restrict__ data, int n,
+                                                               int* __restrict__ sums) {
+  __shared__ int sh_wave[SC_THREADS / 64];
+  const int4 v = load_tile4(data, n, blockIdx.x * SC_TILE);
+  const int tot = block_incl_scan(v.x + v.y + v.z + v.w, sh_wave);
+  if (threadIdx.x == SC_THREADS - 1) sums[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(SC_THREADS) k_scan_sums(int* __restrict__ sums, int ntiles) {
+  __shared__ int sh_wave[SC_THREADS / 64];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < ntiles; base += SC_THREADS) {
+    const int k = base + int(threadIdx.x);
+    const int v = k < ntiles ? sums[k] : 0;
+    const int inc = block_incl_scan(v, sh_wave);
+    const int c = carry;
+    if (k < ntiles) sums[k] = c + inc - v;
     __syncthreads();
-    sh[t] += v;
+    if (threadIdx.x == SC_THREADS - 1) carry = c + inc;
     __syncthreads();
   }
-  int run = sh[t] - sum;
-  for (int k = s; k < e; k++) {
-    int v = data[k];
-    data[k] = run;
-    run += v;
+}
+
+__global__ void __launch_bounds__(SC_THREADS) k_scan_tiles(int* __restrict__ data, int n,
+                                                           const int* __restrict__ sums) {
+  __shared__ int sh_wave[SC_THREADS / 64];
+  const int base = blockIdx.x * SC_TILE;
+  const int4 v = load_tile4(data, n, base);
+  const int own = v.x + v.y + v.z + v.w;
+  const int excl = block_incl_scan(own, sh_wave) - own + sums[blockIdx.x];
+  const int o0 = excl, o1 = o0 + v.x, o2 = o1 + v.y, o3 = o2 + v.z;
+  const int k = base + 4 * int(threadIdx.x);
+  if (k + 3 < n) {
+    *reinterpret_cast<int4*>(data + k) = make_int4(o0, o1, o2, o3);
+  } else {
+    if (k < n) data[k] = o0;
+    if (k + 1 < n) data[k + 1] = o1;
+    if (k + 2 < n) data[k + 2] = o2;
   }
+}
+
+static int scan_tiles(int n) { return (n + SC_TILE - 1) / SC_TILE; }
+
+// wall_near[c] = 1 when any of the 27 cells around cell c (linear key) holds
+// a wall particle, else 0.
+__global__ void __launch_bounds__(256) k_wall_near(const DevParams* __restrict__ P, const int* __restrict__ cbeg,
+                                                   int* __restrict__ wall_near) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= P->ncell_material) return;
+  const int cz = c % P->cnz, cy = (c / P->cnz) % P->cny, cx = c / (P->cny * P->cnz);
+  int any = 0;
+  for (int dz = -1; dz <= 1; dz++)
+    for (int dy = -1; dy <= 1; dy++)
+      for (int dx = -1; dx <= 1; dx++)
+        if (in_grid(P, cx + dx, cy + dy, cz + dz) && cbeg[lin_key(P, cx + dx, cy + dy, cz + dz)] >= 0) any = 1;
+  wall_near[c] = any;
 }
 
 __global__ void __launch_bounds__(RS_THREADS) k_radix_scatter(const int* __restrict__ kin, const int* __restrict__ vin,
@@ -217,7 +283,13 @@ void launch_neighbor_search(hipStream_t s, DevParams* P, int np, int nbits, int 
   const int nb = radix_tiles(np);
   for (int shift = 0; shift < nbits; shift += 8) {
     hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(RS_THREADS), 0, s, (const int*)kin, np, shift, B.hist, nb);
-    hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, s, B.hist, 256 * nb);
+    {
+      const int nh = 256 * nb, nt = scan_tiles(nh);
+      int* sums = B.hist + nh;  // the histogram allocation carries the tile sums
+      hipLaunchKernelGGL(k_scan_tile_sums, dim3(nt), dim3(SC_THREADS), 0, s, (const int*)B.hist, nh, sums);
+      hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SC_THREADS), 0, s, sums, nt);
+      hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(SC_THREADS), 0, s, B.hist, nh, (const int*)sums);
+    }
     hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(RS_THREADS), 0, s, (const int*)kin, (const int*)vin, kout, vout,
                        np, shift, (const int*)B.hist, nb);
     std::swap(kin, kout);
@@ -230,6 +302,10 @@ void launch_neighbor_search(hipStream_t s, DevParams* P, int np, int nbits, int 
   launch_fill_int(s, B.cell_beg, ncell, -1);
   launch_fill_int(s, B.cell_end, ncell, -1);
   hipLaunchKernelGGL(k_beg_end, dim3(blocks(np, 256)), dim3(256), 0, s, P, (const int*)kin, B.cell_beg, B.cell_end);
+  if (ncell > 1) {
+    const int ncm = (ncell - 1) / 2;
+    hipLaunchKernelGGL(k_wall_near, dim3(blocks(ncm, 256)), dim3(256), 0, s, P, (const int*)B.cell_beg, B.wall_near);
+  }
   hipLaunchKernelGGL(k_delete_count, dim3(1), dim3(1), 0, s, P, (const int*)B.cell_beg, (const int*)B.cell_end);
 }
 

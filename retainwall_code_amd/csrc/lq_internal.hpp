@@ -100,6 +100,7 @@ struct Buffers {
   // grid / sort
   int *keys, *vals, *keys_alt, *vals_alt, *part_cel, *part_idx;
   int *cell_beg, *cell_end;
+  int* wall_near;    // per cell: a wall particle lies in its 27-cell neighbourhood
   int* hist;         // radix histogram [256][nblocks]
   double* partial;   // reduction partials
 };

@@ -12,7 +12,8 @@ void launch_step_calc(hipStream_t s, DevParams* P);
 void launch_advance(hipStream_t s, DevParams* P);
 void launch_rigid_body_update(hipStream_t s, DevParams* P);
 void launch_adami(hipStream_t s, const DevParams* P, int npb, Arrays A, const int* cb, const int* ce);
-void launch_corner(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const int* cb, const int* ce);
+void launch_corner(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const int* cb, const int* ce,
+                   const int* wall_near);
 void launch_shepard(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const int* cb, const int* ce,
                     float4* oa, float2* ob, bool copy_back = true);
 void launch_reset_deferred(hipStream_t s, DevParams* P, int list_build);

@@ -200,19 +200,26 @@ __global__ void __launch_bounds__(256) k_adami(const DevParams* __restrict__ P, 
 
 // ------------------------------------------------------------ corner ----
 // cuFindCornerParticleMaterial + loop, adami_boundary.cu:209-292.
+// wall_near[c] != 0 when a wall particle lies in the 27 cells around cell
+// c; elsewhere the scan finds nothing and only the reset of cvar.xy remains.
 __global__ void __launch_bounds__(256) k_corner(const DevParams* __restrict__ P, Arrays A,
-                                                const int* __restrict__ cbeg, const int* __restrict__ cend, int remap) {
+                                                const int* __restrict__ cbeg, const int* __restrict__ cend,
+                                                const int* __restrict__ wall_near, int remap) {
   int idx = xcd_block(remap) * blockDim.x + threadIdx.x;
   if (idx >= P->npm) return;
   const int i = P->npb + idx;
-  const float3 pi = ld3(A.pos[i]);
   const int4 ci = A.cellpos[i];
-  const float two_h = P->two_h, i_h = P->i_h, r2cut = P->r2cut;
   float4 cv = A.cvar[i];
-  float4 cn = A.cn[i];
-  float4 cw = A.cw[i];
   cv.x = 0.f;
   cv.y = 999.f;
+  if (!wall_near[lin_key(P, ci.x, ci.y, ci.z)]) {
+    A.cvar[i] = cv;
+    return;
+  }
+  const float3 pi = ld3(A.pos[i]);
+  const float two_h = P->two_h, i_h = P->i_h, r2cut = P->r2cut;
+  float4 cn = A.cn[i];
+  float4 cw = A.cw[i];
   float dij = 999.f;
   for (int cz = -1; cz <= 1; cz++)
     for (int cy = -1; cy <= 1; cy++)
@@ -362,20 +369,38 @@ __global__ void __launch_bounds__(256) k_interaction_ref(DevParams* __restrict__
 }
 
 // Neighbour-list variant.  The wave walks the 27 cells x {wall, soil}
-// ranges in reference order in lockstep (loop indices are wave-uniform, so
-// the lanes of one home cell read the same candidate at the same time).
-// Each range is tested in chunks of NL_CAP candidates into a per-lane hit
-// mask, and the chunk's hits are then evaluated with every lane taking its
-// own next hit, so the expensive pair terms run on the hits alone instead
-// of on every candidate some lane of the wave accepts.  Each particle still
-// sums its pairs in reference order.
+// ranges in reference order (loop indices are wave-uniform, so the lanes of
+// one home cell read the same candidate at the same time).  Each range is
+// tested in chunks of NL_CAP candidates into a per-lane hit mask; the hits
+// are appended to a per-lane list in LDS, and the lists are evaluated --
+// every lane working through its own entries in order -- only when some
+// lane's next chunk would not fit, and at the end.  The expensive pair
+// terms thus run on hits gathered over many ranges in near-full-width wave
+// iterations, while each particle still sums its pairs in reference order.
+// An entry packs the slot j (bits 7..31), the cell offset (bits 0..5) and
+// the range (bit 6: 0 wall, 1 soil).
 constexpr int NL_WAVE = 64;
 constexpr int NL_WAVES = 4;
 constexpr int NL_CAP = 32;
 
+__device__ __forceinline__ void nl_eval(const DevParams* __restrict__ P, IAcc& S, const Arrays& A, const Buffers& B,
+                                        int e, float3 pi, float two_h, float i_h, float3 vi, float rhoi, Sym sri,
+                                        float artpi, int shift) {
+  const int j = int(unsigned(e) >> 7);
+  const int ox = (e & 3) - 1, oy = ((e >> 2) & 3) - 1, oz = ((e >> 4) & 3) - 1;
+  const float4 pj = A.pos[j];
+  float3 d;
+  const float r = sqrtf(pair_r2(pi, pj, ox, oy, oz, two_h, d));
+  const float q = r * i_h;
+  pair_terms(pair_const(P), S, d, r, q, vi, rhoi, sri, artpi, shift, (e & 64) == 0, A.vel[j], pj.w, B.sr_a[j],
+             B.sr_b[j]);
+}
+
 __global__ void __launch_bounds__(NL_WAVE * NL_WAVES) k_interaction_nl(DevParams* __restrict__ P, Arrays A, Buffers B,
                                                                       const int* __restrict__ cbeg,
                                                                       const int* __restrict__ cend, int remap) {
+  __shared__ int s_list[NL_WAVES][NL_CAP][NL_WAVE];
+  const int lane = threadIdx.x & (NL_WAVE - 1);
   const int idx = xcd_block(remap) * blockDim.x + threadIdx.x;
   const bool active = idx < P->npm;
   const int i = P->npb + (active ? idx : 0);
@@ -391,11 +416,13 @@ __global__ void __launch_bounds__(NL_WAVE * NL_WAVES) k_interaction_nl(DevParams
   const int shift = int(A.cvar[i].z);
   const float two_h = P->two_h, i_h = P->i_h, r2cut = P->r2cut;
   const int ncm = P->ncell_material;
+  int (*lst)[NL_WAVE] = s_list[threadIdx.x / NL_WAVE];
   IAcc S;
   S.accden = 0.f;
   S.as = S.av = S.aa = make_float3(0.f, 0.f, 0.f);
 #pragma unroll
   for (int k = 0; k < 9; k++) S.gv[k] = 0.f;
+  int n = 0;
 #pragma unroll 1
   for (int r = 0; r < 54; r++) {
     const int cell = r >> 1, pass = r & 1;
@@ -407,6 +434,7 @@ __global__ void __launch_bounds__(NL_WAVE * NL_WAVES) k_interaction_nl(DevParams
       j = cbeg[c];
       e = cend[c];
     }
+    const unsigned code = unsigned((cx + 1) | ((cy + 1) << 2) | ((cz + 1) << 4) | (pass << 6));
     while (j < e) {
       const int m = min(e - j, NL_CAP);
       unsigned mask = 0u;
@@ -415,71 +443,94 @@ __global__ void __launch_bounds__(NL_WAVE * NL_WAVES) k_interaction_nl(DevParams
         float3 d;
         if (pair_r2(pi, A.pos[j + t], cx, cy, cz, two_h, d) <= r2cut) mask |= 1u << t;
       }
+      if (__ballot(n + __popc(mask) > NL_CAP)) {  // some lane could overflow: evaluate the lists
 #pragma unroll 1
+        for (int k = 0; k < n; k++) nl_eval(P, S, A, B, lst[k][lane], pi, two_h, i_h, vi, rhoi, sri, artpi, shift);
+        n = 0;
+      }
       while (mask) {
-        const int jj = j + __ffs(mask) - 1;
+        const int t = __ffs(mask) - 1;
         mask &= mask - 1u;
-        const float4 pj = A.pos[jj];
-        float3 d;
-        const float r = sqrtf(pair_r2(pi, pj, cx, cy, cz, two_h, d));
-        const float q = r * i_h;
-        pair_terms(pair_const(P), S, d, r, q, vi, rhoi, sri, artpi, shift, pass == 0, A.vel[jj], pj.w,
-                   B.sr_a[jj], B.sr_b[jj]);
+        lst[n++][lane] = int((unsigned(j + t) << 7) | code);
       }
       j += m;
     }
   }
+#pragma unroll 1
+  for (int k = 0; k < n; k++) nl_eval(P, S, A, B, lst[k][lane], pi, two_h, i_h, vi, rhoi, sri, artpi, shift);
   if (active) interaction_epilogue(P, A, B, i, S, vi);
 }
 
 // Pass B: same-row redistribution for deferred particles
 // (particle_interaction.cu:142-172, 209-237), reading pass-A values only.
-__global__ void k_contact_redistribute(const DevParams* __restrict__ P, Arrays A, Buffers B,
-                                       const int* __restrict__ cbeg, const int* __restrict__ cend) {
+// One wave per deferred particle, its lanes splitting each soil range.  The
+// serial loop keeps K = min(K, Kj) over the qualifying neighbours and
+// overwrites fr = fj * K / Kj at each, so its result is the minimum K and
+// the force of the last qualifying neighbour in traversal order scaled by
+// that minimum: the wave reduces exactly those (a min and an arg-max of the
+// traversal position) and evaluates the same expression.
+__global__ void __launch_bounds__(256) k_contact_redistribute(const DevParams* __restrict__ P, Arrays A, Buffers B,
+                                                              const int* __restrict__ cbeg,
+                                                              const int* __restrict__ cend) {
   const int nd = P->n_deferred;
-  const float two_h = P->two_h;
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nd; k += gridDim.x * blockDim.x) {
+  const float two_h = P->two_h, band = P->band, half_dr = P->half_dr;
+  const int ncm = P->ncell_material;
+  const int lane = threadIdx.x & 63;
+  const int nwaves = int(gridDim.x * blockDim.x) >> 6;
+  for (int k = int(blockIdx.x * blockDim.x + threadIdx.x) >> 6; k < nd; k += nwaves) {
     const int i = B.deferred[k];
     const int4 ci = A.cellpos[i];
     const float3 pi = ld3(A.pos[i]);
     const float3 pa = make_float3(pi.x + float(ci.x) * two_h, pi.y + float(ci.y) * two_h, pi.z + float(ci.z) * two_h);
     const float3 ni = ld3(A.cn[i]);
     float K = A.cvar[i].x;
-    float3 fr = make_float3(0.f, 0.f, 0.f);
-    int count = 0;
-    for (int cz = -1; cz <= 1; cz++)
-      for (int cy = -1; cy <= 1; cy++)
-        for (int cx = -1; cx <= 1; cx++) {
-          int jx = ci.x + cx, jy = ci.y + cy, jz = ci.z + cz;
-          if (!in_grid(P, jx, jy, jz)) continue;
-          int c = lin_key(P, jx, jy, jz) + P->ncell_material;
-          for (int j = cbeg[c], e = cend[c]; j < e; j++) {
-            float4 cvj = A.cvar[j];
-            if (!(cvj.y <= P->band)) continue;
-            int4 cjj = A.cellpos[j];
-            float4 pj = A.pos[j];
-            float3 pja = make_float3(pj.x + float(cjj.x) * two_h, pj.y + float(cjj.y) * two_h, pj.z + float(cjj.z) * two_h);
-            float3 dc = f3sub(pa, pja);
-            float rcol = fabsf(dc.x * ni.z + dc.y * ni.y + dc.z * ni.x);
-            float Kj = cvj.x;
-            if (rcol <= P->half_dr && Kj != 0.f) {
-              if (K > Kj) K = Kj;
-              float4 fj = A.cf[j];
-              fr = make_float3(fj.x * K / Kj, fj.y * K / Kj, fj.z * K / Kj);
-              count++;
-            }
-          }
+    int best = -1, bestj = 0;  // traversal position (range << 20 | offset) and slot of the last qualifying j
+    for (int r = 0; r < 27; r++) {
+      const int cz = r / 9 - 1, cy = (r / 3) % 3 - 1, cx = r % 3 - 1;
+      const int jx = ci.x + cx, jy = ci.y + cy, jz = ci.z + cz;
+      if (!in_grid(P, jx, jy, jz)) continue;
+      const int c = lin_key(P, jx, jy, jz) + ncm;
+      const int b = cbeg[c], e = cend[c];
+      for (int j = b + lane; j < e; j += 64) {
+        const float4 cvj = A.cvar[j];
+        if (!(cvj.y <= band)) continue;
+        const int4 cjj = A.cellpos[j];
+        const float4 pj = A.pos[j];
+        const float3 pja = make_float3(pj.x + float(cjj.x) * two_h, pj.y + float(cjj.y) * two_h,
+                                       pj.z + float(cjj.z) * two_h);
+        const float3 dc = f3sub(pa, pja);
+        const float rcol = fabsf(dc.x * ni.z + dc.y * ni.y + dc.z * ni.x);
+        const float Kj = cvj.x;
+        if (rcol <= half_dr && Kj != 0.f) {
+          if (K > Kj) K = Kj;
+          best = (r << 20) | (j - b);  // j ascends per lane, so this is the lane's last so far
+          bestj = j;
         }
-    if (count == 0) fr = f3mul(fr, 0.f);
-    B.fdef[k] = make_float4(fr.x, fr.y, fr.z, K);
-    float4 s3m = B.s3[i];
-    float3 ast = f3mul(fr, 1.0f / s3m.w);
-    float3 vi = ld3(A.vel[i]);
-    float3 damp = f3mul(vi, P->damping);
-    const float3 g = make_float3(P->gx, P->gy, P->gz);
-    float3 acc = f3add(f3sub(f3add(make_float3(s3m.x, s3m.y, s3m.z), ast), damp), g);
-    float4 old = B.accad[i];
-    B.accad[i] = make_float4(acc.x, acc.y, acc.z, old.w);
+      }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      const float Ko = __shfl_xor(K, o);
+      if (K > Ko) K = Ko;
+      const int bo = __shfl_xor(best, o), jo = __shfl_xor(bestj, o);
+      if (bo > best) { best = bo; bestj = jo; }
+    }
+    if (lane == 0) {
+      float3 fr = make_float3(0.f, 0.f, 0.f);
+      if (best >= 0) {
+        const float4 fj = A.cf[bestj];
+        const float Kj = A.cvar[bestj].x;
+        fr = make_float3(fj.x * K / Kj, fj.y * K / Kj, fj.z * K / Kj);
+      }
+      B.fdef[k] = make_float4(fr.x, fr.y, fr.z, K);
+      const float4 s3m = B.s3[i];
+      const float3 ast = f3mul(fr, 1.0f / s3m.w);
+      const float3 vi = ld3(A.vel[i]);
+      const float3 damp = f3mul(vi, P->damping);
+      const float3 g = make_float3(P->gx, P->gy, P->gz);
+      const float3 acc = f3add(f3sub(f3add(make_float3(s3m.x, s3m.y, s3m.z), ast), damp), g);
+      const float4 old = B.accad[i];
+      B.accad[i] = make_float4(acc.x, acc.y, acc.z, old.w);
+    }
   }
 }
 
@@ -730,8 +781,10 @@ void launch_rigid_body_update(hipStream_t s, DevParams* P) {
 void launch_adami(hipStream_t s, const DevParams* P, int npb, Arrays A, const int* cb, const int* ce) {
   if (npb > 0) hipLaunchKernelGGL(k_adami, dim3(blocks(npb, 256)), dim3(256), 0, s, P, A, cb, ce, 1);
 }
-void launch_corner(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const int* cb, const int* ce) {
-  if (npm_max > 0) hipLaunchKernelGGL(k_corner, dim3(blocks(npm_max, 256)), dim3(256), 0, s, P, A, cb, ce, 1);
+void launch_corner(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const int* cb, const int* ce,
+                   const int* wall_near) {
+  if (npm_max > 0)
+    hipLaunchKernelGGL(k_corner, dim3(blocks(npm_max, 256)), dim3(256), 0, s, P, A, cb, ce, wall_near, 1);
 }
 void launch_shepard(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const int* cb, const int* ce,
                     float4* oa, float2* ob, bool copy_back) {
@@ -763,7 +816,9 @@ void launch_contact(hipStream_t s, const DevParams* P, int npm_max, Arrays A, co
                     const int* ce) {
   if (npm_max <= 0) return;
   int g = std::min(blocks(npm_max, 256), 1024);
-  hipLaunchKernelGGL(k_contact_redistribute, dim3(g), dim3(256), 0, s, P, A, B, cb, ce);
+  // one wave per deferred particle, up to 4 * 4096 at a time
+  hipLaunchKernelGGL(k_contact_redistribute, dim3(std::min(blocks(npm_max, 4), 4096)), dim3(256), 0, s, P, A, B, cb,
+                     ce);
   hipLaunchKernelGGL(k_contact_commit, dim3(g), dim3(256), 0, s, P, A, B);
 }
 void launch_predictor(hipStream_t s, DevParams* P, int np, Arrays A, const Buffers& B, int track_disp) {

@@ -45,6 +45,7 @@ void LoquatSph::GPUInfoExibition(int argc, char* argv[]) {
 }
 
 void LoquatSph::InitializeParameters() {
+  par_host->interaction_variant = LQ_INTERACTION_NEIGHBOR_LIST;  // fastest bit-identical variant
   if (lq_initialize_parameters(par_host) != LQ_OK) {
     std::cout << "The input dimension of problem is " << par_host->dim << std::endl;
     std::cout << "Check the dimension of problem!" << std::endl;

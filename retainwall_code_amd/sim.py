@@ -208,7 +208,7 @@ class Context:
         return counts, ids
 
 
-def open_case(case, key_mode=_abi.KEY_LINEAR, variant=_abi.INTERACTION_REFERENCE_ORDER, device=0, initial_sort=True):
+def open_case(case, key_mode=_abi.KEY_LINEAR, variant=_abi.INTERACTION_NEIGHBOR_LIST, device=0, initial_sort=True):
     """Create a context, upload a case and run the initial search/sort
     (loquat.cu:528-531: GridConfig, Abs2Rel, NeighborSearch, SortArraysPrimary)."""
     from . import cases
