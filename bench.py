@@ -155,7 +155,11 @@ def main():
         traffic = None
         pmc = os.path.join(ROOT, "profiles", f"interaction_pmc_{args.case}.json")
         if os.path.exists(pmc):
-            traffic = json.load(open(pmc)).get("hbm_bytes_per_launch")
+            rec = json.load(open(pmc))
+            if rec.get("kernel") == {"tile": "k_interaction_tile", "reference": "k_interaction_ref",
+                                     "neighbor_list": "k_interaction_nl",
+                                     "pair_list": "k_interaction_plist"}[vname]:
+                traffic = rec.get("hbm_bytes_per_launch")  # rocprofv3 FETCH/WRITE_SIZE, copy-calibrated
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": {"tile": "k_interaction_tile", "reference": "k_interaction_ref",
