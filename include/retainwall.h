@@ -241,9 +241,12 @@ int lq_corrector(lq_ctx* ctx);
  * step in a hipGraph and replays it.  No host synchronisation inside. */
 int lq_step(lq_ctx* ctx, int nsteps, int use_graph);
 
-/* Wall earth-pressure probe, utilities.cu:578-670: F = sum 0.001*fx [kN]
+/* Wall earth-pressure probe, utilities.cu:578-670: F = 0.001 * sum fx [kN]
  * over soil slots with x_abs <= box[0], box[1] <= z_abs <= box[2].  The
- * reference hard-codes box = {0.04+bt+3dr, bt+0.04, bt+1.04}.  Synchronises. */
+ * reference hard-codes box = {0.04+bt+3dr, bt+0.04, bt+1.04} and sums
+ * serially in FP32; here the fx values are summed exactly (128-bit fixed
+ * point, 2^-64 resolution, independent of order) and scaled once.
+ * Synchronises. */
 int lq_wall_force(lq_ctx* ctx, const float box[3], double* force_kn);
 
 /* Absolute positions (PositionFromRelative2Absolute, loquat_grid.cu:273-289)
@@ -311,6 +314,36 @@ int lq_interaction_cutoff_sq(float i_h, float* r2_cut);
  * order, written to ids[s*max_nb ...], counts[s] = number found (may exceed
  * max_nb; only the first max_nb are stored).  Synchronises. */
 int lq_neighbor_lists(lq_ctx* ctx, int max_nb, int32_t* counts, int32_t* ids, size_t ids_bytes);
+
+/* ---- batched contexts (no reference counterpart; the reference runs one
+ * case per process, types.h:100-165) ---------------------------------------
+ * lq_create_batch builds one context for n_cases independent cases, each
+ * described by its own lq_params (after lq_initialize_parameters).  The
+ * cases must agree on every parameter the step reads globally (dr, hdr,
+ * gravity, damping, kernel constants, viscosity, cs, cfl, eta, artificial
+ * pressure, Shepard period, time and dt); geometry, particle counts,
+ * materials (model_par) and motions (mv, motion tables) are per case.  The
+ * cases are laid out as cell-aligned blocks of one global grid separated by
+ * empty cells, so every kernel of a step runs once over all cases and no
+ * neighbour range crosses a case; each case's particles occupy the slots, in
+ * the order, that a context of that case alone would give them, so every
+ * per-case result equals that context's bit for bit.  lq_step,
+ * lq_synchronize, lq_autotune and the instrumentation work on the whole
+ * batch; lq_upload / lq_set_motion_tables / lq_wall_force / lq_set_field
+ * are for one-case contexts only. */
+int lq_create_batch(const lq_params* cases, int n_cases, int device, lq_ctx** out);
+int lq_case_count(const lq_ctx* ctx);
+/* Upload case k (all cases before the first step). */
+int lq_upload_case(lq_ctx* ctx, int k, const lq_host_state* host);
+int lq_set_case_motion_tables(lq_ctx* ctx, int k, int n, const lq_motion_table* tables);
+/* Case k's rows of a per-slot field (lq_field; not the grid arrays), in its
+ * own slot order, codes and cellpos in its own frame: bytes = np_k x row. */
+int lq_get_case_field(lq_ctx* ctx, int k, int field, void* dst, size_t bytes);
+int lq_case_positions_absolute(lq_ctx* ctx, int k, float* dst, size_t bytes);
+int lq_case_counts(lq_ctx* ctx, int k, int32_t* npb, int32_t* npm, int32_t* np_delete_m);
+/* Wall force of every case: boxes[3k..3k+2] as lq_wall_force's box in case
+ * k's frame, force_kn[k] out. */
+int lq_wall_force_cases(lq_ctx* ctx, const float* boxes, double* force_kn);
 
 #ifdef __cplusplus
 }

@@ -124,6 +124,8 @@ EXPORTS = [
     "lq_find_corner_particle", "lq_stress_regularization", "lq_particle_interaction",
     "lq_predictor", "lq_corrector", "lq_step", "lq_wall_force", "lq_positions_absolute",
     "lq_get_counts", "lq_synchronize", "lq_stress_saving_regularization", "lq_stream_copy", "lq_enable_timing", "lq_kernel_times", "lq_neighbor_lists", "lq_set_tile", "lq_set_interaction", "lq_set_pair_list", "lq_pair_list_status", "lq_autotune", "lq_interaction_cutoff_sq",
+    "lq_create_batch", "lq_case_count", "lq_upload_case", "lq_set_case_motion_tables", "lq_get_case_field",
+    "lq_case_positions_absolute", "lq_case_counts", "lq_wall_force_cases",
 ]
 
 
@@ -166,6 +168,14 @@ def _proto(lib):
     lib.lq_pair_list_status.argtypes = [vp, P(C.c_int), P(C.c_float), P(C.c_float)]
     lib.lq_autotune.argtypes = [vp, C.c_int, P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_double)]
     lib.lq_interaction_cutoff_sq.argtypes = [C.c_float, P(C.c_float)]
+    lib.lq_create_batch.argtypes = [P(Params), C.c_int, C.c_int, P(vp)]
+    lib.lq_case_count.argtypes = [vp]
+    lib.lq_upload_case.argtypes = [vp, C.c_int, P(HostState)]
+    lib.lq_set_case_motion_tables.argtypes = [vp, C.c_int, C.c_int, P(MotionTable)]
+    lib.lq_get_case_field.argtypes = [vp, C.c_int, C.c_int, vp, C.c_size_t]
+    lib.lq_case_positions_absolute.argtypes = [vp, C.c_int, vp, C.c_size_t]
+    lib.lq_case_counts.argtypes = [vp, C.c_int, P(i32), P(i32), P(i32)]
+    lib.lq_wall_force_cases.argtypes = [vp, vp, vp]
 
 
 _LIB = None

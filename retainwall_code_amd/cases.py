@@ -296,6 +296,37 @@ def motion_tables(case: Case):
 
 
 # ----------------------------------------------------------- named configs --
+def batch_cases(n_cases: int, seed: int, dr: float = 0.005, n_range=(5.0e3, 2.0e5), h_range=(0.1, 0.6),
+                **over) -> list:
+    """Independent wall cases for a batched context: N_k log-uniform over
+    n_range, backfill height H ~ U(h_range), phi_c ~ U(28, 36) deg, e0 ~
+    U(0.60, 0.80), translation or toe rotation with equal odds at a wall speed
+    ~ U(0.5, 2) mm/s (rotation rate = speed / H), backfill length twice its
+    width.  All cases share dr and the solver constants, as a batch needs."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_cases):
+        n = float(np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]))))
+        height = float(rng.uniform(*h_range))
+        phi = float(rng.uniform(28.0, 36.0))
+        e0 = float(rng.uniform(0.60, 0.80))
+        speed = float(rng.uniform(0.5e-3, 2.0e-3))
+        rotate = bool(rng.integers(0, 2))
+        # the soil also fills the bed step (fix_wall_height) and, with no side
+        # walls, the wall-thickness margins in y: N = 2W (W + 2bt)(H + fwh) / dr^3
+        soil_h = height + over.get("fix_wall_height", 0.04)
+        bt = math.ceil(2.0 * over.get("hdr", 1.3)) * dr
+        a2 = n * dr ** 3 / soil_h
+        width = (-4.0 * bt + math.sqrt(16.0 * bt * bt + 8.0 * a2)) / 4.0
+        motion = ("rotate", -speed / height) if rotate else ("translate", -speed)
+        kw = dict(box_length=2.0 * width, box_width=width, box_height=height, dr=dr, motion=motion, jitter=0.05,
+                  void_range=(e0 - 0.05, e0 + 0.05), seed=seed * 1000 + k,
+                  material=hypo_cs_sand(phi_c_deg=phi, e0=e0))
+        kw.update(over)
+        out.append(retaining_wall(name=f"batch{seed}_{k}", **kw))
+    return out
+
+
 def config(name: str, **over) -> Case:
     """The benchmark configurations (geometry / material / loading)."""
     if name == "c1":  # 0.4 x 0.2 x 0.3 m, dx 0.005, translation away at 1 mm/s

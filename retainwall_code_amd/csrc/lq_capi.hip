@@ -5,6 +5,7 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <type_traits>
 
 #include "lq_internal.hpp"
 #include "lq_launch.hpp"
@@ -43,11 +44,22 @@ struct lq_ctx {
   // graph
   hipGraphExec_t graph = nullptr;
   long long layout_version = 0, graph_version = -1;
+  // cases (a plain context is a batch of one)
+  int n_cases = 1;
+  std::vector<lq_params> case_par;  // each case's own parameters
+  std::vector<DevCase> cases;       // layout in the global grid, table bases
+  std::vector<int> bnd_base, soil_base;  // first wall / soil slot of each case at upload
+  std::vector<char> case_uploaded;
+  int n_uploaded = 0;
+  float (*d_model)[LQ_MATERIAL_SIZE_MAX] = nullptr;
+  HypoConst* d_hypo = nullptr;
+  DevMotion* d_mv = nullptr;
+  DevTable* d_tab = nullptr;
+  DevCase* d_cases = nullptr;
+  float* d_boxes = nullptr;
+  int n_model_rows = 0, n_motion_rows = 0;
   // reductions
-  int wf_blocks = 1;
-  double* wf_out = nullptr;
-  double* wf_host = nullptr;
-  int* nl_counts = nullptr;
+  unsigned long long* wf_host = nullptr;  // pinned copy of the per-case wall-force accumulators
 };
 
 #define HIPCHECK(ctx, call)                                                                  \
@@ -230,44 +242,69 @@ static void fill_static(const lq_params& p, DevParams& d) {
   d.band = p.dr * 1.0001f;      // particle_interaction.cu:147
   d.half_dr = p.dr / 2;         // particle_interaction.cu:157
   d.mu = 0.58108f;              // particle_interaction.cu:188-189
-  for (int k = 0; k < LQ_MODEL_NUMBER_MAX; k++) {
-    for (int j = 0; j < LQ_MATERIAL_SIZE_MAX; j++) d.model_par[k][j] = p.model_par[k][j];
-    // constitutive_model.h:226-230, hoisted per material (float arithmetic)
-    const float* m = p.model_par[k];
-    const float Ei = m[0], Vi = m[1], phic = m[2], pref = m[4];
-    HypoConst& h = d.hypo[k];
-    if (pref != 0.f && phic != 0.f && Vi != 1.f) {
-      h.C1 = -Ei / (3.0f * pref * (1.0f + Vi));
-      const float a = 3.0f / (sqrtf(2.0f) * sinf(phic));
-      h.C2 = h.C1 * ((3.0f * Vi + a * sqrtf(1.0f + Vi * Vi)) / (1.0f - Vi) - a * a / 3.0f);
-      h.C3 = a * a * h.C1;
-      h.C4 = a * h.C1;
-      h.fa = h.C2 / h.C1;
-      h.fb = h.C3 / h.C1;
-      h.fc = h.C4 / h.C1;
-      h.sq2sin = a;
-    }
-  }
   d.npm = p.npm; d.np_delete_m = p.np_delete_m;
   d.time = p.time; d.dt = p.dt; d.time_d = p.time; d.step = 0;
-  for (int k = 0; k < LQ_RIGID_MOTION_MAX; k++) {
-    const lq_rigid_motion& m = p.mv[k];
-    DevMotion& o = d.mv[k];
-    o.is_trans = m.is_trans; o.is_rotat = m.is_rotat; o.pattern = m.pattern;
-    o.trans_vel = make_float3(m.trans_vel[0], m.trans_vel[1], m.trans_vel[2]);
-    o.trans_acc = make_float3(m.trans_acc[0], m.trans_acc[1], m.trans_acc[2]);
-    o.pos_rp = make_float3(m.pos_rp[0], m.pos_rp[1], m.pos_rp[2]);
-    o.rotat_vel = make_float3(m.rotat_vel[0], m.rotat_vel[1], m.rotat_vel[2]);
-    o.rotat_acc = make_float3(m.rotat_acc[0], m.rotat_acc[1], m.rotat_acc[2]);
-  }
   int maxkey = p.ncell_delete_m, nb = 0;
   while ((1LL << nb) <= maxkey) nb++;
   d.nbits = nb;
 }
 
+// constitutive_model.h:226-230, hoisted per material (float arithmetic).
+static HypoConst hypo_const(const float* m) {
+  HypoConst h{};
+  const float Ei = m[0], Vi = m[1], phic = m[2], pref = m[4];
+  if (pref != 0.f && phic != 0.f && Vi != 1.f) {
+    h.C1 = -Ei / (3.0f * pref * (1.0f + Vi));
+    const float a = 3.0f / (sqrtf(2.0f) * sinf(phic));
+    h.C2 = h.C1 * ((3.0f * Vi + a * sqrtf(1.0f + Vi * Vi)) / (1.0f - Vi) - a * a / 3.0f);
+    h.C3 = a * a * h.C1;
+    h.C4 = a * h.C1;
+    h.fa = h.C2 / h.C1;
+    h.fb = h.C3 / h.C1;
+    h.fc = h.C4 / h.C1;
+    h.sq2sin = a;
+  }
+  return h;
+}
+
+static DevMotion motion_to_dev(const lq_rigid_motion& m) {
+  DevMotion o{};
+  o.is_trans = m.is_trans; o.is_rotat = m.is_rotat; o.pattern = m.pattern;
+  o.trans_vel = make_float3(m.trans_vel[0], m.trans_vel[1], m.trans_vel[2]);
+  o.trans_acc = make_float3(m.trans_acc[0], m.trans_acc[1], m.trans_acc[2]);
+  o.pos_rp = make_float3(m.pos_rp[0], m.pos_rp[1], m.pos_rp[2]);
+  o.rotat_vel = make_float3(m.rotat_vel[0], m.rotat_vel[1], m.rotat_vel[2]);
+  o.rotat_acc = make_float3(m.rotat_acc[0], m.rotat_acc[1], m.rotat_acc[2]);
+  return o;
+}
+
+// Cases of a batch must share every parameter the step reads globally.
+static bool same_physics(const lq_params& a, const lq_params& b, std::string& why) {
+  auto chk = [&](bool ok, const char* f) { if (!ok && why.empty()) why = f; return ok; };
+  bool ok = true;
+  ok &= chk(a.dim == b.dim, "dim");
+  ok &= chk(a.dr == b.dr && a.hdr == b.hdr && a.h == b.h && a.i_h == b.i_h, "dr/hdr");
+  ok &= chk(a.vol_def == b.vol_def && a.bound_thickness == b.bound_thickness, "vol_def/bound_thickness");
+  ok &= chk(a.gravity[0] == b.gravity[0] && a.gravity[1] == b.gravity[1] && a.gravity[2] == b.gravity[2], "gravity");
+  ok &= chk(a.damping == b.damping, "damping");
+  ok &= chk(a.kernel_normalization_par == b.kernel_normalization_par &&
+                a.kernel_gradient_normalization_par == b.kernel_gradient_normalization_par &&
+                a.kernel_zero == b.kernel_zero && a.kernel_dr == b.kernel_dr, "kernel constants");
+  ok &= chk(a.stress_shepard_filter == b.stress_shepard_filter, "stress_shepard_filter");
+  ok &= chk(a.viscosity_type == b.viscosity_type && a.artificial_viscosity_alpha_m == b.artificial_viscosity_alpha_m,
+            "viscosity");
+  ok &= chk(a.cs_m == b.cs_m && a.eta == b.eta && a.cfl == b.cfl, "cs/eta/cfl");
+  ok &= chk(a.artificial_pressure_coefficient_material == b.artificial_pressure_coefficient_material,
+            "artificial pressure");
+  ok &= chk(a.time == b.time && a.dt == b.dt, "time/dt");
+  ok &= chk(a.key_mode == b.key_mode, "key_mode");
+  return ok;
+}
+
 static int alloc_arrays(lq_ctx* c, Arrays& A, size_t n) {
   int st = 0;
   st |= dalloc(c, &A.codes, n); st |= dalloc(c, &A.cellpos, n); st |= dalloc(c, &A.orig_id, n);
+  st |= dalloc(c, &A.case_id, n);
   st |= dalloc(c, &A.pres, n);
   st |= dalloc(c, &A.pos, n); st |= dalloc(c, &A.pos_ori, n); st |= dalloc(c, &A.vel, n);
   st |= dalloc(c, &A.idm, n); st |= dalloc(c, &A.normal, n);
@@ -276,29 +313,107 @@ static int alloc_arrays(lq_ctx* c, Arrays& A, size_t n) {
   return st;
 }
 
-extern "C" int lq_create(const lq_params* par, int device, lq_ctx** out) {
-  if (!par || !out) return LQ_ERR_ARG;
+// Places the cases in one global grid: rows of cell blocks along x with one
+// empty cell between neighbours in x and y, all starting at z = 0.  A
+// single case keeps its own grid unchanged.
+static void layout_cases(lq_ctx* c, int gdim[3]) {
+  const int n = c->n_cases;
+  if (n == 1) {
+    const lq_params& p = c->case_par[0];
+    c->cases[0] = DevCase{0, 0, 0, p.cell_num[0], p.cell_num[1], p.cell_num[2], 0, 0};
+    for (int d = 0; d < 3; d++) gdim[d] = p.cell_num[d];
+    return;
+  }
+  double area = 0;
+  for (const lq_params& p : c->case_par) area += double(p.cell_num[0] + 1) * double(p.cell_num[1] + 1);
+  const int row_w = std::max(1, int(std::ceil(std::sqrt(area))));
+  int x = 0, y = 0, row_h = 0, gx = 0, gz = 0;
+  for (int k = 0; k < n; k++) {
+    const lq_params& p = c->case_par[k];
+    if (x > 0 && x + p.cell_num[0] > row_w) { y += row_h + 1; x = 0; row_h = 0; }
+    DevCase& d = c->cases[k];
+    d.ox = x; d.oy = y; d.oz = 0;
+    d.nx = p.cell_num[0]; d.ny = p.cell_num[1]; d.nz = p.cell_num[2];
+    x += p.cell_num[0] + 1;
+    row_h = std::max(row_h, p.cell_num[1]);
+    gx = std::max(gx, x - 1);
+    gz = std::max(gz, p.cell_num[2]);
+  }
+  gdim[0] = gx; gdim[1] = y + row_h; gdim[2] = gz;
+}
+
+static int create_impl(const lq_params* cases, int n_cases, int device, lq_ctx** out) {
+  if (!cases || !out || n_cases < 1) return LQ_ERR_ARG;
   *out = nullptr;
-  if (par->np <= 0 || par->npb < 0 || par->npm < 0 || par->np != par->npb + par->npm)
-    return LQ_ERR_ARG;  // loquat_io.cu:148-151
-  if (par->dim != 3 || par->periodbound) return LQ_ERR_ARG;
-  if (par->key_mode != LQ_KEY_LINEAR) return LQ_ERR_ARG;
-  if (par->n_motion < 0 || par->n_motion > LQ_RIGID_MOTION_MAX) return LQ_ERR_ARG;
-  if (par->np >= (1 << 26)) return LQ_ERR_ARG;  // neighbour-list slot encoding
+  long long np_tot = 0;
+  for (int k = 0; k < n_cases; k++) {
+    const lq_params& p = cases[k];
+    if (p.np <= 0 || p.npb < 0 || p.npm < 0 || p.np != p.npb + p.npm) return LQ_ERR_ARG;  // loquat_io.cu:148-151
+    if (p.dim != 3 || p.periodbound) return LQ_ERR_ARG;
+    if (p.key_mode != LQ_KEY_LINEAR) return LQ_ERR_ARG;
+    if (p.n_motion < 0 || p.n_motion > LQ_RIGID_MOTION_MAX) return LQ_ERR_ARG;
+    np_tot += p.np;
+  }
+  if (np_tot >= (1 << 26)) return LQ_ERR_ARG;  // neighbour-list slot encoding
   lq_ctx* c = new lq_ctx();
   c->device = device;
-  c->par = *par;
+  c->n_cases = n_cases;
+  c->case_par.assign(cases, cases + n_cases);
   auto bail = [&](int st) { *out = c; return st; };
+  for (int k = 1; k < n_cases; k++) {
+    std::string why;
+    if (!same_physics(cases[0], cases[k], why))
+      return bail(fail(c, LQ_ERR_ARG, "lq_create_batch: case " + std::to_string(k) + " differs in " + why));
+  }
+  // combined parameters: case 0's physics, summed counts, the global grid
+  c->cases.assign(n_cases, DevCase{});
+  int gdim[3];
+  layout_cases(c, gdim);
+  lq_params par = cases[0];
+  par.np = par.npb = par.npm = par.np_delete_m = 0;
+  par.n_motion = 0;
+  c->bnd_base.resize(n_cases);
+  c->soil_base.resize(n_cases);
+  for (int k = 0; k < n_cases; k++) {
+    c->bnd_base[k] = par.npb;
+    c->soil_base[k] = par.npm;
+    c->cases[k].model_base = LQ_MODEL_NUMBER_MAX * k;
+    c->cases[k].motion_base = par.n_motion;
+    par.np += cases[k].np; par.npb += cases[k].npb; par.npm += cases[k].npm;
+    par.n_motion += cases[k].n_motion;
+  }
+  for (int d = 0; d < 3; d++) par.cell_num[d] = gdim[d];
+  if (n_cases > 1) {
+    const long long layer = (long long)gdim[0] * gdim[1] * gdim[2];
+    if (2 * layer + 1 >= (1LL << 31)) return bail(fail(c, LQ_ERR_ARG, "lq_create_batch: grid too large"));
+    par.ncell_material = int(layer);
+    par.ncell_delete_m = int(2 * layer);
+    par.ncell = int(2 * layer + 1);
+  }
+  c->par = par;
+  c->case_uploaded.assign(n_cases, 0);
   hipError_t e = hipSetDevice(device);
   if (e != hipSuccess) { c->err = std::string("hipSetDevice: ") + hipGetErrorString(e); return bail(LQ_ERR_HIP); }
   e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { c->err = std::string("hipStreamCreate: ") + hipGetErrorString(e); return bail(LQ_ERR_HIP); }
-  fill_static(*par, c->hp);
+  fill_static(par, c->hp);
+  c->hp.n_cases = n_cases;
+  c->hp.n_model = LQ_MODEL_NUMBER_MAX * n_cases;
+  c->hp.n_motion = par.n_motion;
   c->nbits = c->hp.nbits;
-  c->npm0 = par->npm;
-  const size_t n = size_t(par->np);
+  c->npm0 = par.npm;
+  const size_t n = size_t(par.np);
   int st = 0;
   st |= dalloc(c, &c->dP, 1);
+  // material, motion and case tables
+  c->n_model_rows = LQ_MODEL_NUMBER_MAX * n_cases;
+  c->n_motion_rows = std::max(1, par.n_motion);
+  st |= dalloc(c, &c->d_model, size_t(c->n_model_rows));
+  st |= dalloc(c, &c->d_hypo, size_t(c->n_model_rows));
+  st |= dalloc(c, &c->d_mv, size_t(c->n_motion_rows));
+  st |= dalloc(c, &c->d_tab, size_t(c->n_motion_rows));
+  st |= dalloc(c, &c->d_cases, size_t(n_cases));
+  st |= dalloc(c, &c->d_boxes, size_t(3 * n_cases));
   st |= alloc_arrays(c, c->B.cur, n);
   st |= alloc_arrays(c, c->B.alt, n);
   Buffers& B = c->B;
@@ -313,28 +428,53 @@ extern "C" int lq_create(const lq_params* par, int device, lq_ctx** out) {
   st |= dalloc(c, &B.s3, n); st |= dalloc(c, &B.fdef, n); st |= dalloc(c, &B.deferred, n);
   st |= dalloc(c, &B.str_tmp_a, n); st |= dalloc(c, &B.str_tmp_b, n);
   st |= dalloc(c, &B.sr_a, n); st |= dalloc(c, &B.sr_b, n);
-  if (par->np < (1 << 25)) {  // pair-list entries hold the slot in 25 bits
-    st |= dalloc(c, &B.plist, size_t(kPairListCap) * size_t(std::max(1, par->npm)));
-    st |= dalloc(c, &B.pcount, size_t(std::max(1, par->npm)));
+  if (par.np < (1 << 25)) {  // pair-list entries hold the slot in 25 bits
+    st |= dalloc(c, &B.plist, size_t(kPairListCap) * size_t(std::max(1, par.npm)));
+    st |= dalloc(c, &B.pcount, size_t(std::max(1, par.npm)));
   }
   st |= dalloc(c, &B.keys, n); st |= dalloc(c, &B.vals, n);
   st |= dalloc(c, &B.keys_alt, n); st |= dalloc(c, &B.vals_alt, n);
-  st |= dalloc(c, &B.cell_beg, size_t(par->ncell)); st |= dalloc(c, &B.cell_end, size_t(par->ncell));
-  st |= dalloc(c, &B.wall_near, size_t(std::max(1, par->ncell_material)));
-  st |= dalloc(c, &B.hist, size_t(256) * radix_tiles(par->np) + size_t(radix_tiles(par->np)) + 1);  // + scan tile sums
-  c->wf_blocks = std::max(1, std::min(1024, (par->np + 255) / 256));
-  st |= dalloc(c, &B.partial, size_t(c->wf_blocks));
-  st |= dalloc(c, &c->wf_out, 1);
+  st |= dalloc(c, &B.cell_beg, size_t(par.ncell)); st |= dalloc(c, &B.cell_end, size_t(par.ncell));
+  st |= dalloc(c, &B.wall_near, size_t(std::max(1, par.ncell_material)));
+  st |= dalloc(c, &B.hist, size_t(256) * radix_tiles(par.np) + size_t(radix_tiles(par.np)) + 1);  // + scan tile sums
+  st |= dalloc(c, &B.wf_acc, size_t(2 * n_cases));
   if (st) return bail(LQ_ERR_HIP);
   B.part_cel = B.keys;
   B.part_idx = B.vals;
-  if (hipHostMalloc((void**)&c->wf_host, sizeof(double)) != hipSuccess) return bail(fail(c, LQ_ERR_HIP, "hipHostMalloc"));
+  if (hipHostMalloc((void**)&c->wf_host, sizeof(unsigned long long) * 2 * n_cases) != hipSuccess)
+    return bail(fail(c, LQ_ERR_HIP, "hipHostMalloc"));
+  std::vector<float> mrows(size_t(c->n_model_rows) * LQ_MATERIAL_SIZE_MAX);
+  std::vector<HypoConst> hrows(c->n_model_rows);
+  std::vector<DevMotion> mvrows(c->n_motion_rows);
+  for (int k = 0; k < n_cases; k++) {
+    for (int m = 0; m < LQ_MODEL_NUMBER_MAX; m++) {
+      const int r = LQ_MODEL_NUMBER_MAX * k + m;
+      for (int j = 0; j < LQ_MATERIAL_SIZE_MAX; j++) mrows[size_t(r) * LQ_MATERIAL_SIZE_MAX + j] = cases[k].model_par[m][j];
+      hrows[r] = hypo_const(cases[k].model_par[m]);
+    }
+    for (int m = 0; m < cases[k].n_motion; m++) mvrows[c->cases[k].motion_base + m] = motion_to_dev(cases[k].mv[m]);
+  }
+  HIPCHECK(c, hipMemcpyAsync(c->d_model, mrows.data(), mrows.size() * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(c->d_hypo, hrows.data(), hrows.size() * sizeof(HypoConst), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(c->d_mv, mvrows.data(), mvrows.size() * sizeof(DevMotion), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(c->d_cases, c->cases.data(), c->cases.size() * sizeof(DevCase), hipMemcpyHostToDevice,
+                             c->stream));
+  c->hp.model_par = c->d_model;
+  c->hp.hypo = c->d_hypo;
+  c->hp.cases = c->d_cases;
+  c->hp.mv = c->d_mv;
+  c->hp.tab = c->d_tab;
   HIPCHECK(c, hipMemcpyAsync(c->dP, &c->hp, sizeof(DevParams), hipMemcpyHostToDevice, c->stream));
-  launch_fill_int(c->stream, B.cell_beg, par->ncell, -1);
-  launch_fill_int(c->stream, B.cell_end, par->ncell, -1);
+  launch_fill_int(c->stream, B.cell_beg, par.ncell, -1);
+  launch_fill_int(c->stream, B.cell_end, par.ncell, -1);
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   *out = c;
   return LQ_OK;
+}
+
+extern "C" int lq_create(const lq_params* par, int device, lq_ctx** out) { return create_impl(par, 1, device, out); }
+extern "C" int lq_create_batch(const lq_params* cases, int n_cases, int device, lq_ctx** out) {
+  return create_impl(cases, n_cases, device, out);
 }
 
 extern "C" void lq_destroy(lq_ctx* c) {
@@ -363,50 +503,81 @@ template <class T> static int d2h(lq_ctx* c, std::vector<T>& v, const T* src, si
   return 0;
 }
 
-// CopyDataFromHostToDevice (loquat.cu:223-234) + PositionFromAbsolute2Relative.
-extern "C" int lq_upload(lq_ctx* c, const lq_host_state* h) {
+// CopyDataFromHostToDevice (loquat.cu:223-234) + PositionFromAbsolute2Relative
+// for case k: its wall particles go to slots bnd_base[k].., its soil to
+// npb + soil_base[k].., in .dat order, so after the first stable sort each
+// case's slots are in the order a context of that case alone would hold.
+// codes.w is moved to the case's rows of the material (soil) and motion
+// (wall) tables.
+extern "C" int lq_upload_case(lq_ctx* c, int k, const lq_host_state* h) {
   if (!c || !h || !h->codes || !h->pospres || !h->velrhop || !h->idmassidx || !h->str || !h->stn)
     return fail(c, LQ_ERR_ARG, "lq_upload: null buffer");
-  if (c->par.npb > 0 && !h->normal) return fail(c, LQ_ERR_ARG, "lq_upload: normal required when npb > 0");
+  if (k < 0 || k >= c->n_cases) return fail(c, LQ_ERR_ARG, "lq_upload_case: case out of range");
+  const lq_params& cp = c->case_par[k];
+  if (cp.npb > 0 && !h->normal) return fail(c, LQ_ERR_ARG, "lq_upload: normal required when npb > 0");
   hipSetDevice(c->device);
-  const int np = c->par.np, npb = c->par.npb;
+  const int np = cp.np, npb = cp.npb, npm = cp.npm;
+  const DevCase& dc = c->cases[k];
   std::vector<int4> codes(np);
-  std::vector<int> oid(np);
+  std::vector<int> oid(np), cid(np, k);
   std::vector<float> pres(np);
   std::vector<float4> pos(np), vel(np), idm(np), nrm(np), sa(np), na(np);
   std::vector<float2> sb(np), nb(np);
   for (int i = 0; i < np; i++) {
-    codes[i] = make_int4(h->codes[4 * i], h->codes[4 * i + 1], h->codes[4 * i + 2], h->codes[4 * i + 3]);
+    const int* cd = h->codes + 4 * i;
+    const int base = i < npb ? dc.motion_base : dc.model_base;
+    codes[i] = make_int4(cd[0], cd[1], cd[2], cd[3] + base);
+    if (i < npb && cd[1] == LQ_NONFIX && (cd[3] < 0 || cd[3] >= cp.n_motion))
+      return fail(c, LQ_ERR_ARG, "lq_upload: wall particle with a motion id outside the case's motions");
+    if (i >= npb && (cd[3] < 0 || cd[3] >= LQ_MODEL_NUMBER_MAX))
+      return fail(c, LQ_ERR_ARG, "lq_upload: soil particle with a material id outside 0..9");
     pos[i] = make_float4(h->pospres[4 * i], h->pospres[4 * i + 1], h->pospres[4 * i + 2], h->pospres[4 * i + 3]);
     vel[i] = make_float4(h->velrhop[4 * i], h->velrhop[4 * i + 1], h->velrhop[4 * i + 2], h->velrhop[4 * i + 3]);
     // idmassidx.z = line index (loquat_io.cu:240,253)
     idm[i] = make_float4(h->idmassidx[4 * i], h->idmassidx[4 * i + 1], float(i), h->idmassidx[4 * i + 3]);
-    const float* s = h->str + 6 * i;
-    const float* t = h->stn + 6 * i;
-    sa[i] = make_float4(s[0], s[1], s[2], s[3]);
-    sb[i] = make_float2(s[4], s[5]);
-    na[i] = make_float4(t[0], t[1], t[2], t[3]);
-    nb[i] = make_float2(t[4], t[5]);
+    const float* sv = h->str + 6 * i;
+    const float* tv = h->stn + 6 * i;
+    sa[i] = make_float4(sv[0], sv[1], sv[2], sv[3]);
+    sb[i] = make_float2(sv[4], sv[5]);
+    na[i] = make_float4(tv[0], tv[1], tv[2], tv[3]);
+    nb[i] = make_float2(tv[4], tv[5]);
     nrm[i] = i < npb ? make_float4(h->normal[4 * i], h->normal[4 * i + 1], h->normal[4 * i + 2], h->normal[4 * i + 3])
                      : make_float4(0.f, 0.f, 0.f, 0.f);
     oid[i] = i;
     pres[i] = h->pospres[4 * i + 3];
   }
   Arrays& A = c->B.cur;
+  const size_t b0 = size_t(c->bnd_base[k]), s0 = size_t(c->par.npb) + size_t(c->soil_base[k]);
+  // wall rows [0, npb) to b0.., soil rows [npb, np) to s0..
+  auto put = [&](auto* dst, const auto& v) -> int {
+    using T = typename std::decay<decltype(v[0])>::type;
+    if (npb > 0) HIPCHECK(c, hipMemcpyAsync(dst + b0, v.data(), size_t(npb) * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    if (npm > 0)
+      HIPCHECK(c, hipMemcpyAsync(dst + s0, v.data() + npb, size_t(npm) * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    return 0;
+  };
   int st = 0;
-  st |= h2d(c, A.codes, codes); st |= h2d(c, A.pos, pos); st |= h2d(c, A.pos_ori, pos);
-  st |= h2d(c, A.vel, vel); st |= h2d(c, A.idm, idm); st |= h2d(c, A.normal, nrm);
-  st |= h2d(c, A.str_a, sa); st |= h2d(c, A.str_b, sb); st |= h2d(c, A.stn_a, na); st |= h2d(c, A.stn_b, nb);
-  st |= h2d(c, A.orig_id, oid); st |= h2d(c, A.pres, pres);
+  st |= put(A.codes, codes); st |= put(A.pos, pos); st |= put(A.pos_ori, pos);
+  st |= put(A.vel, vel); st |= put(A.idm, idm); st |= put(A.normal, nrm);
+  st |= put(A.str_a, sa); st |= put(A.str_b, sb); st |= put(A.stn_a, na); st |= put(A.stn_b, nb);
+  st |= put(A.orig_id, oid); st |= put(A.pres, pres); st |= put(A.case_id, cid);
   if (st) return st;
-  const float scale = 0.5f * c->par.i_h;
-  launch_abs2rel(c->stream, c->dP, np, A, scale);
-  launch_set_massflag(c->stream, c->dP, np, A);
+  const float scale = 0.5f * cp.i_h;
+  launch_abs2rel(c->stream, c->dP, A, scale, int(b0), npb);
+  launch_abs2rel(c->stream, c->dP, A, scale, int(s0), npm);
+  launch_set_massflag(c->stream, c->dP, c->par.np, A);
   HIPCHECK(c, hipGetLastError());
   HIPCHECK(c, hipStreamSynchronize(c->stream));
-  c->uploaded = true;
+  if (!c->case_uploaded[k]) { c->case_uploaded[k] = 1; c->n_uploaded++; }
+  c->uploaded = c->n_uploaded == c->n_cases;
   c->layout_version++;
   return LQ_OK;
+}
+
+extern "C" int lq_upload(lq_ctx* c, const lq_host_state* h) {
+  if (!c) return LQ_ERR_ARG;
+  if (c->n_cases != 1) return fail(c, LQ_ERR_ARG, "lq_upload: batched context, use lq_upload_case");
+  return lq_upload_case(c, 0, h);
 }
 
 static void tab_to_dev(const lq_motion_table& a, DevTable& m) {
@@ -444,24 +615,36 @@ static int pull_params(lq_ctx* c, DevParams& d) {
   return 0;
 }
 
-extern "C" int lq_set_motion_tables(lq_ctx* c, int n, const lq_motion_table* t) {
-  if (!c || n < 0 || n > LQ_RIGID_MOTION_MAX || (n > 0 && !t)) return fail(c, LQ_ERR_ARG, "lq_set_motion_tables");
+// Motion tables of case k (MotionInfoALL, loquat_io.cu:266-303), rows
+// motion_base.. of the device table.
+extern "C" int lq_set_case_motion_tables(lq_ctx* c, int k, int n, const lq_motion_table* t) {
+  if (!c || k < 0 || k >= c->n_cases || n < 0 || n > LQ_RIGID_MOTION_MAX || (n > 0 && !t))
+    return fail(c, LQ_ERR_ARG, "lq_set_motion_tables");
+  if (n > c->case_par[k].n_motion && n > 0)
+    return fail(c, LQ_ERR_ARG, "lq_set_motion_tables: more tables than the case's motions");
   hipSetDevice(c->device);
-  DevTable tabs[LQ_RIGID_MOTION_MAX];
-  for (int k = 0; k < n; k++) tab_to_dev(t[k], tabs[k]);
-  size_t off = offsetof(DevParams, tab);
-  HIPCHECK(c, hipMemcpyAsync((char*)c->dP + off, tabs, sizeof(DevTable) * n, hipMemcpyHostToDevice, c->stream));
+  std::vector<DevTable> tabs(std::max(n, 1));
+  for (int j = 0; j < n; j++) tab_to_dev(t[j], tabs[j]);
+  if (n > 0)
+    HIPCHECK(c, hipMemcpyAsync(c->d_tab + c->cases[k].motion_base, tabs.data(), sizeof(DevTable) * n,
+                               hipMemcpyHostToDevice, c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   return LQ_OK;
 }
+extern "C" int lq_set_motion_tables(lq_ctx* c, int n, const lq_motion_table* t) {
+  if (!c) return LQ_ERR_ARG;
+  if (c->n_cases != 1) return fail(c, LQ_ERR_ARG, "lq_set_motion_tables: batched context, use the case form");
+  return lq_set_case_motion_tables(c, 0, n, t);
+}
 
 extern "C" int lq_get_motion_tables(lq_ctx* c, int n, lq_motion_table* t) {
-  if (!c || n < 0 || n > LQ_RIGID_MOTION_MAX || (n > 0 && !t)) return fail(c, LQ_ERR_ARG, "lq_get_motion_tables");
+  if (!c || n < 0 || n > LQ_RIGID_MOTION_MAX || n > c->n_motion_rows || (n > 0 && !t))
+    return fail(c, LQ_ERR_ARG, "lq_get_motion_tables");
   hipSetDevice(c->device);
-  DevParams d;
-  int st = pull_params(c, d);
-  if (st) return st;
-  for (int k = 0; k < n; k++) tab_from_dev(d.tab[k], t[k]);
+  std::vector<DevTable> tabs(std::max(n, 1));
+  if (n > 0) HIPCHECK(c, hipMemcpyAsync(tabs.data(), c->d_tab, sizeof(DevTable) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  for (int k = 0; k < n; k++) tab_from_dev(tabs[k], t[k]);
   return LQ_OK;
 }
 
@@ -474,8 +657,12 @@ extern "C" int lq_get_params(lq_ctx* c, lq_params* out) {
   *out = c->par;
   out->npm = d.npm; out->np_delete_m = d.np_delete_m;
   out->time = d.time; out->dt = d.dt;
-  for (int k = 0; k < LQ_RIGID_MOTION_MAX; k++) {
-    const DevMotion& m = d.mv[k];
+  const int nm = std::min(LQ_RIGID_MOTION_MAX, c->n_motion_rows);
+  std::vector<DevMotion> mvs(nm);
+  HIPCHECK(c, hipMemcpyAsync(mvs.data(), c->d_mv, sizeof(DevMotion) * nm, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  for (int k = 0; k < nm; k++) {
+    const DevMotion& m = mvs[k];
     lq_rigid_motion& o = out->mv[k];
     o.trans_vel[0] = m.trans_vel.x; o.trans_vel[1] = m.trans_vel.y; o.trans_vel[2] = m.trans_vel.z;
     o.trans_acc[0] = m.trans_acc.x; o.trans_acc[1] = m.trans_acc.y; o.trans_acc[2] = m.trans_acc.z;
@@ -616,6 +803,7 @@ extern "C" int lq_get_field(lq_ctx* c, int f, void* dst, size_t bytes) {
 
 extern "C" int lq_set_field(lq_ctx* c, int f, const void* src, size_t bytes) {
   if (!c || !src) return fail(c, LQ_ERR_ARG, "lq_set_field: null");
+  if (c->n_cases != 1) return fail(c, LQ_ERR_ARG, "lq_set_field: not available on batched contexts");
   FieldView v = field_view(c, f);
   if (v.width == 0) return fail(c, LQ_ERR_ARG, "lq_set_field: unknown field");
   if (bytes != v.rows * v.width * 4) return fail(c, LQ_ERR_ARG, "lq_set_field: size mismatch");
@@ -666,6 +854,90 @@ extern "C" int lq_set_field(lq_ctx* c, int f, const void* src, size_t bytes) {
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   return LQ_OK;
 }
+
+// ------------------------------------------------------------- batches ----
+static int case_rows(lq_ctx* c, int k, std::vector<int>& rows) {
+  std::vector<int> cid;
+  int st = d2h(c, cid, c->B.cur.case_id, size_t(c->par.np));
+  if (st) return st;
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  rows.clear();
+  for (int i = 0; i < c->par.np; i++)
+    if (cid[i] == k) rows.push_back(i);
+  return 0;
+}
+
+// One case's rows of a per-slot field, in slot order (the order a context of
+// that case alone holds), with codes.w and cellpos in the case's own frame.
+extern "C" int lq_get_case_field(lq_ctx* c, int k, int f, void* dst, size_t bytes) {
+  if (!c || !dst) return fail(c, LQ_ERR_ARG, "lq_get_case_field: null");
+  if (k < 0 || k >= c->n_cases) return fail(c, LQ_ERR_ARG, "lq_get_case_field: case out of range");
+  if (f == LQ_F_PART_CEL || f == LQ_F_PART_IDX || f == LQ_F_CELL_BEG || f == LQ_F_CELL_END)
+    return fail(c, LQ_ERR_ARG, "lq_get_case_field: grid arrays are global");
+  FieldView v = field_view(c, f);
+  if (v.width == 0) return fail(c, LQ_ERR_ARG, "lq_get_case_field: unknown field");
+  const lq_params& cp = c->case_par[k];
+  if (bytes != size_t(cp.np) * v.width * 4) return fail(c, LQ_ERR_ARG, "lq_get_case_field: size mismatch");
+  std::vector<uint32_t> all(size_t(c->par.np) * v.width);
+  int st = lq_get_field(c, f, all.data(), all.size() * 4);
+  if (st) return st;
+  std::vector<int> rows;
+  if ((st = case_rows(c, k, rows))) return st;
+  if (int(rows.size()) != cp.np) return fail(c, LQ_ERR_STATE, "lq_get_case_field: case slot count changed");
+  uint32_t* o = static_cast<uint32_t*>(dst);
+  const DevCase& dc = c->cases[k];
+  const int lnx = dc.nx, lny = dc.ny, lnz = dc.nz, lncm = lnx * lny * lnz;
+  for (size_t r = 0; r < rows.size(); r++) {
+    const size_t i = size_t(rows[r]);
+    std::memcpy(o + r * v.width, all.data() + i * v.width, size_t(v.width) * 4);
+    int32_t* q = reinterpret_cast<int32_t*>(o + r * v.width);
+    if (f == LQ_F_CODES) {
+      q[3] -= int(i) < c->par.npb ? dc.motion_base : dc.model_base;
+    } else if (f == LQ_F_CELLPOS) {
+      const int key = q[3];
+      q[0] -= dc.ox; q[1] -= dc.oy; q[2] -= dc.oz;
+      const int lin = q[0] * lny * lnz + q[1] * lnz + q[2];  // loquat_grid.cu:88
+      q[3] = key == c->par.ncell_delete_m ? 2 * lncm : (key >= c->par.ncell_material ? lncm + lin : lin);
+    }
+  }
+  return LQ_OK;
+}
+
+// Absolute positions (x y z p) of case k in its own frame, slot order.
+extern "C" int lq_case_positions_absolute(lq_ctx* c, int k, float* dst, size_t bytes) {
+  if (!c || !dst) return fail(c, LQ_ERR_ARG, "lq_case_positions_absolute: null");
+  if (k < 0 || k >= c->n_cases) return fail(c, LQ_ERR_ARG, "lq_case_positions_absolute: case out of range");
+  if (bytes != size_t(c->case_par[k].np) * 16) return fail(c, LQ_ERR_ARG, "lq_case_positions_absolute: size");
+  std::vector<float> all(size_t(c->par.np) * 4);
+  int st = lq_positions_absolute(c, all.data(), all.size() * 4);
+  if (st) return st;
+  std::vector<int> rows;
+  if ((st = case_rows(c, k, rows))) return st;
+  for (size_t r = 0; r < rows.size(); r++) std::memcpy(dst + 4 * r, all.data() + 4 * size_t(rows[r]), 16);
+  return LQ_OK;
+}
+
+// Wall, live soil and deleted particle counts of case k (synchronises).
+extern "C" int lq_case_counts(lq_ctx* c, int k, int32_t* npb, int32_t* npm, int32_t* ndel) {
+  if (!c || k < 0 || k >= c->n_cases) return fail(c, LQ_ERR_ARG, "lq_case_counts");
+  int32_t gb = 0, gm = 0, gd = 0;
+  int st = lq_get_counts(c, &gb, &gm, &gd);
+  if (st) return st;
+  std::vector<int> rows;
+  if ((st = case_rows(c, k, rows))) return st;
+  int32_t b = 0, m = 0, d = 0;
+  for (int i : rows) {
+    if (i < gb) b++;
+    else if (i < gb + gm) m++;
+    else d++;
+  }
+  if (npb) *npb = b;
+  if (npm) *npm = m;
+  if (ndel) *ndel = d;
+  return LQ_OK;
+}
+
+extern "C" int lq_case_count(const lq_ctx* c) { return c ? c->n_cases : 0; }
 
 // ------------------------------------------------------------ step pieces --
 #define NEED_UPLOAD(c) \
@@ -864,16 +1136,31 @@ extern "C" int lq_get_counts(lq_ctx* c, int32_t* npb, int32_t* npm, int32_t* nde
   return LQ_OK;
 }
 
-extern "C" int lq_wall_force(lq_ctx* c, const float box[3], double* F) {
+// Earth-pressure resultant of every case (GetValuesAtPoint,
+// utilities.cu:578-640): boxes[3k..3k+2] = case k's probe x_max, z_min,
+// z_max in its own frame; F[k] = 0.001 * (exact sum of contact_mag.xx).
+extern "C" int lq_wall_force_cases(lq_ctx* c, const float* boxes, double* F) {
   NEED_UPLOAD(c);
-  if (!box || !F) return fail(c, LQ_ERR_ARG, "lq_wall_force: null");
+  if (!boxes || !F) return fail(c, LQ_ERR_ARG, "lq_wall_force: null");
   hipSetDevice(c->device);
-  launch_wall_force(c->stream, c->dP, c->par.np, c->B.cur, box, c->B.partial, c->wf_blocks, c->wf_out);
+  const int n = c->n_cases;
+  HIPCHECK(c, hipMemcpyAsync(c->d_boxes, boxes, sizeof(float) * 3 * n, hipMemcpyHostToDevice, c->stream));
+  launch_wall_force(c->stream, c->dP, c->par.np, n, c->B.cur, c->d_boxes, c->B.wf_acc);
   HIPCHECK(c, hipGetLastError());
-  HIPCHECK(c, hipMemcpyAsync(c->wf_host, c->wf_out, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(c->wf_host, c->B.wf_acc, sizeof(unsigned long long) * 2 * n, hipMemcpyDeviceToHost,
+                             c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
-  *F = *c->wf_host;
+  for (int k = 0; k < n; k++) {
+    const __int128 v = (__int128)((unsigned __int128)c->wf_host[2 * k + 1] << 64 | c->wf_host[2 * k]);
+    F[k] = 0.001 * std::ldexp(double(v), -64);
+  }
   return LQ_OK;
+}
+
+extern "C" int lq_wall_force(lq_ctx* c, const float box[3], double* F) {
+  if (!c) return LQ_ERR_ARG;
+  if (c->n_cases != 1) return fail(c, LQ_ERR_ARG, "lq_wall_force: batched context, use lq_wall_force_cases");
+  return lq_wall_force_cases(c, box, F);
 }
 
 extern "C" int lq_positions_absolute(lq_ctx* c, float* dst, size_t bytes) {

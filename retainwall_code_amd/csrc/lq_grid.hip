@@ -227,6 +227,7 @@ __global__ void __launch_bounds__(256) k_gather(const DevParams* __restrict__ P,
   dst.codes[i] = src.codes[s];
   dst.cellpos[i] = src.cellpos[s];
   dst.orig_id[i] = src.orig_id[s];
+  dst.case_id[i] = src.case_id[s];
   dst.pres[i] = src.pres[s];
   dst.pos[i] = src.pos[s];
   dst.pos_ori[i] = src.pos_ori[s];

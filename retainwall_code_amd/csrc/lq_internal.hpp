@@ -41,6 +41,15 @@ struct HypoConst {
   float C1, C2, C3, C4, fa, fb, fc, sq2sin;
 };
 
+// One case of a batched context.  Cases sit in one global cell grid as
+// cell-aligned blocks separated by at least one empty cell, so no candidate
+// range ever reaches another case.  off = the case's first cell, n = its own
+// grid (out-of-grid deletion and the domain check use it), and its
+// materials and motions start at model_base / motion_base in the tables.
+struct DevCase {
+  int ox, oy, oz, nx, ny, nz, model_base, motion_base;
+};
+
 struct DevParams {
   // ---- static (set at create) ----
   int np, npb, cnx, cny, cnz, ncell, ncell_material, ncell_delete_m;
@@ -53,8 +62,10 @@ struct DevParams {
   // (entries per soil slot) and column stride (soil slots)
   float r2list, disp2_lim;
   int plist_cap, plist_stride;
-  float model_par[LQ_MODEL_NUMBER_MAX][LQ_MATERIAL_SIZE_MAX];
-  HypoConst hypo[LQ_MODEL_NUMBER_MAX];
+  int n_cases, pad_c0, pad_c1, pad_c2;
+  const float (*model_par)[LQ_MATERIAL_SIZE_MAX];  // [n_model] material rows (device table)
+  const HypoConst* hypo;                           // [n_model]
+  const DevCase* cases;                            // [n_cases]
   // ---- dynamic (device-updated) ----
   int npm, np_delete_m, err, n_deferred;
   int sort_changed, pad2, pad3, pad4;
@@ -64,8 +75,8 @@ struct DevParams {
   float time, dt;
   double time_d;
   long long step;
-  DevMotion mv[LQ_RIGID_MOTION_MAX];
-  DevTable tab[LQ_RIGID_MOTION_MAX];
+  DevMotion* mv;  // [n_motion] current imposed motion of each body (device table)
+  DevTable* tab;  // [n_motion] motion tables, advanced on the device
 };
 
 // Device error codes written to DevParams::err (first error wins).
@@ -74,6 +85,7 @@ enum { DERR_NONE = 0, DERR_DELETE = 1, DERR_DOMAIN = 2 };
 struct Arrays {
   int4 *codes, *cellpos;
   int* orig_id;
+  int* case_id;  // case of the particle (batched contexts; 0 otherwise)
   float* pres;
   float4 *pos, *pos_ori, *vel, *idm, *normal;
   float4 *str_a, *stn_a;
@@ -102,7 +114,7 @@ struct Buffers {
   int *cell_beg, *cell_end;
   int* wall_near;    // per cell: a wall particle lies in its 27-cell neighbourhood
   int* hist;         // radix histogram [256][nblocks]
-  double* partial;   // reduction partials
+  unsigned long long* wf_acc;  // per case: exact wall-force sum, 128-bit fixed point (lo, hi)
 };
 
 // ------------------------------------------------------------- helpers ----
@@ -121,5 +133,15 @@ __device__ __forceinline__ bool in_grid(const DevParams* P, int cx, int cy, int 
   return cx >= 0 && cy >= 0 && cz >= 0 && cx < P->cnx && cy < P->cny && cz < P->cnz;
 }
 __device__ __forceinline__ void set_err(DevParams* P, int code) { atomicCAS(&P->err, 0, code); }
+// Cell coordinates of a particle in its own case's grid.
+__device__ __forceinline__ int3 case_cell(const DevParams* P, const int* case_id, int i, int4 c) {
+  const DevCase& k = P->cases[case_id[i]];
+  return make_int3(c.x - k.ox, c.y - k.oy, c.z - k.oz);
+}
+__device__ __forceinline__ bool in_case_grid(const DevParams* P, const int* case_id, int i, int4 c) {
+  const DevCase& k = P->cases[case_id[i]];
+  const int x = c.x - k.ox, y = c.y - k.oy, z = c.z - k.oz;
+  return x >= 0 && y >= 0 && z >= 0 && x < k.nx && y < k.ny && z < k.nz;
+}
 
 }  // namespace lq

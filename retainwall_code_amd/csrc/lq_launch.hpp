@@ -4,7 +4,7 @@
 
 namespace lq {
 
-void launch_abs2rel(hipStream_t s, const DevParams* P, int np, Arrays A, float scale);
+void launch_abs2rel(hipStream_t s, const DevParams* P, Arrays A, float scale, int first, int n);
 void launch_set_massflag(hipStream_t s, const DevParams* P, int np, Arrays A);
 void launch_copy_w(hipStream_t s, const DevParams* P, int np, const float4* src, float4* dst);
 void launch_rel2abs(hipStream_t s, const DevParams* P, int np, Arrays A, float4* out);
@@ -28,8 +28,8 @@ void launch_contact(hipStream_t s, const DevParams* P, int npm_max, Arrays A, co
                     const int* ce);
 void launch_predictor(hipStream_t s, DevParams* P, int np, Arrays A, const Buffers& B, int track_disp);
 void launch_corrector(hipStream_t s, DevParams* P, int npb, int npm_max, int n_motion, Arrays A, const Buffers& B);
-void launch_wall_force(hipStream_t s, const DevParams* P, int np, Arrays A, const float box[3], double* partial,
-                       int nblocks, double* out);
+void launch_wall_force(hipStream_t s, const DevParams* P, int np, int n_cases, Arrays A, const float* boxes,
+                       unsigned long long* acc);
 void launch_neighbor_list(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const int* cb, const int* ce,
                           int max_nb, int* counts, int* ids);
 

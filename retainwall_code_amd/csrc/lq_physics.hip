@@ -9,14 +9,18 @@
 namespace lq {
 
 // ------------------------------------------------------------ frames ----
-// cuPositionFromAbsolute2Relative, loquat_grid.cu:253-261.
-__global__ void k_abs2rel(const DevParams* __restrict__ P, Arrays A, float scale) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P->np) return;
+// cuPositionFromAbsolute2Relative, loquat_grid.cu:253-261, for slots
+// [first, first + n): the cell and the relative position are taken in the
+// particle's own case frame, then the cell is moved to the case's block.
+__global__ void k_abs2rel(const DevParams* __restrict__ P, Arrays A, float scale, int first, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int i = first + k;
   float4 p = A.pos[i];
   int cx = int(p.x * scale), cy = int(p.y * scale), cz = int(p.z * scale);
-  A.cellpos[i] = make_int4(cx, cy, cz, 0);
   A.pos[i] = make_float4(p.x - float(cx) * P->two_h, p.y - float(cy) * P->two_h, p.z - float(cz) * P->two_h, p.w);
+  const DevCase& c = P->cases[A.case_id[i]];
+  A.cellpos[i] = make_int4(cx + c.ox, cy + c.oy, cz + c.oz, 0);
 }
 
 // pos.w carries the neighbour data the interaction needs with the position:
@@ -45,7 +49,7 @@ __global__ void k_rel2abs(const DevParams* __restrict__ P, Arrays A, float4* __r
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P->np) return;
   float4 p = A.pos[i];
-  int4 c = A.cellpos[i];
+  const int3 c = case_cell(P, A.case_id, i, A.cellpos[i]);
   out[i] = make_float4(p.x + float(c.x) * P->two_h, p.y + float(c.y) * P->two_h, p.z + float(c.z) * P->two_h,
                        A.pres[i]);
 }
@@ -173,7 +177,8 @@ __global__ void __launch_bounds__(256) k_adami(const DevParams* __restrict__ P, 
       const DevMotion& m = P->mv[cd.w];
       vrb = m.trans_vel;
       if (m.is_rotat) {
-        float3 pa = make_float3(pi.x + float(ci.x) * two_h, pi.y + float(ci.y) * two_h, pi.z + float(ci.z) * two_h);
+        const int3 lc = case_cell(P, A.case_id, i, ci);
+        float3 pa = make_float3(pi.x + float(lc.x) * two_h, pi.y + float(lc.y) * two_h, pi.z + float(lc.z) * two_h);
         vrb = f3add(vrb, f3cross(m.rotat_vel, f3sub(pa, m.pos_rp)));
       }
     }
@@ -481,7 +486,9 @@ __global__ void __launch_bounds__(256) k_contact_redistribute(const DevParams* _
     const int i = B.deferred[k];
     const int4 ci = A.cellpos[i];
     const float3 pi = ld3(A.pos[i]);
-    const float3 pa = make_float3(pi.x + float(ci.x) * two_h, pi.y + float(ci.y) * two_h, pi.z + float(ci.z) * two_h);
+    const DevCase& cs = P->cases[A.case_id[i]];  // neighbours share the case
+    const float3 pa = make_float3(pi.x + float(ci.x - cs.ox) * two_h, pi.y + float(ci.y - cs.oy) * two_h,
+                                  pi.z + float(ci.z - cs.oz) * two_h);
     const float3 ni = ld3(A.cn[i]);
     float K = A.cvar[i].x;
     int best = -1, bestj = 0;  // traversal position (range << 20 | offset) and slot of the last qualifying j
@@ -496,8 +503,8 @@ __global__ void __launch_bounds__(256) k_contact_redistribute(const DevParams* _
         if (!(cvj.y <= band)) continue;
         const int4 cjj = A.cellpos[j];
         const float4 pj = A.pos[j];
-        const float3 pja = make_float3(pj.x + float(cjj.x) * two_h, pj.y + float(cjj.y) * two_h,
-                                       pj.z + float(cjj.z) * two_h);
+        const float3 pja = make_float3(pj.x + float(cjj.x - cs.ox) * two_h, pj.y + float(cjj.y - cs.oy) * two_h,
+                                       pj.z + float(cjj.z - cs.oz) * two_h);
         const float3 dc = f3sub(pa, pja);
         const float rcol = fabsf(dc.x * ni.z + dc.y * ni.y + dc.z * ni.x);
         const float Kj = cvj.x;
@@ -620,7 +627,8 @@ __global__ void __launch_bounds__(256) k_corrector_bound(DevParams* __restrict__
   int4 ci = A.cellpos[i];
   if (m.is_rotat && m.pattern != LQ_CIRCLE) {
     float4 pp = A.pos[i];
-    float3 pa = make_float3(pp.x + float(ci.x) * two_h, pp.y + float(ci.y) * two_h, pp.z + float(ci.z) * two_h);
+    const int3 lc = case_cell(P, A.case_id, i, ci);
+    float3 pa = make_float3(pp.x + float(lc.x) * two_h, pp.y + float(lc.y) * two_h, pp.z + float(lc.z) * two_h);
     vel = f3add(vel, f3cross(m.rotat_vel, f3sub(pa, m.pos_rp)));
     float3 ang = f3mul(m.rotat_vel, dt);  // GetRodriguesMatrix, functions.h:126-131
     float th = f3norm(ang);
@@ -643,7 +651,7 @@ __global__ void __launch_bounds__(256) k_corrector_bound(DevParams* __restrict__
   A.pos[i] = make_float4(p.x, p.y, p.z, pini.w);
   ci.x += dx; ci.y += dy; ci.z += dz;
   A.cellpos[i] = ci;
-  if (!in_grid(P, ci.x, ci.y, ci.z)) set_err(P, DERR_DOMAIN);
+  if (!in_case_grid(P, A.case_id, i, ci)) set_err(P, DERR_DOMAIN);
 }
 
 // cuCorrectorMaterial, predictor_corrector.cu:139-199.
@@ -675,7 +683,7 @@ __global__ void __launch_bounds__(256) k_corrector_material(const DevParams* __r
   float4 id = A.idm[i];
   id.x = 100.f;
   id.w = Mcrt;
-  if (!in_grid(P, ci.x, ci.y, ci.z)) { ci.w = P->ncell_delete_m; id.x = 999.f; }
+  if (!in_case_grid(P, A.case_id, i, ci)) { ci.w = P->ncell_delete_m; id.x = 999.f; }
   A.idm[i] = id;
   A.cellpos[i] = ci;
   A.pos[i] = make_float4(p.x, p.y, p.z, pini.w);
@@ -688,39 +696,43 @@ __global__ void __launch_bounds__(256) k_corrector_material(const DevParams* __r
 }
 
 // -------------------------------------------------------- wall force ----
-// cuGetValuesAtPoint, utilities.cu:578-640, as a two-stage FP64 reduction.
-__global__ void __launch_bounds__(256) k_wall_force_partial(const DevParams* __restrict__ P, Arrays A, float bx,
-                                                            float bz0, float bz1, double* __restrict__ partial) {
-  __shared__ double sh[256];
-  double acc = 0.0;
-  const float two_h = P->two_h;
-  for (int i = P->npb + blockIdx.x * blockDim.x + threadIdx.x; i < P->np; i += gridDim.x * blockDim.x) {
-    float4 p = A.pos[i];
-    int4 c = A.cellpos[i];
-    float xa = p.x + float(c.x) * two_h;
-    float za = p.z + float(c.z) * two_h;
-    if (xa <= bx && za >= bz0 && za <= bz1) acc += 0.001 * double(A.cf[i].x);
-  }
-  sh[threadIdx.x] = acc;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+// cuGetValuesAtPoint, utilities.cu:578-640, per case: the contact force x
+// of every soil particle inside the case's probe box is summed exactly, as
+// a 128-bit fixed-point integer in units of 2^-64 N (values below 2^-64 N
+// truncate toward zero), so the sum does not depend on the order in which
+// slots are visited; the host scales it by 0.001 once.
+__device__ __forceinline__ void acc128(unsigned long long* a, float v) {
+  const unsigned u = __float_as_uint(v);
+  const int e = int((u >> 23) & 0xffu);
+  if (e == 0 || e == 255) return;  // zero, subnormal (< 2^-126) or non-finite: contributes nothing
+  const unsigned long long m = (u & 0x7fffffu) | 0x800000u;
+  const int sh = e - 127 - 23 + 64;  // value = m * 2^(e-150) = m * 2^(sh) units of 2^-64
+  unsigned __int128 q;
+  if (sh >= 0) q = sh >= 104 ? 0 : (unsigned __int128)m << sh;  // |v| >= 2^63 N is out of range
+  else q = sh <= -24 ? 0 : (unsigned __int128)(m >> -sh);
+  if (u >> 31) q = ~q + 1;  // two's complement
+  const unsigned long long lo = (unsigned long long)q, hi = (unsigned long long)(q >> 64);
+  const unsigned long long old = atomicAdd(a, lo);
+  atomicAdd(a + 1, hi + (old + lo < old ? 1ull : 0ull));
 }
 
-__global__ void k_reduce_partials(const double* __restrict__ partial, int n, double* __restrict__ out) {
-  __shared__ double sh[256];
-  double acc = 0.0;
-  for (int k = threadIdx.x; k < n; k += blockDim.x) acc += partial[k];
-  sh[threadIdx.x] = acc;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-    __syncthreads();
+__global__ void __launch_bounds__(256) k_wall_force(const DevParams* __restrict__ P, Arrays A,
+                                                    const float* __restrict__ boxes,
+                                                    unsigned long long* __restrict__ acc) {
+  const float two_h = P->two_h;
+  for (int i = P->npb + blockIdx.x * blockDim.x + threadIdx.x; i < P->npb + P->npm; i += gridDim.x * blockDim.x) {
+    const int k = A.case_id[i];
+    const float4 p = A.pos[i];
+    const int3 c = case_cell(P, A.case_id, i, A.cellpos[i]);
+    const float xa = p.x + float(c.x) * two_h;
+    const float za = p.z + float(c.z) * two_h;
+    if (xa <= boxes[3 * k] && za >= boxes[3 * k + 1] && za <= boxes[3 * k + 2]) acc128(acc + 2 * k, A.cf[i].x);
   }
-  if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+__global__ void k_fill_u64(unsigned long long* a, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0ull;
 }
 
 // ------------------------------------------------- neighbour lists ------
@@ -761,8 +773,8 @@ __global__ void __launch_bounds__(256) k_stream_copy(const float4* __restrict__ 
 // --------------------------------------------------------- launchers ----
 static inline int blocks(long long n, int b) { return int((n + b - 1) / b); }
 
-void launch_abs2rel(hipStream_t s, const DevParams* P, int np, Arrays A, float scale) {
-  if (np > 0) hipLaunchKernelGGL(k_abs2rel, dim3(blocks(np, 256)), dim3(256), 0, s, P, A, scale);
+void launch_abs2rel(hipStream_t s, const DevParams* P, Arrays A, float scale, int first, int n) {
+  if (n > 0) hipLaunchKernelGGL(k_abs2rel, dim3(blocks(n, 256)), dim3(256), 0, s, P, A, scale, first, n);
 }
 void launch_set_massflag(hipStream_t s, const DevParams* P, int np, Arrays A) {
   if (np > 0) hipLaunchKernelGGL(k_set_massflag, dim3(blocks(np, 256)), dim3(256), 0, s, P, A);
@@ -830,10 +842,11 @@ void launch_corrector(hipStream_t s, DevParams* P, int npb, int npm_max, int n_m
   if (npm_max > 0)
     hipLaunchKernelGGL(k_corrector_material, dim3(blocks(npm_max, 256)), dim3(256), 0, s, P, A, B);
 }
-void launch_wall_force(hipStream_t s, const DevParams* P, int np, Arrays A, const float box[3], double* partial,
-                       int nblocks, double* out) {
-  hipLaunchKernelGGL(k_wall_force_partial, dim3(nblocks), dim3(256), 0, s, P, A, box[0], box[1], box[2], partial);
-  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, s, (const double*)partial, nblocks, out);
+void launch_wall_force(hipStream_t s, const DevParams* P, int np, int n_cases, Arrays A, const float* boxes,
+                       unsigned long long* acc) {
+  hipLaunchKernelGGL(k_fill_u64, dim3(blocks(2 * n_cases, 256)), dim3(256), 0, s, acc, 2 * n_cases);
+  hipLaunchKernelGGL(k_wall_force, dim3(std::max(1, std::min(blocks(np, 256), 2048))), dim3(256), 0, s, P, A, boxes,
+                     acc);
 }
 void launch_stream_copy(hipStream_t s, const float4* a, float4* b, size_t n) {
   int g = int(std::min<size_t>((n + 255) / 256, 8192));

@@ -178,7 +178,7 @@ class Context:
     def set_tile(self, shape, tz):
         self._chk(self.lib.lq_set_tile(self.h, shape, tz), "lq_set_tile")
 
-    def set_interaction(self, variant, xcd_remap=0):
+    def set_interaction(self, variant, xcd_remap=1):
         self._chk(self.lib.lq_set_interaction(self.h, variant, xcd_remap), "lq_set_interaction")
 
     def set_pair_list(self, cap=160, skin=0.01):
@@ -206,6 +206,87 @@ class Context:
         self._chk(self.lib.lq_neighbor_lists(self.h, max_nb, counts.ctypes.data, ids.ctypes.data, ids.nbytes),
                   "lq_neighbor_lists")
         return counts, ids
+
+
+class BatchContext(Context):
+    """Many independent cases stepped together (lq_create_batch): one launch
+    per kernel over all cases; case k's results equal a Context of case k
+    alone bit for bit."""
+
+    def __init__(self, params_list, device: int = 0, lib=None):
+        self.lib = lib or _abi.load_library()
+        self.case_params = list(params_list)
+        arr = (_abi.Params * len(self.case_params))(*self.case_params)
+        h = C.c_void_p()
+        st = self.lib.lq_create_batch(arr, len(self.case_params), device, C.byref(h))
+        self.h = h
+        if st:
+            msg = self.lib.lq_last_error(h).decode() if h.value else ""
+            if h.value:
+                self.lib.lq_destroy(h)
+            self.h = None
+            raise LqError(f"lq_create_batch failed ({st}): {msg}")
+        self.params = self.get_params()
+        self._keep = []
+
+    @property
+    def n_cases(self):
+        return len(self.case_params)
+
+    def upload_case(self, k, case):
+        keep = [np.ascontiguousarray(case.codes, np.int32), np.ascontiguousarray(case.pospres, np.float32),
+                np.ascontiguousarray(case.velrhop, np.float32), np.ascontiguousarray(case.idmassidx, np.float32),
+                np.ascontiguousarray(case.str, np.float32), np.ascontiguousarray(case.stn, np.float32),
+                np.ascontiguousarray(case.normal, np.float32)]
+        hs = _abi.HostState(*[a.ctypes.data for a in keep])
+        self._chk(self.lib.lq_upload_case(self.h, k, C.byref(hs)), "lq_upload_case")
+
+    def set_case_motion(self, k, tables, n=None):
+        n = len(tables) if n is None else n
+        self._chk(self.lib.lq_set_case_motion_tables(self.h, k, n, tables), "lq_set_case_motion_tables")
+
+    def get_case(self, k, name):
+        fid, dt, w, rows = _abi.FIELDS[name]
+        n = self.case_params[k].np
+        out = np.empty((n, w) if w > 1 else (n,), np.dtype(dt))
+        self._chk(self.lib.lq_get_case_field(self.h, k, fid, out.ctypes.data, out.nbytes), f"lq_get_case_field({name})")
+        return out
+
+    def case_snapshot(self, k, names):
+        return {n: self.get_case(k, n) for n in names}
+
+    def case_positions_absolute(self, k):
+        out = np.empty((self.case_params[k].np, 4), np.float32)
+        self._chk(self.lib.lq_case_positions_absolute(self.h, k, out.ctypes.data, out.nbytes),
+                  "lq_case_positions_absolute")
+        return out
+
+    def case_counts(self, k):
+        a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
+        self._chk(self.lib.lq_case_counts(self.h, k, C.byref(a), C.byref(b), C.byref(c)), "lq_case_counts")
+        return a.value, b.value, c.value
+
+    def wall_force_cases(self, boxes):
+        b = np.ascontiguousarray(np.asarray(boxes, np.float32).reshape(-1, 3))
+        F = np.zeros(len(b), np.float64)
+        self._chk(self.lib.lq_wall_force_cases(self.h, b.ctypes.data, F.ctypes.data), "lq_wall_force_cases")
+        return F
+
+
+def open_batch(case_list, key_mode=_abi.KEY_LINEAR, variant=_abi.INTERACTION_NEIGHBOR_LIST, device=0):
+    """Batched context over several cases: upload every case, its motion
+    tables, then the initial search/sort (loquat.cu:528-531)."""
+    from . import cases
+    lib = _abi.load_library()
+    plist = [cases.make_params(c, key_mode, variant, lib=lib) for c in case_list]
+    ctx = BatchContext(plist, device, lib)
+    for k, c in enumerate(case_list):
+        ctx.upload_case(k, c)
+        ctx.set_case_motion(k, cases.motion_tables(c), len(c.motion))
+    ctx.neighbor_search()
+    ctx.sort_arrays_primary()
+    ctx.sync()
+    return ctx
 
 
 def open_case(case, key_mode=_abi.KEY_LINEAR, variant=_abi.INTERACTION_NEIGHBOR_LIST, device=0, initial_sort=True):
