@@ -72,6 +72,69 @@ def cpu_baseline(case_name, seconds_target=20.0):
             "seconds": secs}
 
 
+def bench_batch(args):
+    """N independent cases (sizes log-uniform 5k-200k soil particles, heights,
+    materials and wall motions drawn per case, dx = 5 mm) stepped together in
+    one batched context; value = total soil particle-steps/s.  For comparison,
+    16 of the cases are also stepped alone (one context each) and the time of
+    running all N one after another is estimated from a least-squares fit of
+    ms/step against particle count."""
+    from retainwall_code_amd import _abi, cases, sim
+    t0 = time.perf_counter()
+    cs = cases.batch_cases(args.batch, seed=4, dr=0.005)
+    npm = sum(c.npm for c in cs)
+    npb = sum(c.npb for c in cs)
+    print(f"[bench] batch of {len(cs)} cases: {npm} soil + {npb} wall particles "
+          f"(generated in {time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    ctx = sim.open_batch(cs)
+    ctx.step(args.warmup, graph=args.graph)
+    ctx.sync()
+    variant, shape, tz, remap, _ = ctx.autotune(reps=2)
+    names = {_abi.INTERACTION_TILE: "tile", _abi.INTERACTION_REFERENCE_ORDER: "reference",
+             _abi.INTERACTION_NEIGHBOR_LIST: "neighbor_list", _abi.INTERACTION_PAIR_LIST: "pair_list"}
+    if not args.graph:
+        ctx.enable_timing(True)
+    ctx.sync()
+    t1 = time.perf_counter()
+    ctx.step(args.steps, graph=args.graph)
+    ctx.sync()
+    el = time.perf_counter() - t1
+    kt = ctx.kernel_times() if not args.graph else {}
+    F = ctx.wall_force_cases([c.probe_box for c in cs])
+    ctx.close()
+    # sequential one-case contexts on a sample
+    rng = np.random.default_rng(1)
+    sample = sorted(rng.choice(len(cs), size=min(16, len(cs)), replace=False))
+    xs, ys = [], []
+    for k in sample:
+        one = sim.open_case(cs[k])
+        one.step(2)
+        one.sync()
+        ta = time.perf_counter()
+        one.step(5, graph=args.graph)
+        one.sync()
+        ys.append(1e3 * (time.perf_counter() - ta) / 5)
+        xs.append(cs[k].npm)
+        one.close()
+    b1, b0 = np.polyfit(np.array(xs, float), np.array(ys, float), 1)
+    seq_ms = float(sum(b0 + b1 * c.npm for c in cs))
+    out = {
+        "metric": "particle-steps/sec", "value": round(npm * args.steps / el, 1), "unit": "particle-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * el / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"batch of {len(cs)} independent wall cases (5k-200k soil each, dx 5 mm, "
+                               "H U(0.1,0.6) m, phi_c U(28,36), e0 U(0.6,0.8), translation or toe rotation)",
+                   "cases": len(cs), "soil_particles": npm, "boundary_particles": npb, "graph": bool(args.graph),
+                   "interaction_variant": names[variant], "xcd_remap": remap},
+        "one_case_per_context": {"sampled_cases": len(sample), "fit_ms_per_step": [round(b0, 4), round(b1, 9)],
+                                 "sequential_ms_per_step_estimate": round(seq_ms, 2),
+                                 "batch_speedup": round(seq_ms / (1e3 * el / args.steps), 2)},
+        "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kt.items()} if kt else None,
+        "wall_force_kN_first_cases": [round(float(f), 6) for f in F[:4]],
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +147,8 @@ def main():
                     help="interaction kernel variant (all bit-identical); auto times each and keeps the fastest")
     ap.add_argument("--remap", type=int, default=0, help="XCD block renumbering for reference/neighbor_list")
     ap.add_argument("--tile", default="", metavar="SHAPE,TZ", help="fix the tile shape/depth instead of autotuning")
+    ap.add_argument("--batch", type=int, default=0, metavar="N",
+                    help="batched workload: N independent wall cases in one context (256 = the c4 workload)")
     ap.add_argument("--probe-copy", type=int, default=0, metavar="BYTES",
                     help="only run the STREAM copy probe over BYTES (HBM bandwidth / counter calibration)")
     args = ap.parse_args()
@@ -102,6 +167,9 @@ def main():
                           "ms_per_launch": ms, "GBps": 2 * args.probe_copy / (ms * 1e-3) / 1e9}), flush=True)
         ctx.close()
         return
+
+    if args.batch:
+        return bench_batch(args)
 
     case = cases.config(args.case)
     print(f"[bench] case={args.case} np={case.np} npb={case.npb} npm={case.npm}", file=sys.stderr, flush=True)

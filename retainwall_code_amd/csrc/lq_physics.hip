@@ -468,7 +468,8 @@ __global__ void __launch_bounds__(NL_WAVE * NL_WAVES) k_interaction_nl(DevParams
 
 // Pass B: same-row redistribution for deferred particles
 // (particle_interaction.cu:142-172, 209-237), reading pass-A values only.
-// One wave per deferred particle, its lanes splitting each soil range.  The
+// A group of 16 lanes per deferred particle, splitting each soil range
+// (cells hold ~18 candidates).  The
 // serial loop keeps K = min(K, Kj) over the qualifying neighbours and
 // overwrites fr = fj * K / Kj at each, so its result is the minimum K and
 // the force of the last qualifying neighbour in traversal order scaled by
@@ -480,9 +481,11 @@ __global__ void __launch_bounds__(256) k_contact_redistribute(const DevParams* _
   const int nd = P->n_deferred;
   const float two_h = P->two_h, band = P->band, half_dr = P->half_dr;
   const int ncm = P->ncell_material;
-  const int lane = threadIdx.x & 63;
-  const int nwaves = int(gridDim.x * blockDim.x) >> 6;
-  for (int k = int(blockIdx.x * blockDim.x + threadIdx.x) >> 6; k < nd; k += nwaves) {
+  constexpr int G = 16;  // lanes per deferred particle
+  const int lane = threadIdx.x & (G - 1);
+  const int ngroups = int(gridDim.x * blockDim.x) / G;
+  // k is uniform within a group and the shuffles below stay inside it
+  for (int k = int(blockIdx.x * blockDim.x + threadIdx.x) / G; k < nd; k += ngroups) {
     const int i = B.deferred[k];
     const int4 ci = A.cellpos[i];
     const float3 pi = ld3(A.pos[i]);
@@ -498,7 +501,7 @@ __global__ void __launch_bounds__(256) k_contact_redistribute(const DevParams* _
       if (!in_grid(P, jx, jy, jz)) continue;
       const int c = lin_key(P, jx, jy, jz) + ncm;
       const int b = cbeg[c], e = cend[c];
-      for (int j = b + lane; j < e; j += 64) {
+      for (int j = b + lane; j < e; j += G) {
         const float4 cvj = A.cvar[j];
         if (!(cvj.y <= band)) continue;
         const int4 cjj = A.cellpos[j];
@@ -515,7 +518,7 @@ __global__ void __launch_bounds__(256) k_contact_redistribute(const DevParams* _
         }
       }
     }
-    for (int o = 32; o > 0; o >>= 1) {
+    for (int o = G / 2; o > 0; o >>= 1) {
       const float Ko = __shfl_xor(K, o);
       if (K > Ko) K = Ko;
       const int bo = __shfl_xor(best, o), jo = __shfl_xor(bestj, o);
@@ -828,8 +831,8 @@ void launch_contact(hipStream_t s, const DevParams* P, int npm_max, Arrays A, co
                     const int* ce) {
   if (npm_max <= 0) return;
   int g = std::min(blocks(npm_max, 256), 1024);
-  // one wave per deferred particle, up to 4 * 4096 at a time
-  hipLaunchKernelGGL(k_contact_redistribute, dim3(std::min(blocks(npm_max, 4), 4096)), dim3(256), 0, s, P, A, B, cb,
+  // 16 lanes per deferred particle, up to 16 * 4096 at a time
+  hipLaunchKernelGGL(k_contact_redistribute, dim3(std::min(blocks(npm_max, 16), 4096)), dim3(256), 0, s, P, A, B, cb,
                      ce);
   hipLaunchKernelGGL(k_contact_commit, dim3(g), dim3(256), 0, s, P, A, B);
 }
