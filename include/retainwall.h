@@ -345,6 +345,54 @@ int lq_case_counts(lq_ctx* ctx, int k, int32_t* npb, int32_t* npm, int32_t* np_d
  * k's frame, force_kn[k] out. */
 int lq_wall_force_cases(lq_ctx* ctx, const float* boxes, double* force_kn);
 
+/* ---- HypoCS surrogate (no reference counterpart; replaces the cuMatModHypo
+ * call of predictor_corrector.cu:75 and :172 when enabled) -----------------
+ * MLP 14 -> 128 -> 128 -> 128 -> 7 with GELU (tanh form) on the hidden
+ * layers.  Inputs x: s = sigma - cohesion (xx yy zz xy xz yz), D*dt (6), the
+ * clamped void ratio e, dt.  Outputs y: (rate - Jaumann term)*dt (6) and
+ * M_crt = e - e_c.  The Jaumann rotation, the I1 > -4 and tensile branches
+ * and the shear cap stay exact.  Parameters (LQ_MLP_NPARAM floats): W1[14][128]
+ * b1[128] W2[128][128] b2[128] W3[128][128] b3[128] W4[128][7] b4[7];
+ * norm (LQ_MLP_NNORM floats): x_mean[14] x_std[14] y_mean[7] y_std[7]. */
+#define LQ_MLP_IN 14
+#define LQ_MLP_OUT 7
+#define LQ_MLP_HIDDEN 128
+#define LQ_MLP_NPARAM 35847
+#define LQ_MLP_NNORM 42
+
+/* Standalone (no context, current device).  lq_mlp_train runs Adam over
+ * shuffled mini-batches of normalised tuples (FP32 forward/backward
+ * kernels); lq_mlp_loss_grad returns mean((y_n(x) - y_n)^2) and its
+ * gradient; lq_mlp_forward evaluates in FP32 or with the BF16-MFMA kernel
+ * the step uses. */
+int lq_mlp_normalisation(const float* x, const float* y, size_t n, float* norm);
+int lq_mlp_train(const float* x, const float* y, size_t n, const float* norm, int epochs, int batch, float lr,
+                 uint64_t seed, int init, float* params, float* loss_per_epoch);
+int lq_mlp_loss_grad(const float* params, const float* norm, const float* x, const float* y, size_t n, float* grad,
+                     double* loss);
+int lq_mlp_forward(const float* params, const float* norm, const float* x, size_t n, float* y, int bf16);
+const char* lq_mlp_last_error(void);
+
+/* Context side.  lq_surrogate_set installs 1 or 2 ensemble members, the
+ * normalisation, the guard box [lo, hi] of the raw inputs, the ensemble
+ * tolerance (largest normalised disagreement accepted) and the audited
+ * fraction.  With the surrogate enabled every soil HypoCS call evaluates the
+ * members (BF16 MFMA); a slot outside the box or with members disagreeing
+ * takes the exact update; audit_frac of the accepted slots (hash of id, step
+ * and stage) also run the exact update and record the normalised error.
+ * lq_surrogate_stats: counts[0] surrogate, [1] range fallback, [2] ensemble
+ * fallback, [3] audited, [4] I1 > -4 branch; audit_max = largest audited
+ * error; reset != 0 clears them. */
+int lq_surrogate_set(lq_ctx* ctx, int members, const float* params, const float* norm, const float* lo,
+                     const float* hi, float ens_tol, float audit_frac);
+int lq_surrogate_enable(lq_ctx* ctx, int on);
+int lq_surrogate_stats(lq_ctx* ctx, int64_t counts[5], float* audit_max, int reset);
+/* Logging of exact (x, y) tuples while the exact update runs: every
+ * `every`-th soil slot per call (rotating), up to capacity tuples.
+ * lq_surrogate_log_read copies min(logged, max_rows) tuples out. */
+int lq_surrogate_log(lq_ctx* ctx, int every, int capacity);
+int lq_surrogate_log_read(lq_ctx* ctx, float* x, float* y, int max_rows, int* rows);
+
 #ifdef __cplusplus
 }
 #endif

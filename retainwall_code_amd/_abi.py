@@ -22,6 +22,7 @@ FIX, NONFIX = 0, 1
 CUBOID, CIRCLE = 0, 1
 KEY_LINEAR, KEY_MORTON = 0, 1
 INTERACTION_REFERENCE_ORDER, INTERACTION_TILE, INTERACTION_NEIGHBOR_LIST, INTERACTION_PAIR_LIST = 0, 1, 2, 3
+MLP_IN, MLP_OUT, MLP_HIDDEN, MLP_NPARAM, MLP_NNORM = 14, 7, 128, 35847, 42
 
 # lq_field
 (F_CODES, F_CELLPOS, F_POS, F_POS_ORI, F_VEL, F_IDM, F_POS_INI, F_VEL_INI, F_ACCAD, F_SPIN,
@@ -126,6 +127,8 @@ EXPORTS = [
     "lq_get_counts", "lq_synchronize", "lq_stress_saving_regularization", "lq_stream_copy", "lq_enable_timing", "lq_kernel_times", "lq_neighbor_lists", "lq_set_tile", "lq_set_interaction", "lq_set_pair_list", "lq_pair_list_status", "lq_autotune", "lq_interaction_cutoff_sq",
     "lq_create_batch", "lq_case_count", "lq_upload_case", "lq_set_case_motion_tables", "lq_get_case_field",
     "lq_case_positions_absolute", "lq_case_counts", "lq_wall_force_cases",
+    "lq_mlp_normalisation", "lq_mlp_train", "lq_mlp_loss_grad", "lq_mlp_forward", "lq_mlp_last_error",
+    "lq_surrogate_set", "lq_surrogate_enable", "lq_surrogate_stats", "lq_surrogate_log", "lq_surrogate_log_read",
 ]
 
 
@@ -176,6 +179,17 @@ def _proto(lib):
     lib.lq_case_positions_absolute.argtypes = [vp, C.c_int, vp, C.c_size_t]
     lib.lq_case_counts.argtypes = [vp, C.c_int, P(i32), P(i32), P(i32)]
     lib.lq_wall_force_cases.argtypes = [vp, vp, vp]
+    lib.lq_mlp_normalisation.argtypes = [vp, vp, C.c_size_t, vp]
+    lib.lq_mlp_train.argtypes = [vp, vp, C.c_size_t, vp, C.c_int, C.c_int, C.c_float, C.c_uint64, C.c_int, vp, vp]
+    lib.lq_mlp_loss_grad.argtypes = [vp, vp, vp, vp, C.c_size_t, vp, P(C.c_double)]
+    lib.lq_mlp_forward.argtypes = [vp, vp, vp, C.c_size_t, vp, C.c_int]
+    lib.lq_mlp_last_error.argtypes = []
+    lib.lq_mlp_last_error.restype = C.c_char_p
+    lib.lq_surrogate_set.argtypes = [vp, C.c_int, vp, vp, vp, vp, C.c_float, C.c_float]
+    lib.lq_surrogate_enable.argtypes = [vp, C.c_int]
+    lib.lq_surrogate_stats.argtypes = [vp, vp, P(C.c_float), C.c_int]
+    lib.lq_surrogate_log.argtypes = [vp, C.c_int, C.c_int]
+    lib.lq_surrogate_log_read.argtypes = [vp, vp, vp, C.c_int, P(C.c_int)]
 
 
 _LIB = None

@@ -58,6 +58,15 @@ struct lq_ctx {
   DevCase* d_cases = nullptr;
   float* d_boxes = nullptr;
   int n_model_rows = 0, n_motion_rows = 0;
+  // HypoCS surrogate
+  int n_cu = 1;
+  int sur_members = 0;
+  float* d_sur_norm = nullptr;  // x_mean x_std y_mean y_std
+  float* d_sur_lo = nullptr;
+  float* d_sur_hi = nullptr;
+  unsigned long long* d_sur_stats = nullptr;
+  unsigned* d_sur_amax = nullptr;
+  int log_capacity = 0;
   // reductions
   unsigned long long* wf_host = nullptr;  // pinned copy of the per-case wall-force accumulators
 };
@@ -438,7 +447,14 @@ static int create_impl(const lq_params* cases, int n_cases, int device, lq_ctx**
   st |= dalloc(c, &B.wall_near, size_t(std::max(1, par.ncell_material)));
   st |= dalloc(c, &B.hist, size_t(256) * radix_tiles(par.np) + size_t(radix_tiles(par.np)) + 1);  // + scan tile sums
   st |= dalloc(c, &B.wf_acc, size_t(2 * n_cases));
+  st |= dalloc(c, &c->d_sur_norm, size_t(LQ_MLP_NNORM));
+  st |= dalloc(c, &c->d_sur_lo, size_t(LQ_MLP_IN));
+  st |= dalloc(c, &c->d_sur_hi, size_t(LQ_MLP_IN));
+  st |= dalloc(c, &c->d_sur_stats, 8);
+  st |= dalloc(c, &c->d_sur_amax, 1);
   if (st) return bail(LQ_ERR_HIP);
+  hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device);
+  c->n_cu = std::max(1, c->n_cu);
   B.part_cel = B.keys;
   B.part_idx = B.vals;
   if (hipHostMalloc((void**)&c->wf_host, sizeof(unsigned long long) * 2 * n_cases) != hipSuccess)
@@ -459,6 +475,11 @@ static int create_impl(const lq_params* cases, int n_cases, int device, lq_ctx**
   HIPCHECK(c, hipMemcpyAsync(c->d_mv, mvrows.data(), mvrows.size() * sizeof(DevMotion), hipMemcpyHostToDevice, c->stream));
   HIPCHECK(c, hipMemcpyAsync(c->d_cases, c->cases.data(), c->cases.size() * sizeof(DevCase), hipMemcpyHostToDevice,
                              c->stream));
+  c->hp.sur_norm = c->d_sur_norm;
+  c->hp.sur_lo = c->d_sur_lo;
+  c->hp.sur_hi = c->d_sur_hi;
+  c->hp.sur_stats = c->d_sur_stats;
+  c->hp.sur_audit_max = c->d_sur_amax;
   c->hp.model_par = c->d_model;
   c->hp.hypo = c->d_hypo;
   c->hp.cases = c->d_cases;
@@ -1024,15 +1045,23 @@ static int enq_interaction(lq_ctx* c, int stage) {
   }
   return 0;
 }
+// The HypoCS update of soil slots after the predictor (stage 0) or corrector
+// (stage 1) kinematics: the surrogate when enabled, and tuple logging.
+static void enq_constitutive(lq_ctx* c, int stage) {
+  if (c->hp.sur_on) launch_surrogate(c->stream, c->dP, c->npm0, c->sur_members, c->n_cu, c->B.cur, c->B, stage);
+  if (c->hp.log_every > 0) launch_sur_log(c->stream, c->dP, c->npm0, c->B.cur, c->B, stage);
+}
 static int enq_predictor(lq_ctx* c) {
   Section s(c, LQ_K_PREDICTOR);
   launch_predictor(c->stream, c->dP, c->par.np, c->B.cur, c->B,
                    c->par.interaction_variant == LQ_INTERACTION_PAIR_LIST ? 1 : 0);
+  enq_constitutive(c, 0);
   return 0;
 }
 static int enq_corrector(lq_ctx* c) {
   Section s(c, LQ_K_CORRECTOR);
   launch_corrector(c->stream, c->dP, c->par.npb, c->npm0, c->par.n_motion, c->B.cur, c->B);
+  enq_constitutive(c, 1);
   return 0;
 }
 
@@ -1250,6 +1279,123 @@ extern "C" int lq_set_tile(lq_ctx* c, int shape, int tz) {
   c->tile_shape = shape;
   c->tile_tz = tz;
   c->layout_version++;  // re-capture graphs
+  return LQ_OK;
+}
+
+// ------------------------------------------------------------ surrogate --
+// Copies the surrogate block of the host DevParams image to the device.
+static int push_sur(lq_ctx* c) {
+  const size_t off = offsetof(DevParams, sur_on);
+  const size_t end = offsetof(DevParams, log_count) + sizeof(unsigned*);
+  HIPCHECK(c, hipMemcpyAsync(reinterpret_cast<char*>(c->dP) + off, reinterpret_cast<const char*>(&c->hp) + off,
+                             end - off, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  c->layout_version++;  // re-capture graphs
+  return LQ_OK;
+}
+
+extern "C" int lq_surrogate_set(lq_ctx* c, int members, const float* params, const float* norm, const float* lo,
+                                const float* hi, float ens_tol, float audit_frac) {
+  if (!c) return LQ_ERR_ARG;
+  if (members < 1 || members > 2 || !params || !norm || !lo || !hi || !(ens_tol >= 0.f) ||
+      !(audit_frac >= 0.f && audit_frac <= 1.f))
+    return fail(c, LQ_ERR_ARG, "lq_surrogate_set: members 1..2, params/norm/lo/hi, ens_tol >= 0, audit in [0,1]");
+  for (int k = 0; k < LQ_MLP_NNORM; k++)
+    if (!std::isfinite(norm[k])) return fail(c, LQ_ERR_ARG, "lq_surrogate_set: non-finite normalisation");
+  for (int k = LQ_MLP_IN; k < 2 * LQ_MLP_IN; k++)
+    if (!(norm[k] > 0.f)) return fail(c, LQ_ERR_ARG, "lq_surrogate_set: input std must be > 0");
+  hipSetDevice(c->device);
+  Buffers& B = c->B;
+  const size_t nm = size_t(std::max(1, c->npm0));
+  if (!B.sur_x) {
+    int st = 0;
+    st |= dalloc(c, &B.sur_x, nm * 16);
+    st |= dalloc(c, &B.sur_y, 2 * nm * 8);
+    st |= dalloc(c, &B.sur_w, size_t(2) * sur_wpack());
+    st |= dalloc(c, &B.sur_b, size_t(2) * sur_bpack());
+    if (st) return LQ_ERR_HIP;
+  }
+  std::vector<unsigned short> w(size_t(members) * sur_wpack());
+  std::vector<float> b(size_t(members) * sur_bpack());
+  for (int m = 0; m < members; m++)
+    sur_pack(params + size_t(m) * LQ_MLP_NPARAM, w.data() + size_t(m) * sur_wpack(), b.data() + size_t(m) * sur_bpack());
+  HIPCHECK(c, hipMemcpyAsync(B.sur_w, w.data(), w.size() * 2, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(B.sur_b, b.data(), b.size() * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(c->d_sur_norm, norm, LQ_MLP_NNORM * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(c->d_sur_lo, lo, LQ_MLP_IN * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(c->d_sur_hi, hi, LQ_MLP_IN * 4, hipMemcpyHostToDevice, c->stream));
+  c->sur_members = members;
+  c->hp.sur_members = members;
+  c->hp.sur_ens_tol = ens_tol;
+  c->hp.sur_audit_frac = audit_frac;
+  return push_sur(c);
+}
+
+extern "C" int lq_surrogate_enable(lq_ctx* c, int on) {
+  if (!c) return LQ_ERR_ARG;
+  if (on && c->sur_members == 0) return fail(c, LQ_ERR_STATE, "lq_surrogate_enable: call lq_surrogate_set first");
+  hipSetDevice(c->device);
+  c->hp.sur_on = on ? 1 : 0;
+  return push_sur(c);
+}
+
+extern "C" int lq_surrogate_stats(lq_ctx* c, int64_t counts[5], float* audit_max, int reset) {
+  if (!c) return LQ_ERR_ARG;
+  hipSetDevice(c->device);
+  unsigned long long st[8];
+  unsigned am = 0;
+  HIPCHECK(c, hipMemcpyAsync(st, c->d_sur_stats, sizeof st, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(&am, c->d_sur_amax, sizeof am, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  if (counts)
+    for (int k = 0; k < 5; k++) counts[k] = int64_t(st[k]);
+  if (audit_max) std::memcpy(audit_max, &am, 4);
+  if (reset) {
+    HIPCHECK(c, hipMemsetAsync(c->d_sur_stats, 0, sizeof st, c->stream));
+    HIPCHECK(c, hipMemsetAsync(c->d_sur_amax, 0, sizeof am, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+  }
+  return LQ_OK;
+}
+
+extern "C" int lq_surrogate_log(lq_ctx* c, int every, int capacity) {
+  if (!c || every < 0 || capacity < 0) return fail(c, LQ_ERR_ARG, "lq_surrogate_log: every, capacity >= 0");
+  hipSetDevice(c->device);
+  if (every > 0 && capacity > c->log_capacity) {
+    float *lx = nullptr, *ly = nullptr;
+    unsigned* lc = nullptr;
+    int st = dalloc(c, &lx, size_t(capacity) * 16) | dalloc(c, &ly, size_t(capacity) * 8) | dalloc(c, &lc, 1);
+    if (st) return LQ_ERR_HIP;
+    c->hp.log_x = lx;
+    c->hp.log_y = ly;
+    c->hp.log_count = lc;
+    c->log_capacity = capacity;
+  }
+  if (c->hp.log_count) HIPCHECK(c, hipMemsetAsync(c->hp.log_count, 0, sizeof(unsigned), c->stream));
+  c->hp.log_every = every;
+  c->hp.log_cap = every > 0 ? capacity : 0;
+  return push_sur(c);
+}
+
+extern "C" int lq_surrogate_log_read(lq_ctx* c, float* x, float* y, int max_rows, int* rows) {
+  if (!c || !rows || max_rows < 0 || (max_rows > 0 && (!x || !y))) return fail(c, LQ_ERR_ARG, "lq_surrogate_log_read");
+  hipSetDevice(c->device);
+  unsigned cnt = 0;
+  if (c->hp.log_count)
+    HIPCHECK(c, hipMemcpyAsync(&cnt, c->hp.log_count, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  const int n = std::min<int>(int(std::min<unsigned>(cnt, unsigned(c->hp.log_cap))), max_rows);
+  std::vector<float> tx(size_t(std::max(n, 1)) * 16), ty(size_t(std::max(n, 1)) * 8);
+  if (n > 0) {
+    HIPCHECK(c, hipMemcpyAsync(tx.data(), c->hp.log_x, size_t(n) * 64, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(ty.data(), c->hp.log_y, size_t(n) * 32, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+  }
+  for (int r = 0; r < n; r++) {
+    std::memcpy(x + size_t(r) * LQ_MLP_IN, &tx[size_t(r) * 16], LQ_MLP_IN * 4);
+    std::memcpy(y + size_t(r) * LQ_MLP_OUT, &ty[size_t(r) * 8], LQ_MLP_OUT * 4);
+  }
+  *rows = n;
   return LQ_OK;
 }
 

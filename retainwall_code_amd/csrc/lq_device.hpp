@@ -248,4 +248,92 @@ __device__ __forceinline__ void hypo_cs(const DevParams* __restrict__ P, int mod
   sig = Sym{nd6.xx + sph + coh, nd6.yy + sph + coh, nd6.zz + sph + coh, nd6.xy, nd6.xz, nd6.yz};
 }
 
+// ------------------------------------------------- HypoCS surrogate ----
+// The surrogate learns the costly middle of hypo_cs: from
+//   x = (s = sigma - coh (xx yy zz xy xz yz), D dt (6), e (clamped), dt)
+// it predicts y = ((rate - Jaumann term) dt (6), M_crt).  The Jaumann term,
+// the I1 > -4 and tensile branches and the shear cap stay exact
+// (hypo_cs_apply).  hypo_cs_parts computes x and, when want_y, the exact y
+// with the operations of hypo_cs; it returns false when I1 > -4 (hypo_cs
+// then returns a fixed tensor and no surrogate is involved).
+constexpr int SUR_IN = 14, SUR_OUT = 7, SUR_XS = 16, SUR_YS = 8;
+
+__device__ __forceinline__ Sym jaumann(float3 w, const Sym& s) {
+  const float Sxx = s.xx, Syy = s.yy, Szz = s.zz, Sxy = s.xy, Sxz = s.xz, Syz = s.yz;
+  float WSxx = -w.z * Sxy + w.y * Sxz, WSxy = -w.z * Syy + w.y * Syz, WSxz = -w.z * Syz + w.y * Szz;
+  float WSyx = w.z * Sxx - w.x * Sxz, WSyy = w.z * Sxy - w.x * Syz, WSyz = w.z * Sxz - w.x * Szz;
+  float WSzx = -w.y * Sxx + w.x * Sxy, WSzy = -w.y * Sxy + w.x * Syy, WSzz = -w.y * Sxz + w.x * Syz;
+  float SWxx = Sxy * w.z - Sxz * w.y, SWxy = -Sxx * w.z + Sxz * w.x, SWxz = Sxx * w.y - Sxy * w.x;
+  float SWyx = Syy * w.z - Syz * w.y, SWyy = -Sxy * w.z + Syz * w.x, SWyz = Sxy * w.y - Syy * w.x;
+  float SWzx = Syz * w.z - Szz * w.y, SWzy = -Sxz * w.z + Szz * w.x, SWzz = Sxz * w.y - Syz * w.x;
+  return Sym{WSxx - SWxx, WSyy - SWyy, WSzz - SWzz, 0.5f * ((WSxy - SWxy) + (WSyx - SWyx)),
+             0.5f * ((WSxz - SWxz) + (WSzx - SWzx)), 0.5f * ((WSyz - SWyz) + (WSzy - SWzy))};
+}
+
+__device__ __forceinline__ bool hypo_cs_parts(const DevParams* __restrict__ P, int model, float dt, float rho,
+                                              Sym D, const Sym& sig, float* x, float* y, bool want_y) {
+  const float* mp = P->model_par[model];
+  const HypoConst& hc = P->hypo[model];
+  const float coh = mp[3], pref = mp[4], ec0 = mp[5], lam = mp[6], xi = mp[7], alpha = mp[8];
+  const float emin = mp[9], emax = mp[10], e0 = mp[11], rho0 = mp[12];
+  Sym s = sig;
+  if (coh > 0.f) { s.xx = s.xx - coh; s.yy = s.yy - coh; s.zz = s.zz - coh; }
+  const float I1 = strace(s);
+  if (I1 > -4.0f) return false;
+  float e = (1.0f + e0) * rho0 / rho - 1.0f;
+  e = fmaxf(e, emin);
+  e = fminf(e, emax);
+  x[0] = s.xx; x[1] = s.yy; x[2] = s.zz; x[3] = s.xy; x[4] = s.xz; x[5] = s.yz;
+  x[6] = D.xx * dt; x[7] = D.yy * dt; x[8] = D.zz * dt; x[9] = D.xy * dt; x[10] = D.xz * dt; x[11] = D.yz * dt;
+  x[12] = e; x[13] = dt;
+  if (!want_y) return true;
+  Sym dev = s;
+  const float third = I1 / 3.0f;
+  dev.xx = dev.xx - third; dev.yy = dev.yy - third; dev.zz = dev.zz - third;
+  const float trs = strace(s), trd = strace(D), trsd = sddot(s, D), nd = sqrtf(sddot(D, D));
+  const float p = -I1 / 3.0f;
+  const float ec = ec0 * expf(-lam * powf(p / pref, xi));
+  const float fd = powf((e - emin) / (ec - emin), alpha);
+  const Sym sd = sadd(s, dev);
+  const Sym r = sadd(sadd(sadd(smul(D, hc.C1 * trs), smul(s, hc.C2 * trd)), smul(s, hc.C3 * trsd / trs)),
+                     smul(sd, fd * hc.C4 * nd));
+  y[0] = r.xx * dt; y[1] = r.yy * dt; y[2] = r.zz * dt; y[3] = r.xy * dt; y[4] = r.xz * dt; y[5] = r.yz * dt;
+  y[6] = e - ec;
+  return true;
+}
+
+// Exact tail of hypo_cs (constitutive_model.h:266-299) applied to a
+// predicted increment dsh = (rate - Jaumann) dt and M_crt = e - e_c.
+__device__ __forceinline__ void hypo_cs_apply(const DevParams* __restrict__ P, int model, float dt, float3 w,
+                                              const float* yv, float e, Sym& sig) {
+  const float* mp = P->model_par[model];
+  const HypoConst& hc = P->hypo[model];
+  const float coh = mp[3], alpha = mp[8], emin = mp[9];
+  Sym s = sig;
+  if (coh > 0.f) { s.xx = s.xx - coh; s.yy = s.yy - coh; s.zz = s.zz - coh; }
+  const Sym jau = jaumann(w, s);
+  const Sym dsh = {yv[0], yv[1], yv[2], yv[3], yv[4], yv[5]};
+  const Sym ns = sadd(sig, sadd(dsh, smul(jau, dt)));
+  const float sph = strace(ns) / 3.0f;
+  if (sph > 0.f) { sig = Sym{-1.0f + coh, -1.0f + coh, -2.0f + coh, 0.f, 0.f, 0.f}; return; }
+  Sym nd6 = ns;
+  nd6.xx = nd6.xx - sph; nd6.yy = nd6.yy - sph; nd6.zz = nd6.zz - sph;
+  const float ec = e - yv[6];
+  const float fd = powf(fmaxf((e - emin) / (ec - emin), 0.f), alpha);
+  const float fa = hc.fa, fb = hc.fb, fc = hc.fc;
+  const float u = 1.0f + fa + fb / 3.0f;
+  const float f2 = fd * fd * fc * fc;
+  const float A_ = f2 / 3.0f - u * u;
+  const float B_ = 2.0f * f2 * (u * u + 2.0f * fa + 3.0f) - 4.0f * fb * u;
+  const float C_ = 4.0f / 3.0f * f2 * fb * fb - 4.0f * fb * fb;
+  const float k2 = (-B_ - sqrtf(B_ * B_ - 4.0f * A_ * C_)) / (2.0f * A_);
+  const float kphi = 3.0f / sqrtf(k2);
+  const float tau = sqrtf(0.5f * sddot(nd6, nd6));
+  if (tau - (-sph * kphi) > 0.f) {
+    const float ratio = tau < 1e-10f ? 1.0f : (-kphi * sph) / tau;
+    nd6 = smul(nd6, ratio);
+  }
+  sig = Sym{nd6.xx + sph + coh, nd6.yy + sph + coh, nd6.zz + sph + coh, nd6.xy, nd6.xz, nd6.yz};
+}
+
 }  // namespace lq

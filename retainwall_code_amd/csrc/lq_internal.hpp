@@ -72,6 +72,17 @@ struct DevParams {
   unsigned max_disp2_bits;  // largest squared soil displacement since the pair-list build (float bits)
   int plist_bad;            // a soil slot overflowed its pair-list capacity
   int pad5, pad6;
+  // HypoCS surrogate (MLP) state
+  int sur_on, sur_members, log_every, log_cap;
+  float sur_ens_tol, sur_audit_frac, pad7, pad8;
+  const float* sur_norm;        // x_mean[14] x_std[14] y_mean[7] y_std[7]
+  const float* sur_lo;          // [14] guard box of the raw inputs
+  const float* sur_hi;          // [14]
+  unsigned long long* sur_stats;  // [8]: surrogate, fallback range, fallback ensemble, audited, tensile, (spare)
+  unsigned* sur_audit_max;      // [1] largest audited normalised error (float bits)
+  float* log_x;                 // [log_cap][16] logged raw inputs
+  float* log_y;                 // [log_cap][8]  logged exact targets
+  unsigned* log_count;
   float time, dt;
   double time_d;
   long long step;
@@ -107,6 +118,10 @@ struct Buffers {
   float2* str_tmp_b;
   float4* sr_a;      // per-slot sigma/rho^2 (xx yy zz xy), written by k_pair_prep
   float4* sr_b;      // per-slot sigma/rho^2 (xz yz), m/rho, trace
+  float* sur_x;      // [npm][16] surrogate inputs (normalised), per soil slot
+  float* sur_y;      // [members][npm][8] surrogate outputs (normalised)
+  unsigned short* sur_w;  // packed bf16 weights per member (MFMA operand layout)
+  float* sur_b;      // biases per member
   int* plist;        // pair lists, entry k of soil slot s at plist[k * stride + s]
   int* pcount;       // pair-list length per soil slot
   // grid / sort

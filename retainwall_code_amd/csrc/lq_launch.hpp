@@ -41,6 +41,23 @@ void launch_interaction_tile(hipStream_t s, DevParams* P, int cnx, int cny, int 
                              const int* cb, const int* ce, int shape, int tz);
 void launch_fill_int(hipStream_t s, int* a, int n, int v);
 int radix_tiles(int n);
+// HypoCS surrogate (lq_surrogate.hip)
+void launch_surrogate(hipStream_t s, DevParams* P, int npm_max, int members, int n_cu, Arrays A, const Buffers& B,
+                      int stage);
+void launch_sur_log(hipStream_t s, DevParams* P, int npm_max, Arrays A, const Buffers& B, int stage);
+int sur_wpack();
+int sur_bpack();
+void sur_pack(const float* params, unsigned short* w, float* b);
+void launch_mlp_bf16(hipStream_t s, const float* X, int n, const unsigned short* W, const float* b, float* Y,
+                     int n_cu);
+void launch_sgemm(hipStream_t s, int M, int N, int K, const float* A, int lda, int ta, const float* B, int ldb,
+                  int tb, float* C, int ldc, int accumulate);
+void launch_bias_act(hipStream_t s, float* Z, const float* b, float* A, int M, int N, int act);
+void launch_out_grad(hipStream_t s, const float* Y, const float* T, float* dY, int M, int N, double* loss);
+void launch_gelu_back(hipStream_t s, float* dA, const float* Z, long long n);
+void launch_colsum(hipStream_t s, const float* dZ, int M, int N, float* db);
+void launch_adam(hipStream_t s, float* p, const float* g, float* m, float* v, int n, float lr, float c1, float c2);
+void launch_gather_rows(hipStream_t s, const float* src, int ld, const int* idx, int rows, int cols, float* dst);
 void launch_neighbor_search(hipStream_t s, DevParams* P, int np, int nbits, int ncell, Arrays A, Buffers& B,
                             const int** sorted_keys, const int** sorted_vals);
 void launch_gather(hipStream_t s, const DevParams* P, int np, Arrays src, Arrays dst, const int* idx, const Buffers& B);

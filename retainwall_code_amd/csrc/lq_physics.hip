@@ -588,7 +588,7 @@ __device__ __forceinline__ float predictor_slot(const DevParams* __restrict__ P,
   stsym(A.stn_a, A.stn_b, i, sadd(stn, smul(D, hdt)));
   Sym sig = {sa.x, sa.y, sa.z, sa.w, sb.x, sb.y};
   float Mcrt = 0.f;
-  if (cd.z == LQ_HYPO_CS) hypo_cs(P, cd.w, hdt, rho, ld3(B.spin[i]), D, sig, Mcrt);
+  if (cd.z == LQ_HYPO_CS && !P->sur_on) hypo_cs(P, cd.w, hdt, rho, ld3(B.spin[i]), D, sig, Mcrt);
   stsym(A.str_a, A.str_b, i, sig);
   A.pos[i] = p;
   A.vel[i] = v;
@@ -681,7 +681,7 @@ __global__ void __launch_bounds__(256) k_corrector_material(const DevParams* __r
   stsym(A.stn_a, A.stn_b, i, sadd(stn0, smul(D, dt)));
   Sym sig = ldsym(B.str_ini_a, B.str_ini_b, i);
   float Mcrt = 0.f;
-  if (cd.z == LQ_HYPO_CS) hypo_cs(P, cd.w, dt, rho, ld3(B.spin[i]), D, sig, Mcrt);
+  if (cd.z == LQ_HYPO_CS && !P->sur_on) hypo_cs(P, cd.w, dt, rho, ld3(B.spin[i]), D, sig, Mcrt);
   stsym(A.str_a, A.str_b, i, sig);
   float4 id = A.idm[i];
   id.x = 100.f;
