@@ -135,6 +135,69 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_surrogate(args):
+    """Gain test of the HypoCS surrogate: two contexts of the case advance
+    identically through warm-up while one logs exact (x, y) tuples; two
+    members are trained on them; then both contexts run the same steps, one
+    exact and one with the surrogate (guard + 1 % audit), each timed, with the
+    wall force every 10 steps.  Wall-force error = max_t |F_s - F_p| /
+    max_t |F_p|."""
+    from retainwall_code_amd import cases, sim, surrogate
+    case = cases.config(args.case)
+    plain = sim.open_case(case)
+    sur = sim.open_case(case)
+    surrogate.start_log(sur, every=16, capacity=2_000_000)
+    plain.step(args.warmup)
+    sur.step(args.warmup)
+    sur.sync()
+    x, y = surrogate.read_log(sur, max_rows=2_000_000)
+    surrogate.start_log(sur, every=0, capacity=0)
+    t0 = time.perf_counter()
+    norm = surrogate.normalisation(x, y)
+    members, losses = [], []
+    for seed in (1, 2):
+        p, hist = surrogate.train(x, y, norm, epochs=args.sur_epochs, batch=4096, lr=1e-3, seed=seed)
+        members.append(p)
+        losses.append([round(float(v), 6) for v in hist])
+    t_train = time.perf_counter() - t0
+    lo, hi = surrogate.guard_box(x)
+    surrogate.install(sur, members, norm, lo, hi, ens_tol=args.ens_tol, audit_frac=0.01)
+    surrogate.enable(sur)
+    surrogate.stats(sur, reset=True)
+    res = {}
+    for name, ctx in (("exact", plain), ("surrogate", sur)):
+        forces, el = [], 0.0
+        for _ in range(max(1, args.steps // 10)):
+            ctx.sync()
+            ta = time.perf_counter()
+            ctx.step(10, graph=True)
+            ctx.sync()
+            el += time.perf_counter() - ta
+            forces.append(ctx.wall_force(case.probe_box))
+        res[name] = (np.array(forces), el, 10 * len(forces))
+    fp, fs = res["exact"][0], res["surrogate"][0]
+    werr = float(np.max(np.abs(fs - fp)) / max(np.max(np.abs(fp)), 1e-30))
+    st = surrogate.stats(sur)
+    calls = st["surrogate"] + st["fallback_range"] + st["fallback_ensemble"]
+    v_exact = case.npm * res["exact"][2] / res["exact"][1]
+    v_sur = case.npm * res["surrogate"][2] / res["surrogate"][1]
+    out = {
+        "metric": "particle-steps/sec", "value": round(v_sur, 1), "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": res["surrogate"][2], "warmup": args.warmup, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 (surrogate bf16 MFMA)", "data": "synthetic",
+        "config": {"workload": f"{args.case} with the HypoCS surrogate (2 members, guard, 1% audit)",
+                   "soil_particles": case.npm, "graph": True},
+        "exact_value": round(v_exact, 1), "gain": round(v_sur / v_exact, 3),
+        "wall_force_error": werr, "wall_force_kN": {"exact": [round(float(v), 6) for v in fp],
+                                                    "surrogate": [round(float(v), 6) for v in fs]},
+        "accepted_fraction": round(st["surrogate"] / max(calls, 1), 4), "stats": st,
+        "training": {"tuples": int(len(x)), "seconds": round(t_train, 2), "loss_per_epoch": losses},
+    }
+    print(json.dumps(out), flush=True)
+    plain.close()
+    sur.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -149,6 +212,10 @@ def main():
     ap.add_argument("--tile", default="", metavar="SHAPE,TZ", help="fix the tile shape/depth instead of autotuning")
     ap.add_argument("--batch", type=int, default=0, metavar="N",
                     help="batched workload: N independent wall cases in one context (256 = the c4 workload)")
+    ap.add_argument("--surrogate", action="store_true",
+                    help="gain test of the HypoCS surrogate on the case: log, train, then time both forms")
+    ap.add_argument("--ens-tol", type=float, default=0.1, help="surrogate ensemble tolerance (normalised units)")
+    ap.add_argument("--sur-epochs", type=int, default=16, help="surrogate training epochs per member")
     ap.add_argument("--probe-copy", type=int, default=0, metavar="BYTES",
                     help="only run the STREAM copy probe over BYTES (HBM bandwidth / counter calibration)")
     args = ap.parse_args()
@@ -170,6 +237,8 @@ def main():
 
     if args.batch:
         return bench_batch(args)
+    if args.surrogate:
+        return bench_surrogate(args)
 
     case = cases.config(args.case)
     print(f"[bench] case={args.case} np={case.np} npb={case.npb} npm={case.npm}", file=sys.stderr, flush=True)

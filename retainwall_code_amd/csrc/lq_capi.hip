@@ -450,8 +450,8 @@ static int create_impl(const lq_params* cases, int n_cases, int device, lq_ctx**
   st |= dalloc(c, &c->d_sur_norm, size_t(LQ_MLP_NNORM));
   st |= dalloc(c, &c->d_sur_lo, size_t(LQ_MLP_IN));
   st |= dalloc(c, &c->d_sur_hi, size_t(LQ_MLP_IN));
-  st |= dalloc(c, &c->d_sur_stats, 8);
-  st |= dalloc(c, &c->d_sur_amax, 1);
+  st |= dalloc(c, &c->d_sur_stats, size_t(8) * sur_slots());
+  st |= dalloc(c, &c->d_sur_amax, size_t(16) * sur_slots());
   if (st) return bail(LQ_ERR_HIP);
   hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device);
   c->n_cu = std::max(1, c->n_cu);
@@ -1342,17 +1342,23 @@ extern "C" int lq_surrogate_enable(lq_ctx* c, int on) {
 extern "C" int lq_surrogate_stats(lq_ctx* c, int64_t counts[5], float* audit_max, int reset) {
   if (!c) return LQ_ERR_ARG;
   hipSetDevice(c->device);
-  unsigned long long st[8];
-  unsigned am = 0;
-  HIPCHECK(c, hipMemcpyAsync(st, c->d_sur_stats, sizeof st, hipMemcpyDeviceToHost, c->stream));
-  HIPCHECK(c, hipMemcpyAsync(&am, c->d_sur_amax, sizeof am, hipMemcpyDeviceToHost, c->stream));
+  const int ns = sur_slots();
+  std::vector<unsigned long long> st(size_t(8) * ns);
+  std::vector<unsigned> am(size_t(16) * ns);
+  HIPCHECK(c, hipMemcpyAsync(st.data(), c->d_sur_stats, st.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(am.data(), c->d_sur_amax, am.size() * 4, hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   if (counts)
-    for (int k = 0; k < 5; k++) counts[k] = int64_t(st[k]);
-  if (audit_max) std::memcpy(audit_max, &am, 4);
+    for (int k = 0; k < 5; k++) {
+      counts[k] = 0;
+      for (int q = 0; q < ns; q++) counts[k] += int64_t(st[size_t(8) * q + k]);
+    }
+  unsigned mx = 0;
+  for (int q = 0; q < ns; q++) mx = std::max(mx, am[size_t(16) * q]);  // non-negative floats order as bits
+  if (audit_max) std::memcpy(audit_max, &mx, 4);
   if (reset) {
-    HIPCHECK(c, hipMemsetAsync(c->d_sur_stats, 0, sizeof st, c->stream));
-    HIPCHECK(c, hipMemsetAsync(c->d_sur_amax, 0, sizeof am, c->stream));
+    HIPCHECK(c, hipMemsetAsync(c->d_sur_stats, 0, st.size() * 8, c->stream));
+    HIPCHECK(c, hipMemsetAsync(c->d_sur_amax, 0, am.size() * 4, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
   }
   return LQ_OK;

@@ -78,8 +78,8 @@ struct DevParams {
   const float* sur_norm;        // x_mean[14] x_std[14] y_mean[7] y_std[7]
   const float* sur_lo;          // [14] guard box of the raw inputs
   const float* sur_hi;          // [14]
-  unsigned long long* sur_stats;  // [8]: surrogate, fallback range, fallback ensemble, audited, tensile, (spare)
-  unsigned* sur_audit_max;      // [1] largest audited normalised error (float bits)
+  unsigned long long* sur_stats;  // [slots][8]: surrogate, fallback range, fallback ensemble, audited, I1 branch
+  unsigned* sur_audit_max;      // [slots][16]: largest audited normalised error (float bits) in [q][0]
   float* log_x;                 // [log_cap][16] logged raw inputs
   float* log_y;                 // [log_cap][8]  logged exact targets
   unsigned* log_count;

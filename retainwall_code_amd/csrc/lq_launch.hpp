@@ -46,6 +46,7 @@ void launch_surrogate(hipStream_t s, DevParams* P, int npm_max, int members, int
                       int stage);
 void launch_sur_log(hipStream_t s, DevParams* P, int npm_max, Arrays A, const Buffers& B, int stage);
 int sur_wpack();
+int sur_slots();
 int sur_bpack();
 void sur_pack(const float* params, unsigned short* w, float* b);
 void launch_mlp_bf16(hipStream_t s, const float* X, int n, const unsigned short* W, const float* b, float* Y,

@@ -34,6 +34,7 @@ constexpr int W1T = H * K1P, W2T = H * KP, W4T = 16 * KP;
 constexpr int WPACK = W1T + 2 * W2T + W4T;  // bf16 per member
 constexpr int BPACK = 3 * H + 16;           // biases per member (FP32, b4 padded)
 constexpr int MLP_WAVES = 8;
+constexpr int SUR_SLOTS = 64;  // statistics counter lines (8 u64 each; audit max every 16 u32)
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
@@ -50,26 +51,35 @@ __device__ __forceinline__ float gelu_grad(float x) {
   const float du = 0.7978845608f * fmaf(0.134145f, x2, 1.0f);
   return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * du;
 }
-__device__ __forceinline__ unsigned short f2bf(float f) {  // round to nearest even
-  unsigned u = __float_as_uint(f);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (unsigned short)(u >> 16);
+// The same function as gelu(): 0.5 x (1 + tanh(u)) = x / (1 + exp(-2u)),
+// evaluated with the hardware exp2 and reciprocal (about 1 ulp each; the
+// result is rounded to bf16 next).
+__device__ __forceinline__ float gelu_fast(float x) {
+  const float u = 0.7978845608f * fmaf(0.044715f * x, x * x, x);
+  return x * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-2.885390082f * u));  // exp(-2u)
+}
+__device__ __forceinline__ unsigned short bf16_bits(float f) {  // v_cvt_pk_bf16_f32, round to nearest even
+  return __builtin_bit_cast(unsigned short, static_cast<__bf16>(f));
 }
 
 // ------------------------------------------------------ forward (MFMA) ----
-// One 512-thread block per CU keeps a member's weights in LDS (84 KB) and
-// its 8 waves walk 16-row tiles of the soil slots.  v_mfma_f32_16x16x32_bf16:
-// A (16 rows x 32 k) and B (32 k x 16 cols) take 8 consecutive k per lane,
-// row / column = lane % 16, k = 32 kb + 8 (lane / 16) + i; D row =
-// 4 (lane / 16) + i, column = lane % 16.  Padded row strides make the
-// 16-byte LDS fragment reads conflict-free.  Every wave runs the same number
-// of tile iterations so the block barriers stay uniform.
+// One 512-thread block per CU keeps a member's weights in LDS (84 KB); each
+// of its 8 waves walks pairs of 16-row tiles, so every weight fragment read
+// from LDS feeds two MFMAs.  v_mfma_f32_16x16x32_bf16: A (16 rows x 32 k)
+// and B (32 k x 16 cols) take 8 consecutive k per lane, row / column =
+// lane % 16, k = 32 kb + 8 (lane / 16) + i; D row = 4 (lane / 16) + i,
+// column = lane % 16.  A layer reads all its input fragments into registers,
+// then overwrites the same per-wave activation rows with its outputs.
+// Padded row strides make the 16-byte fragment reads conflict-free.  Every
+// wave runs the same number of iterations, so the block barriers (which
+// order a wave's own LDS reads before its overwrites) stay uniform.
+constexpr int MLP_TILES = 2;  // 16-row tiles per wave and iteration
 __global__ void __launch_bounds__(64 * MLP_WAVES) k_mlp_bf16(const float* __restrict__ X, int n,
                                                              const unsigned short* __restrict__ Wg,
                                                              const float* __restrict__ bg, float* __restrict__ Y) {
   __shared__ __attribute__((aligned(16))) unsigned short sW[WPACK];
   __shared__ float sB[BPACK];
-  __shared__ __attribute__((aligned(16))) unsigned short sH[MLP_WAVES][2][16][KP];
+  __shared__ __attribute__((aligned(16))) unsigned short sH[MLP_WAVES][MLP_TILES][16][KP];
   for (int k = threadIdx.x; k < WPACK / 8; k += blockDim.x)
     reinterpret_cast<uint4*>(sW)[k] = reinterpret_cast<const uint4*>(Wg)[k];
   for (int k = threadIdx.x; k < BPACK; k += blockDim.x) sB[k] = bg[k];
@@ -80,61 +90,64 @@ __global__ void __launch_bounds__(64 * MLP_WAVES) k_mlp_bf16(const float* __rest
   const unsigned short* W4 = W3 + W2T;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int r16 = lane & 15, kg = lane >> 4;
-  const int ntiles = (n + 15) / 16;
+  const int ngroups = (n + 16 * MLP_TILES - 1) / (16 * MLP_TILES);
   const int stride = gridDim.x * MLP_WAVES;
-  const int iters = (ntiles + stride - 1) / stride;
-  unsigned short(*h0)[KP] = sH[wv][0];
-  unsigned short(*h1)[KP] = sH[wv][1];
+  const int iters = (ngroups + stride - 1) / stride;
   for (int it = 0; it < iters; it++) {
-    const int tile = (it * gridDim.x + blockIdx.x) * MLP_WAVES + wv;
-    const int row0 = tile * 16;
+    const int row0 = ((it * gridDim.x + blockIdx.x) * MLP_WAVES + wv) * 16 * MLP_TILES;
     // layer 1: 14 (padded to 32) -> 128
-    bf16x8 a;
-    {
-      const int row = row0 + r16;
+    bf16x8 a[MLP_TILES];
+    for (int t = 0; t < MLP_TILES; t++) {
+      const int row = row0 + 16 * t + r16;
       for (int i = 0; i < 8; i++) {
         const int k = 8 * kg + i;
-        const float v = (row < n && k < SUR_IN) ? X[size_t(row) * SUR_XS + k] : 0.f;
-        a[i] = __builtin_bit_cast(__bf16, f2bf(v));
+        a[t][i] = static_cast<__bf16>((row < n && k < SUR_IN) ? X[size_t(row) * SUR_XS + k] : 0.f);
       }
     }
     for (int nb = 0; nb < 8; nb++) {
       const int col = nb * 16 + r16;
       const bf16x8 b = *reinterpret_cast<const bf16x8*>(W1 + col * K1P + 8 * kg);
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
-      for (int i = 0; i < 4; i++) h0[4 * kg + i][col] = f2bf(gelu(acc[i] + sB[col]));
+      const float bias = sB[col];
+      for (int t = 0; t < MLP_TILES; t++) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t], b, acc, 0, 0, 0);
+        for (int i = 0; i < 4; i++) sH[wv][t][4 * kg + i][col] = bf16_bits(gelu_fast(acc[i] + bias));
+      }
     }
     __syncthreads();
-    // layers 2 and 3: 128 -> 128
+    // layers 2 and 3: 128 -> 128, in place
     for (int layer = 0; layer < 2; layer++) {
       const unsigned short* Wl = layer == 0 ? W2 : W3;
-      unsigned short(*hin)[KP] = layer == 0 ? h0 : h1;
-      unsigned short(*hout)[KP] = layer == 0 ? h1 : h0;
       const float* bl = sB + H * (1 + layer);
-      bf16x8 af[4];
-      for (int kb = 0; kb < 4; kb++) af[kb] = *reinterpret_cast<const bf16x8*>(&hin[r16][kb * 32 + 8 * kg]);
+      bf16x8 af[MLP_TILES][4];
+      for (int t = 0; t < MLP_TILES; t++)
+        for (int kb = 0; kb < 4; kb++)
+          af[t][kb] = *reinterpret_cast<const bf16x8*>(&sH[wv][t][r16][kb * 32 + 8 * kg]);
+      __syncthreads();
       for (int nb = 0; nb < 8; nb++) {
         const int col = nb * 16 + r16;
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        f32x4 acc[MLP_TILES];
+        for (int t = 0; t < MLP_TILES; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
         for (int kb = 0; kb < 4; kb++) {
           const bf16x8 b = *reinterpret_cast<const bf16x8*>(Wl + col * KP + kb * 32 + 8 * kg);
-          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[kb], b, acc, 0, 0, 0);
+          for (int t = 0; t < MLP_TILES; t++) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[t][kb], b, acc[t], 0, 0, 0);
         }
-        for (int i = 0; i < 4; i++) hout[4 * kg + i][col] = f2bf(gelu(acc[i] + bl[col]));
+        const float bias = bl[col];
+        for (int t = 0; t < MLP_TILES; t++)
+          for (int i = 0; i < 4; i++) sH[wv][t][4 * kg + i][col] = bf16_bits(gelu_fast(acc[t][i] + bias));
       }
       __syncthreads();
     }
     // layer 4: 128 -> 7 (padded to 16), no activation
-    {
+    for (int t = 0; t < MLP_TILES; t++) {
       f32x4 acc = {0.f, 0.f, 0.f, 0.f};
       for (int kb = 0; kb < 4; kb++) {
-        const bf16x8 af = *reinterpret_cast<const bf16x8*>(&h0[r16][kb * 32 + 8 * kg]);
+        const bf16x8 af = *reinterpret_cast<const bf16x8*>(&sH[wv][t][r16][kb * 32 + 8 * kg]);
         const bf16x8 b = *reinterpret_cast<const bf16x8*>(W4 + r16 * KP + kb * 32 + 8 * kg);
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, b, acc, 0, 0, 0);
       }
       for (int i = 0; i < 4; i++) {
-        const int row = row0 + 4 * kg + i;
+        const int row = row0 + 16 * t + 4 * kg + i;
         if (row < n && r16 < SUR_OUT) Y[size_t(row) * SUR_YS + r16] = acc[i] + sB[3 * H + r16];
       }
     }
@@ -154,6 +167,12 @@ __device__ __forceinline__ float stage_dt(const DevParams* __restrict__ P, int s
 }
 
 __global__ void __launch_bounds__(256) k_sur_inputs(const DevParams* __restrict__ P, Arrays A, Buffers B, int stage) {
+  __shared__ float s_mean[SUR_IN], s_inv[SUR_IN];
+  if (threadIdx.x < SUR_IN) {
+    s_mean[threadIdx.x] = P->sur_norm[threadIdx.x];
+    s_inv[threadIdx.x] = 1.0f / P->sur_norm[SUR_IN + threadIdx.x];
+  }
+  __syncthreads();
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= P->npm) return;
   const float dt = stage_dt(P, stage);
@@ -166,8 +185,7 @@ __global__ void __launch_bounds__(256) k_sur_inputs(const DevParams* __restrict_
     use = hypo_cs_parts(P, A.codes[i].w, dt, A.vel[i].w, D, sig, x, nullptr, false);
   }
   float* o = B.sur_x + size_t(idx) * SUR_XS;
-  const float* nm = P->sur_norm;
-  for (int k = 0; k < SUR_IN; k++) o[k] = use ? (x[k] - nm[k]) / nm[SUR_IN + k] : 0.f;
+  for (int k = 0; k < SUR_IN; k++) o[k] = use ? (x[k] - s_mean[k]) * s_inv[k] : 0.f;
   o[14] = 0.f;
   o[15] = 0.f;
 }
@@ -245,21 +263,30 @@ __global__ void __launch_bounds__(256) k_sur_finish(DevParams* __restrict__ P, A
       A.idm[i] = id;
     }
   }
-  // statistics: one atomic per wave and category
+  // statistics: counts summed over the block, then one atomic per category
+  // into one of SUR_SLOTS counter lines (so blocks do not all hit one
+  // address); the host adds the slots up
+  __shared__ unsigned s_cnt[5];
+  __shared__ unsigned s_amax;
+  if (threadIdx.x < 5) s_cnt[threadIdx.x] = 0u;
+  if (threadIdx.x == 0) s_amax = 0u;
+  __syncthreads();
   const unsigned long long m1 = __ballot(cat == 1), m2 = __ballot(cat == 2), m3 = __ballot(cat == 3),
                            m4 = __ballot(cat == 4), ma = __ballot(audited);
   float em = aerr;
   for (int o = 32; o > 0; o >>= 1) em = fmaxf(em, __shfl_xor(em, o));
   if ((threadIdx.x & 63) == 0) {
-    if (m1) atomicAdd(&P->sur_stats[0], (unsigned long long)__popcll(m1));
-    if (m2) atomicAdd(&P->sur_stats[1], (unsigned long long)__popcll(m2));
-    if (m3) atomicAdd(&P->sur_stats[2], (unsigned long long)__popcll(m3));
-    if (m4) atomicAdd(&P->sur_stats[4], (unsigned long long)__popcll(m4));
-    if (ma) {
-      atomicAdd(&P->sur_stats[3], (unsigned long long)__popcll(ma));
-      atomicMax(P->sur_audit_max, __float_as_uint(em));
-    }
+    if (m1) atomicAdd(&s_cnt[0], unsigned(__popcll(m1)));
+    if (m2) atomicAdd(&s_cnt[1], unsigned(__popcll(m2)));
+    if (m3) atomicAdd(&s_cnt[2], unsigned(__popcll(m3)));
+    if (ma) atomicAdd(&s_cnt[3], unsigned(__popcll(ma)));
+    if (m4) atomicAdd(&s_cnt[4], unsigned(__popcll(m4)));
+    if (ma) atomicMax(&s_amax, __float_as_uint(em));
   }
+  __syncthreads();
+  unsigned long long* slot = P->sur_stats + 8 * (blockIdx.x & (SUR_SLOTS - 1));
+  if (threadIdx.x < 5 && s_cnt[threadIdx.x]) atomicAdd(&slot[threadIdx.x], (unsigned long long)s_cnt[threadIdx.x]);
+  if (threadIdx.x == 5 && s_amax) atomicMax(P->sur_audit_max + (blockIdx.x & (SUR_SLOTS - 1)) * 16, s_amax);
 }
 
 // Logs raw (x, exact y) of every log_every-th soil slot (rotating with the
@@ -276,7 +303,14 @@ __global__ void __launch_bounds__(256) k_sur_log(DevParams* __restrict__ P, Arra
   const Sym D = ldsym(B.acestn_a, B.acestn_b, i);
   float x[SUR_IN], y[SUR_OUT];
   if (!hypo_cs_parts(P, A.codes[i].w, dt, A.vel[i].w, D, sig, x, y, true)) return;
-  const unsigned slot = atomicAdd(P->log_count, 1u);
+  // one atomic per wave reserves the rows of all its logging lanes
+  const unsigned long long m = __ballot(1);
+  const int leader = __ffsll((long long)m) - 1;
+  const unsigned rank = unsigned(__popcll(m & ((1ull << (threadIdx.x & 63)) - 1ull)));
+  unsigned base = 0u;
+  if (int(threadIdx.x & 63) == leader) base = atomicAdd(P->log_count, unsigned(__popcll(m)));
+  base = __shfl(base, leader);
+  const unsigned slot = base + rank;
   if (slot >= unsigned(P->log_cap)) return;
   for (int k = 0; k < SUR_IN; k++) P->log_x[size_t(slot) * SUR_XS + k] = x[k];
   for (int k = 0; k < SUR_OUT; k++) P->log_y[size_t(slot) * SUR_YS + k] = y[k];
@@ -288,8 +322,8 @@ void launch_surrogate(hipStream_t s, DevParams* P, int npm_max, int members, int
                       int stage) {
   if (npm_max <= 0) return;
   hipLaunchKernelGGL(k_sur_inputs, dim3(blocks_of(npm_max, 256)), dim3(256), 0, s, P, A, B, stage);
-  const int ntiles = (npm_max + 15) / 16;
-  const int grid = std::max(1, std::min(n_cu, (ntiles + MLP_WAVES - 1) / MLP_WAVES));
+  const int ngroups = (npm_max + 16 * MLP_TILES - 1) / (16 * MLP_TILES);
+  const int grid = std::max(1, std::min(n_cu, (ngroups + MLP_WAVES - 1) / MLP_WAVES));
   for (int m = 0; m < members; m++)
     hipLaunchKernelGGL(k_mlp_bf16, dim3(grid), dim3(64 * MLP_WAVES), 0, s, (const float*)B.sur_x, npm_max,
                        (const unsigned short*)(B.sur_w + size_t(m) * WPACK), (const float*)(B.sur_b + m * BPACK),
@@ -300,6 +334,7 @@ void launch_sur_log(hipStream_t s, DevParams* P, int npm_max, Arrays A, const Bu
   if (npm_max > 0) hipLaunchKernelGGL(k_sur_log, dim3(blocks_of(npm_max, 256)), dim3(256), 0, s, P, A, B, stage);
 }
 int sur_wpack() { return WPACK; }
+int sur_slots() { return SUR_SLOTS; }
 int sur_bpack() { return BPACK; }
 
 // Host-side packing of one member's FP32 parameters into the LDS layout.
@@ -455,8 +490,8 @@ void launch_gather_rows(hipStream_t s, const float* src, int ld, const int* idx,
 }
 void launch_mlp_bf16(hipStream_t s, const float* X, int n, const unsigned short* W, const float* b, float* Y,
                      int n_cu) {
-  const int ntiles = (n + 15) / 16;
-  const int grid = std::max(1, std::min(n_cu, (ntiles + MLP_WAVES - 1) / MLP_WAVES));
+  const int ngroups = (n + 16 * MLP_TILES - 1) / (16 * MLP_TILES);
+  const int grid = std::max(1, std::min(n_cu, (ngroups + MLP_WAVES - 1) / MLP_WAVES));
   hipLaunchKernelGGL(k_mlp_bf16, dim3(grid), dim3(64 * MLP_WAVES), 0, s, X, n, W, b, Y);
 }
 
