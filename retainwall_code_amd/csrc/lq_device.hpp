@@ -61,6 +61,21 @@ __device__ __forceinline__ float pair_r2(float3 pi, float4 pj, int ox, int oy, i
   return fmaf(d.z, d.z, fmaf(d.y, d.y, d.x * d.x));
 }
 
+// Hit mask of the candidates [j0, j0 + m), m <= 32: bit t set when slot
+// j0 + t passes the cut-off.  The loop is unrolled so several candidate
+// loads are in flight; callers then walk the set bits in ascending order,
+// which keeps the reference's summation order.
+__device__ __forceinline__ unsigned hit_mask(const float4* __restrict__ pos, float3 pi, int j0, int m, int ox, int oy,
+                                             int oz, float two_h, float r2cut) {
+  unsigned mask = 0u;
+#pragma unroll 4
+  for (int t = 0; t < m; t++) {
+    float3 d;
+    if (pair_r2(pi, pos[j0 + t], ox, oy, oz, two_h, d) <= r2cut) mask |= 1u << t;
+  }
+  return mask;
+}
+
 // Hardware places workgroup b on XCD b % 8.  Renumber the 1-D grid so each
 // XCD runs one contiguous run of logical blocks: consecutive particles are
 // cell neighbours, so their candidate data then share that XCD's L2.

@@ -132,34 +132,40 @@ __global__ void __launch_bounds__(256) k_adami(const DevParams* __restrict__ P, 
         int cb = lin_key(P, jx, jy, jz);
         // boundary range: normal integral (contact walls only)
         if (contact) {
-          for (int j = cbeg[cb], e = cend[cb]; j < e; j++) {
-            float3 d;
-            const float r2 = pair_r2(pi, A.pos[j], cx, cy, cz, two_h, d);
-            if (!(r2 <= r2cut)) continue;
-            float r = sqrtf(r2);
-            float q = r * i_h;
-            float dwdr = kG(P->kgrad, q);
-            float3 g = f3mul(d, dwdr);
-            n = f3sub(n, f3mul(g, P->vol_def));
+          for (int j0 = cbeg[cb], e = cend[cb]; j0 < e; j0 += 32) {
+            unsigned mask = hit_mask(A.pos, pi, j0, min(e - j0, 32), cx, cy, cz, two_h, r2cut);
+            while (mask) {
+              const int j = j0 + __ffs(mask) - 1;
+              mask &= mask - 1u;
+              float3 d;
+              float r = sqrtf(pair_r2(pi, A.pos[j], cx, cy, cz, two_h, d));
+              float q = r * i_h;
+              float dwdr = kG(P->kgrad, q);
+              float3 g = f3mul(d, dwdr);
+              n = f3sub(n, f3mul(g, P->vol_def));
+            }
           }
         }
         int cm = cb + P->ncell_material;
-        for (int j = cbeg[cm], e = cend[cm]; j < e; j++) {
-          float3 d;
-          const float r2 = pair_r2(pi, A.pos[j], cx, cy, cz, two_h, d);
-          if (!(r2 <= r2cut)) continue;
-          float r = sqrtf(r2);
-          float q = r * i_h;
-          float ker = kW(P->knorm, q);
-          float4 idmj = A.idm[j];
-          float4 vj = A.vel[j];
-          float volj = idmj.y / vj.w;
-          float wv = ker * volj;
-          Sym sj = ldsym(A.str_a, A.str_b, j);
-          rs = sadd(rs, smul(sj, wv));
-          rpos = f3add(rpos, f3mul(d, ker * idmj.y));
-          rker = rker + wv;
-          rv = make_float4(rv.x + vj.x * wv, rv.y + vj.y * wv, rv.z + vj.z * wv, rv.w + vj.w * wv);
+        for (int j0 = cbeg[cm], e = cend[cm]; j0 < e; j0 += 32) {
+          unsigned mask = hit_mask(A.pos, pi, j0, min(e - j0, 32), cx, cy, cz, two_h, r2cut);
+          while (mask) {
+            const int j = j0 + __ffs(mask) - 1;
+            mask &= mask - 1u;
+            float3 d;
+            float r = sqrtf(pair_r2(pi, A.pos[j], cx, cy, cz, two_h, d));
+            float q = r * i_h;
+            float ker = kW(P->knorm, q);
+            float4 idmj = A.idm[j];
+            float4 vj = A.vel[j];
+            float volj = idmj.y / vj.w;
+            float wv = ker * volj;
+            Sym sj = ldsym(A.str_a, A.str_b, j);
+            rs = sadd(rs, smul(sj, wv));
+            rpos = f3add(rpos, f3mul(d, ker * idmj.y));
+            rker = rker + wv;
+            rv = make_float4(rv.x + vj.x * wv, rv.y + vj.y * wv, rv.z + vj.z * wv, rv.w + vj.w * wv);
+          }
         }
       }
   float nn = f3norm(n);
@@ -232,29 +238,32 @@ __global__ void __launch_bounds__(256) k_corner(const DevParams* __restrict__ P,
         int jx = ci.x + cx, jy = ci.y + cy, jz = ci.z + cz;
         if (!in_grid(P, jx, jy, jz)) continue;
         int c = lin_key(P, jx, jy, jz);
-        for (int j = cbeg[c], e = cend[c]; j < e; j++) {
-          float3 d;
-          const float r2 = pair_r2(pi, A.pos[j], cx, cy, cz, two_h, d);
-          if (!(r2 <= r2cut)) continue;
-          float r = sqrtf(r2);
-          float q = r * i_h;
-          int cj = A.codes[j].x;
-          if (cj < 10) continue;
-          float dwdr = kG(P->kgrad, q);
-          float3 g = f3mul(d, dwdr);
-          if (r < dij && r > 0.f) {
-            float3 na = ld3(A.cn[j]);
-            float3 nm = f3div(d, r);
-            dij = r;
-            cv.y = fabsf(f3dot(d, na));
-            cv.w = float(cj);
-            float4 vj = A.vel[j];
-            cw = make_float4(vj.x, vj.y, vj.z, 0.f);
-            float cosc = fabsf(f3dot(nm, na));
-            if (cosc < 0.866f) cn = make_float4(nm.x, nm.y, nm.z, 0.f);
-            else cn = make_float4(na.x, na.y, na.z, 0.f);
+        for (int j0 = cbeg[c], e = cend[c]; j0 < e; j0 += 32) {
+          unsigned mask = hit_mask(A.pos, pi, j0, min(e - j0, 32), cx, cy, cz, two_h, r2cut);
+          while (mask) {
+            const int j = j0 + __ffs(mask) - 1;
+            mask &= mask - 1u;
+            int cj = A.codes[j].x;
+            if (cj < 10) continue;
+            float3 d;
+            float r = sqrtf(pair_r2(pi, A.pos[j], cx, cy, cz, two_h, d));
+            float q = r * i_h;
+            float dwdr = kG(P->kgrad, q);
+            float3 g = f3mul(d, dwdr);
+            if (r < dij && r > 0.f) {
+              float3 na = ld3(A.cn[j]);
+              float3 nm = f3div(d, r);
+              dij = r;
+              cv.y = fabsf(f3dot(d, na));
+              cv.w = float(cj);
+              float4 vj = A.vel[j];
+              cw = make_float4(vj.x, vj.y, vj.z, 0.f);
+              float cosc = fabsf(f3dot(nm, na));
+              if (cosc < 0.866f) cn = make_float4(nm.x, nm.y, nm.z, 0.f);
+              else cn = make_float4(na.x, na.y, na.z, 0.f);
+            }
+            cv.x = cv.x + fabsf((cn.x * g.x + cn.y * g.y + cn.z * g.z) * P->vol_def);
           }
-          cv.x = cv.x + fabsf((cn.x * g.x + cn.y * g.y + cn.z * g.z) * P->vol_def);
         }
       }
   A.cvar[i] = cv;
@@ -385,7 +394,7 @@ __global__ void __launch_bounds__(256) k_interaction_ref(DevParams* __restrict__
 // An entry packs the slot j (bits 7..31), the cell offset (bits 0..5) and
 // the range (bit 6: 0 wall, 1 soil).
 constexpr int NL_WAVE = 64;
-constexpr int NL_WAVES = 4;
+constexpr int NL_WAVES = 2;
 constexpr int NL_CAP = 32;
 
 __device__ __forceinline__ void nl_eval(const DevParams* __restrict__ P, IAcc& S, const Arrays& A, const Buffers& B,
@@ -443,8 +452,8 @@ __global__ void __launch_bounds__(NL_WAVE * NL_WAVES) k_interaction_nl(DevParams
     while (j < e) {
       const int m = min(e - j, NL_CAP);
       unsigned mask = 0u;
-#pragma unroll 1
-      for (int t = 0; t < m; t++) {
+#pragma unroll 4
+      for (int t = 0; t < m; t++) {  // unrolled: several candidate loads in flight
         float3 d;
         if (pair_r2(pi, A.pos[j + t], cx, cy, cz, two_h, d) <= r2cut) mask |= 1u << t;
       }
