@@ -89,9 +89,13 @@ def bench_batch(args):
     ctx = sim.open_batch(cs)
     ctx.step(args.warmup, graph=args.graph)
     ctx.sync()
-    variant, shape, tz, remap, _ = ctx.autotune(reps=2)
     names = {_abi.INTERACTION_TILE: "tile", _abi.INTERACTION_REFERENCE_ORDER: "reference",
              _abi.INTERACTION_NEIGHBOR_LIST: "neighbor_list", _abi.INTERACTION_PAIR_LIST: "pair_list"}
+    if args.variant == "auto":
+        variant, shape, tz, remap, _ = ctx.autotune(reps=2)
+    else:
+        variant, remap = {v: k for k, v in names.items()}[args.variant], args.remap
+        ctx.set_interaction(variant, remap)
     if not args.graph:
         ctx.enable_timing(True)
     ctx.sync()
@@ -104,7 +108,7 @@ def bench_batch(args):
     ctx.close()
     # sequential one-case contexts on a sample
     rng = np.random.default_rng(1)
-    sample = sorted(rng.choice(len(cs), size=min(16, len(cs)), replace=False))
+    sample = [] if args.no_sample else sorted(rng.choice(len(cs), size=min(16, len(cs)), replace=False))
     xs, ys = [], []
     for k in sample:
         one = sim.open_case(cs[k])
@@ -116,7 +120,7 @@ def bench_batch(args):
         ys.append(1e3 * (time.perf_counter() - ta) / 5)
         xs.append(cs[k].npm)
         one.close()
-    b1, b0 = np.polyfit(np.array(xs, float), np.array(ys, float), 1)
+    b1, b0 = np.polyfit(np.array(xs, float), np.array(ys, float), 1) if len(xs) > 1 else (float("nan"), float("nan"))
     seq_ms = float(sum(b0 + b1 * c.npm for c in cs))
     out = {
         "metric": "particle-steps/sec", "value": round(npm * args.steps / el, 1), "unit": "particle-steps/s",
@@ -212,6 +216,8 @@ def main():
     ap.add_argument("--tile", default="", metavar="SHAPE,TZ", help="fix the tile shape/depth instead of autotuning")
     ap.add_argument("--batch", type=int, default=0, metavar="N",
                     help="batched workload: N independent wall cases in one context (256 = the c4 workload)")
+    ap.add_argument("--no-sample", action="store_true",
+                    help="with --batch: skip the one-case-per-context comparison runs")
     ap.add_argument("--surrogate", action="store_true",
                     help="gain test of the HypoCS surrogate on the case: log, train, then time both forms")
     ap.add_argument("--ens-tol", type=float, default=0.1, help="surrogate ensemble tolerance (normalised units)")

@@ -586,7 +586,8 @@ extern "C" int lq_upload_case(lq_ctx* c, int k, const lq_host_state* h) {
   const float scale = 0.5f * cp.i_h;
   launch_abs2rel(c->stream, c->dP, A, scale, int(b0), npb);
   launch_abs2rel(c->stream, c->dP, A, scale, int(s0), npm);
-  launch_set_massflag(c->stream, c->dP, c->par.np, A);
+  launch_set_massflag_range(c->stream, c->dP, A, int(b0), npb);
+  launch_set_massflag_range(c->stream, c->dP, A, int(s0), npm);
   HIPCHECK(c, hipGetLastError());
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   if (!c->case_uploaded[k]) { c->case_uploaded[k] = 1; c->n_uploaded++; }

@@ -6,6 +6,7 @@ namespace lq {
 
 void launch_abs2rel(hipStream_t s, const DevParams* P, Arrays A, float scale, int first, int n);
 void launch_set_massflag(hipStream_t s, const DevParams* P, int np, Arrays A);
+void launch_set_massflag_range(hipStream_t s, const DevParams* P, Arrays A, int first, int n);
 void launch_copy_w(hipStream_t s, const DevParams* P, int np, const float4* src, float4* dst);
 void launch_rel2abs(hipStream_t s, const DevParams* P, int np, Arrays A, float4* out);
 void launch_step_calc(hipStream_t s, DevParams* P);

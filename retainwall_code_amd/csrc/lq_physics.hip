@@ -26,9 +26,10 @@ __global__ void k_abs2rel(const DevParams* __restrict__ P, Arrays A, float scale
 // pos.w carries the neighbour data the interaction needs with the position:
 // the mass of a soil particle (idmassidx.y) and, in the sign, whether the
 // particle's code marks a contact wall (codes.x >= 10).
-__global__ void k_set_massflag(const DevParams* __restrict__ P, Arrays A) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P->np) return;
+__global__ void k_set_massflag(const DevParams* __restrict__ P, Arrays A, int first, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int i = first + k;
   float4 p = A.pos[i];
   const float4 id = A.idm[i];
   const float m = id.x >= 100.0f ? id.y : 1.0f;
@@ -80,7 +81,8 @@ __global__ void k_reset_deferred(DevParams* P, int list_build) {
 // RigidBodyUpdate, adami_boundary.cu:175-203 (one thread).
 __global__ void k_rigid_body_update(DevParams* P) {
   const float t = P->time, dt = P->dt;
-  for (int i = 0; i < P->n_motion; i++) {
+  // one thread per body: each body touches only its own table and motion
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P->n_motion; i += gridDim.x * blockDim.x) {
     DevTable& m = P->tab[i];
     DevMotion& mv = P->mv[i];
     if (m.is_rotation) {
@@ -111,7 +113,7 @@ __global__ void k_rigid_body_update(DevParams* P) {
 
 // ------------------------------------------------------------- Adami ----
 // cuAdamiBoundaryMaterial + loops, adami_boundary.cu:26-158.
-__global__ void __launch_bounds__(256) k_adami(const DevParams* __restrict__ P, Arrays A,
+__global__ void __launch_bounds__(64) k_adami(const DevParams* __restrict__ P, Arrays A,
                                                const int* __restrict__ cbeg, const int* __restrict__ cend, int remap) {
   int i = xcd_block(remap) * blockDim.x + threadIdx.x;
   if (i >= P->npb) return;
@@ -789,7 +791,10 @@ void launch_abs2rel(hipStream_t s, const DevParams* P, Arrays A, float scale, in
   if (n > 0) hipLaunchKernelGGL(k_abs2rel, dim3(blocks(n, 256)), dim3(256), 0, s, P, A, scale, first, n);
 }
 void launch_set_massflag(hipStream_t s, const DevParams* P, int np, Arrays A) {
-  if (np > 0) hipLaunchKernelGGL(k_set_massflag, dim3(blocks(np, 256)), dim3(256), 0, s, P, A);
+  launch_set_massflag_range(s, P, A, 0, np);
+}
+void launch_set_massflag_range(hipStream_t s, const DevParams* P, Arrays A, int first, int n) {
+  if (n > 0) hipLaunchKernelGGL(k_set_massflag, dim3(blocks(n, 256)), dim3(256), 0, s, P, A, first, n);
 }
 void launch_copy_w(hipStream_t s, const DevParams* P, int np, const float4* src, float4* dst) {
   if (np > 0) hipLaunchKernelGGL(k_copy_w, dim3(blocks(np, 256)), dim3(256), 0, s, P, src, dst);
@@ -800,10 +805,11 @@ void launch_rel2abs(hipStream_t s, const DevParams* P, int np, Arrays A, float4*
 void launch_step_calc(hipStream_t s, DevParams* P) { hipLaunchKernelGGL(k_step_calc, dim3(1), dim3(1), 0, s, P); }
 void launch_advance(hipStream_t s, DevParams* P) { hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, s, P); }
 void launch_rigid_body_update(hipStream_t s, DevParams* P) {
-  hipLaunchKernelGGL(k_rigid_body_update, dim3(1), dim3(1), 0, s, P);
+  hipLaunchKernelGGL(k_rigid_body_update, dim3(4), dim3(64), 0, s, P);
 }
 void launch_adami(hipStream_t s, const DevParams* P, int npb, Arrays A, const int* cb, const int* ce) {
-  if (npb > 0) hipLaunchKernelGGL(k_adami, dim3(blocks(npb, 256)), dim3(256), 0, s, P, A, cb, ce, 1);
+  // 64-thread blocks: ~100k wall particles spread evenly over the CUs
+  if (npb > 0) hipLaunchKernelGGL(k_adami, dim3(blocks(npb, 64)), dim3(64), 0, s, P, A, cb, ce, 1);
 }
 void launch_corner(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const int* cb, const int* ce,
                    const int* wall_near) {
