@@ -113,7 +113,8 @@ def _cmp(tag, g, o32, o64, keys, rows, state_keys=()):
     return out
 
 
-@pytest.mark.parametrize("variant", [_abi.INTERACTION_TILE, _abi.INTERACTION_REFERENCE_ORDER])
+@pytest.mark.parametrize("variant", [_abi.INTERACTION_NEIGHBOR_LIST, _abi.INTERACTION_TILE,
+                                     _abi.INTERACTION_REFERENCE_ORDER])
 def test_step_pieces(variant, oracle_mod):
     case = cases.config("small")
     say(f"[pieces] case=small variant={variant}")
