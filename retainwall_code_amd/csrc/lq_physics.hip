@@ -387,29 +387,55 @@ __global__ void __launch_bounds__(256) k_interaction_ref(DevParams* __restrict__
 // Neighbour-list variant.  The wave walks the 27 cells x {wall, soil}
 // ranges in reference order (loop indices are wave-uniform, so the lanes of
 // one home cell read the same candidate at the same time).  Each range is
-// tested in chunks of NL_CAP candidates into a per-lane hit mask; the hits
-// are appended to a per-lane list in LDS, and the lists are evaluated --
-// every lane working through its own entries in order -- only when some
-// lane's next chunk would not fit, and at the end.  The expensive pair
-// terms thus run on hits gathered over many ranges in near-full-width wave
-// iterations, while each particle still sums its pairs in reference order.
+// tested in chunks of NL_CHUNK candidates into a per-lane hit mask; the hits
+// are appended to a per-lane FIFO ring of NL_CAP entries in LDS.  When some
+// lane's next hits would not fit, the wave evaluates ring entries -- every
+// lane with entries pops its oldest one per iteration -- only until they do,
+// and drains the rings at the end.  Lanes thus stay busy on pair terms
+// gathered over many ranges, while each particle still sums its pairs in
+// reference order (a FIFO keeps the insertion order).
 // An entry packs the slot j (bits 7..31), the cell offset (bits 0..5) and
 // the range (bit 6: 0 wall, 1 soil).
 constexpr int NL_WAVE = 64;
 constexpr int NL_WAVES = 2;
-constexpr int NL_CAP = 32;
+constexpr int NL_CAP = 32;    // ring entries per lane (power of two)
+constexpr int NL_CHUNK = 32;  // candidates per hit mask
+static_assert((NL_CAP & (NL_CAP - 1)) == 0 && NL_CAP >= NL_CHUNK, "ring must hold a full chunk of hits");
 
-__device__ __forceinline__ void nl_eval(const DevParams* __restrict__ P, IAcc& S, const Arrays& A, const Buffers& B,
-                                        int e, float3 pi, float two_h, float i_h, float3 vi, float rhoi, Sym sri,
-                                        float artpi, int shift) {
+// The neighbour's four 16-byte records (position and mass, velocity and
+// density, sigma/rho^2, its last two components with m/rho and the trace).
+struct NbrRec {
+  float4 p, v, a, b;
+};
+
+__device__ __forceinline__ NbrRec nl_load(const Arrays& A, const Buffers& B, int e) {
   const int j = int(unsigned(e) >> 7);
+  return {A.pos[j], A.vel[j], B.sr_a[j], B.sr_b[j]};
+}
+
+__device__ __forceinline__ void nl_eval(const PairConst& K, IAcc& S, int e, NbrRec n, float3 pi, float two_h,
+                                        float i_h, float3 vi, float rhoi, Sym sri, float artpi, int shift) {
+  // Use every component here so each record is fetched by one 16-byte load
+  // (otherwise parts only the soil branch needs are fetched separately).
+  asm volatile("" : "+v"(n.b.x), "+v"(n.b.y), "+v"(n.b.z), "+v"(n.b.w), "+v"(n.a.x), "+v"(n.a.y), "+v"(n.a.z),
+               "+v"(n.a.w));
   const int ox = (e & 3) - 1, oy = ((e >> 2) & 3) - 1, oz = ((e >> 4) & 3) - 1;
-  const float4 pj = A.pos[j];
   float3 d;
-  const float r = sqrtf(pair_r2(pi, pj, ox, oy, oz, two_h, d));
+  const float r = sqrtf(pair_r2(pi, n.p, ox, oy, oz, two_h, d));
   const float q = r * i_h;
-  pair_terms(pair_const(P), S, d, r, q, vi, rhoi, sri, artpi, shift, (e & 64) == 0, A.vel[j], pj.w, B.sr_a[j],
-             B.sr_b[j]);
+  pair_terms(K, S, d, r, q, vi, rhoi, sri, artpi, shift, (e & 64) == 0, n.v, n.p.w, n.a, n.b);
+}
+
+// Evaluate the lane's `pop` oldest ring entries in order.
+__device__ __forceinline__ void nl_drain(const PairConst& K, IAcc& S, const Arrays& A, const Buffers& B,
+                                         const int (*lst)[NL_WAVE], int lane, int head, int pop, float3 pi,
+                                         float two_h, float i_h, float3 vi, float rhoi, Sym sri, float artpi,
+                                         int shift) {
+#pragma unroll 1
+  for (int k = 0; k < pop; k++) {
+    const int en = lst[(head + k) & (NL_CAP - 1)][lane];
+    nl_eval(K, S, en, nl_load(A, B, en), pi, two_h, i_h, vi, rhoi, sri, artpi, shift);
+  }
 }
 
 __global__ void __launch_bounds__(NL_WAVE * NL_WAVES) k_interaction_nl(DevParams* __restrict__ P, Arrays A, Buffers B,
@@ -432,13 +458,14 @@ __global__ void __launch_bounds__(NL_WAVE * NL_WAVES) k_interaction_nl(DevParams
   const int shift = int(A.cvar[i].z);
   const float two_h = P->two_h, i_h = P->i_h, r2cut = P->r2cut;
   const int ncm = P->ncell_material;
+  const PairConst K = pair_const(P);
   int (*lst)[NL_WAVE] = s_list[threadIdx.x / NL_WAVE];
   IAcc S;
   S.accden = 0.f;
   S.as = S.av = S.aa = make_float3(0.f, 0.f, 0.f);
 #pragma unroll
   for (int k = 0; k < 9; k++) S.gv[k] = 0.f;
-  int n = 0;
+  int head = 0, cnt = 0;  // ring: oldest entry, number of entries
 #pragma unroll 1
   for (int r = 0; r < 54; r++) {
     const int cell = r >> 1, pass = r & 1;
@@ -452,28 +479,37 @@ __global__ void __launch_bounds__(NL_WAVE * NL_WAVES) k_interaction_nl(DevParams
     }
     const unsigned code = unsigned((cx + 1) | ((cy + 1) << 2) | ((cz + 1) << 4) | (pass << 6));
     while (j < e) {
-      const int m = min(e - j, NL_CAP);
+      const int m = min(e - j, NL_CHUNK);
       unsigned mask = 0u;
 #pragma unroll 4
       for (int t = 0; t < m; t++) {  // unrolled: several candidate loads in flight
         float3 d;
         if (pair_r2(pi, A.pos[j + t], cx, cy, cz, two_h, d) <= r2cut) mask |= 1u << t;
       }
-      if (__ballot(n + __popc(mask) > NL_CAP)) {  // some lane could overflow: evaluate the lists
-#pragma unroll 1
-        for (int k = 0; k < n; k++) nl_eval(P, S, A, B, lst[k][lane], pi, two_h, i_h, vi, rhoi, sri, artpi, shift);
-        n = 0;
+      const int over = cnt + __popc(mask) - NL_CAP;
+      if (__ballot(over > 0)) {
+        // make room: need = largest overflow over the active lanes (binary
+        // search with ballots, 0 < need <= NL_CHUNK); every lane with entries
+        // pops up to need of them, the overflowing ones exactly enough
+        int need = 0;
+#pragma unroll
+        for (int b = NL_CHUNK; b; b >>= 1)
+          if (__ballot(over >= need + b)) need += b;
+        const int pop = min(cnt, need);
+        nl_drain(K, S, A, B, lst, lane, head, pop, pi, two_h, i_h, vi, rhoi, sri, artpi, shift);
+        head = (head + pop) & (NL_CAP - 1);
+        cnt -= pop;
       }
       while (mask) {
         const int t = __ffs(mask) - 1;
         mask &= mask - 1u;
-        lst[n++][lane] = int((unsigned(j + t) << 7) | code);
+        lst[(head + cnt) & (NL_CAP - 1)][lane] = int((unsigned(j + t) << 7) | code);
+        cnt++;
       }
       j += m;
     }
   }
-#pragma unroll 1
-  for (int k = 0; k < n; k++) nl_eval(P, S, A, B, lst[k][lane], pi, two_h, i_h, vi, rhoi, sri, artpi, shift);
+  nl_drain(K, S, A, B, lst, lane, head, cnt, pi, two_h, i_h, vi, rhoi, sri, artpi, shift);
   if (active) interaction_epilogue(P, A, B, i, S, vi);
 }
 
