@@ -121,7 +121,6 @@ __device__ __forceinline__ void pair_terms(const PairConst& P, IAcc& S, float3 d
   // per-particle terms from k_pair_prep: sigma/rho^2 (xx yy zz xy | xz yz),
   // m/rho and trace(sigma/rho^2), each rounded exactly as the inline form.
   const bool contactj = wj < 0.f;
-  float ker = kW(P.knorm, q);
   float dwdr = kG(P.kgrad, q);
   float3 g = f3mul(d, dwdr);
   float rhoj = vj.w;
@@ -133,26 +132,32 @@ __device__ __forceinline__ void pair_terms(const PairConst& P, IAcc& S, float3 d
   S.gv[0] = fmaf(-dvv.x, g.x, S.gv[0]); S.gv[1] = fmaf(-dvv.x, g.y, S.gv[1]); S.gv[2] = fmaf(-dvv.x, g.z, S.gv[2]);
   S.gv[3] = fmaf(-dvv.y, g.x, S.gv[3]); S.gv[4] = fmaf(-dvv.y, g.y, S.gv[4]); S.gv[5] = fmaf(-dvv.y, g.z, S.gv[5]);
   S.gv[6] = fmaf(-dvv.z, g.x, S.gv[6]); S.gv[7] = fmaf(-dvv.z, g.y, S.gv[7]); S.gv[8] = fmaf(-dvv.z, g.z, S.gv[8]);
-  if (contactj) return;
+  // The remaining terms do not apply to contact-wall neighbours
+  // (particle_interaction.cu:66).  They are added with a zero factor rather
+  // than branched around: fmaf(x, 0, s) == s for every finite x, because
+  // these sums start at +0 and a round-to-nearest sum is never -0 unless
+  // both addends are, so no sum here ever holds -0.
   float dvg = fmaf(dv.z, g.z, fmaf(dv.y, g.y, dv.x * g.x));
-  S.accden = fmaf(rhoi * dvg, volj, S.accden);
+  S.accden = fmaf(contactj ? 0.f : rhoi * dvg, volj, S.accden);
   const Sym srj = {sra.x, sra.y, sra.z, sra.w, srb.x, srb.y};
   Sym st = sadd(sri, srj);
   float3 f = smatvec(st, g);
-  S.as.x = fmaf(f.x, mj, S.as.x); S.as.y = fmaf(f.y, mj, S.as.y); S.as.z = fmaf(f.z, mj, S.as.z);
+  const float mf = contactj ? 0.f : mj;
+  S.as.x = fmaf(f.x, mf, S.as.x); S.as.y = fmaf(f.y, mf, S.as.y); S.as.z = fmaf(f.z, mf, S.as.z);
   if (P.viscosity_type == 1) {
+    // evaluated for every pair (in a wave some lane nearly always needs
+    // it); only approaching pairs (visc < 0) add it
     float visc = fmaf(dv.z, d.z, fmaf(dv.y, d.y, dv.x * d.x));
-    if (visc < 0.f) {
-      float phi = P.h * visc / fmaf(r, r, P.eta);
-      float artv = P.alpha_cs * phi / (0.5f * (rhoi + rhoj));
-      float s = mj * artv;
-      S.av.x = fmaf(g.x, s, S.av.x); S.av.y = fmaf(g.y, s, S.av.y); S.av.z = fmaf(g.z, s, S.av.z);
-    }
+    float phi = P.h * visc / fmaf(r, r, P.eta);
+    float artv = P.alpha_cs * phi / (0.5f * (rhoi + rhoj));
+    float s = (visc < 0.f && !contactj) ? mj * artv : 0.f;
+    S.av.x = fmaf(g.x, s, S.av.x); S.av.y = fmaf(g.y, s, S.av.y); S.av.z = fmaf(g.z, s, S.av.z);
   }
   if (P.art_p != 0.f) {
     float trj = srb.w;
     float artp = artpi + (trj > 0.f ? trj * P.art_p : 0.f);
-    if (artp > 0.f) {
+    if (artp > 0.f && !contactj) {
+      float ker = kW(P.knorm, q);
       artp = artp * powf(ker / P.kernel_dr, P.art_exp);
       float s = mj * artp;
       S.aa.x = fmaf(-g.x, s, S.aa.x); S.aa.y = fmaf(-g.y, s, S.aa.y); S.aa.z = fmaf(-g.z, s, S.aa.z);

@@ -413,15 +413,15 @@ __device__ __forceinline__ NbrRec nl_load(const Arrays& A, const Buffers& B, int
   return {A.pos[j], A.vel[j], B.sr_a[j], B.sr_b[j]};
 }
 
-__device__ __forceinline__ void nl_eval(const PairConst& K, IAcc& S, int e, NbrRec n, float3 pi, float two_h,
+__device__ __forceinline__ void nl_eval(const PairConst& K, IAcc& S, int e, const NbrRec& n, float3 pi, float two_h,
                                         float i_h, float3 vi, float rhoi, Sym sri, float artpi, int shift) {
-  // Use every component here so each record is fetched by one 16-byte load
-  // (otherwise parts only the soil branch needs are fetched separately).
-  asm volatile("" : "+v"(n.b.x), "+v"(n.b.y), "+v"(n.b.z), "+v"(n.b.w), "+v"(n.a.x), "+v"(n.a.y), "+v"(n.a.z),
-               "+v"(n.a.w));
-  const int ox = (e & 3) - 1, oy = ((e >> 2) & 3) - 1, oz = ((e >> 4) & 3) - 1;
+  // cell offset o in {-1, 0, 1} per axis, stored as o + 1: (o + 1) * 2h - 2h
+  // is exact, so these equal float(o) * 2h of pair_r2
   float3 d;
-  const float r = sqrtf(pair_r2(pi, n.p, ox, oy, oz, two_h, d));
+  d.x = (pi.x - n.p.x) - fmaf(float(e & 3), two_h, -two_h);
+  d.y = (pi.y - n.p.y) - fmaf(float((e >> 2) & 3), two_h, -two_h);
+  d.z = (pi.z - n.p.z) - fmaf(float((e >> 4) & 3), two_h, -two_h);
+  const float r = sqrtf(fmaf(d.z, d.z, fmaf(d.y, d.y, d.x * d.x)));
   const float q = r * i_h;
   pair_terms(K, S, d, r, q, vi, rhoi, sri, artpi, shift, (e & 64) == 0, n.v, n.p.w, n.a, n.b);
 }
@@ -438,7 +438,10 @@ __device__ __forceinline__ void nl_drain(const PairConst& K, IAcc& S, const Arra
   }
 }
 
-__global__ void __launch_bounds__(NL_WAVE * NL_WAVES) k_interaction_nl(DevParams* __restrict__ P, Arrays A, Buffers B,
+// At most 96 VGPRs (5 waves per SIMD, 20 per CU, which the 8 KB of ring per
+// wave also allows): the kernel is bound by its neighbour gathers and needs
+// the waves to keep them in flight.
+__global__ void __launch_bounds__(NL_WAVE * NL_WAVES, 5) k_interaction_nl(DevParams* __restrict__ P, Arrays A, Buffers B,
                                                                       const int* __restrict__ cbeg,
                                                                       const int* __restrict__ cend, int remap) {
   __shared__ int s_list[NL_WAVES][NL_CAP][NL_WAVE];
