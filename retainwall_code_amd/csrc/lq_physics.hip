@@ -113,65 +113,95 @@ __global__ void k_rigid_body_update(DevParams* P) {
 
 // ------------------------------------------------------------- Adami ----
 // cuAdamiBoundaryMaterial + loops, adami_boundary.cu:26-158.
+// Two thread roles in one launch, each walking the 27 cells in reference
+// order: threads [0, npb) sum the soil terms of wall slot t (the Adami
+// extrapolation), threads [npb, 2 npb) the wall-normal integral of
+// contact-wall slot t - npb.  The two sums have disjoint inputs and outputs,
+// so splitting them changes no result while halving the chain of dependent
+// loads of a contact-wall thread; the next cell's range is fetched while
+// the current one is scanned.  The kernel is latency-bound (one thread per
+// wall slot fills about one wave per SIMD).
 __global__ void __launch_bounds__(64) k_adami(const DevParams* __restrict__ P, Arrays A,
                                                const int* __restrict__ cbeg, const int* __restrict__ cend, int remap) {
-  int i = xcd_block(remap) * blockDim.x + threadIdx.x;
-  if (i >= P->npb) return;
+  const int t = xcd_block(remap) * blockDim.x + threadIdx.x;
+  const int npb = P->npb;
+  if (t >= 2 * npb) return;
+  const bool normal_role = t >= npb;
+  const int i = normal_role ? t - npb : t;
+  const bool contact = A.codes[i].x >= 10;
+  if (normal_role && !contact) return;
   const float3 pi = ld3(A.pos[i]);
   const int4 ci = A.cellpos[i];
-  const bool contact = A.codes[i].x >= 10;
   const float two_h = P->two_h, i_h = P->i_h, r2cut = P->r2cut;
+  const int roff = normal_role ? 0 : P->ncell_material;  // wall ranges for the normal, soil ranges otherwise
   float3 n = make_float3(0.f, 0.f, 0.f);
   Sym rs = {0, 0, 0, 0, 0, 0};
   float3 rpos = make_float3(0.f, 0.f, 0.f);
   float rker = 0.f;
   float4 rv = make_float4(0.f, 0.f, 0.f, 0.f);
-  for (int cz = -1; cz <= 1; cz++)
-    for (int cy = -1; cy <= 1; cy++)
-      for (int cx = -1; cx <= 1; cx++) {
-        int jx = ci.x + cx, jy = ci.y + cy, jz = ci.z + cz;
-        if (!in_grid(P, jx, jy, jz)) continue;
-        int cb = lin_key(P, jx, jy, jz);
-        // boundary range: normal integral (contact walls only)
-        if (contact) {
-          for (int j0 = cbeg[cb], e = cend[cb]; j0 < e; j0 += 32) {
-            unsigned mask = hit_mask(A.pos, pi, j0, min(e - j0, 32), cx, cy, cz, two_h, r2cut);
-            while (mask) {
-              const int j = j0 + __ffs(mask) - 1;
-              mask &= mask - 1u;
-              float3 d;
-              float r = sqrtf(pair_r2(pi, A.pos[j], cx, cy, cz, two_h, d));
-              float q = r * i_h;
-              float dwdr = kG(P->kgrad, q);
-              float3 g = f3mul(d, dwdr);
-              n = f3sub(n, f3mul(g, P->vol_def));
-            }
-          }
-        }
-        int cm = cb + P->ncell_material;
-        for (int j0 = cbeg[cm], e = cend[cm]; j0 < e; j0 += 32) {
-          unsigned mask = hit_mask(A.pos, pi, j0, min(e - j0, 32), cx, cy, cz, two_h, r2cut);
-          while (mask) {
-            const int j = j0 + __ffs(mask) - 1;
-            mask &= mask - 1u;
-            float3 d;
-            float r = sqrtf(pair_r2(pi, A.pos[j], cx, cy, cz, two_h, d));
-            float q = r * i_h;
-            float ker = kW(P->knorm, q);
-            float4 idmj = A.idm[j];
-            float4 vj = A.vel[j];
-            float volj = idmj.y / vj.w;
-            float wv = ker * volj;
-            Sym sj = ldsym(A.str_a, A.str_b, j);
-            rs = sadd(rs, smul(sj, wv));
-            rpos = f3add(rpos, f3mul(d, ker * idmj.y));
-            rker = rker + wv;
-            rv = make_float4(rv.x + vj.x * wv, rv.y + vj.y * wv, rv.z + vj.z * wv, rv.w + vj.w * wv);
-          }
+  int b = 0, e = 0;  // range of cell k, fetched one cell ahead
+  if (in_grid(P, ci.x - 1, ci.y - 1, ci.z - 1)) {
+    const int c = lin_key(P, ci.x - 1, ci.y - 1, ci.z - 1) + roff;
+    b = cbeg[c];
+    e = cend[c];
+  }
+#pragma unroll 1
+  for (int k = 0; k < 27; k++) {
+    const int cz = k / 9 - 1, cy = (k / 3) % 3 - 1, cx = k % 3 - 1;
+    int bn = 0, en = 0;
+    if (k < 26) {
+      const int kn = k + 1;
+      const int nx = ci.x + kn % 3 - 1, ny = ci.y + (kn / 3) % 3 - 1, nz = ci.z + kn / 9 - 1;
+      if (in_grid(P, nx, ny, nz)) {
+        const int c = lin_key(P, nx, ny, nz) + roff;
+        bn = cbeg[c];
+        en = cend[c];
+      }
+    }
+    if (normal_role) {
+      for (int j0 = b; j0 < e; j0 += 32) {
+        unsigned mask = hit_mask(A.pos, pi, j0, min(e - j0, 32), cx, cy, cz, two_h, r2cut);
+        while (mask) {
+          const int j = j0 + __ffs(mask) - 1;
+          mask &= mask - 1u;
+          float3 d;
+          float r = sqrtf(pair_r2(pi, A.pos[j], cx, cy, cz, two_h, d));
+          float q = r * i_h;
+          float dwdr = kG(P->kgrad, q);
+          float3 g = f3mul(d, dwdr);
+          n = f3sub(n, f3mul(g, P->vol_def));
         }
       }
-  float nn = f3norm(n);
-  if (nn > 0.f) A.cn[i] = make_float4(n.x / nn, n.y / nn, n.z / nn, 0.f);
+    } else {
+      for (int j0 = b; j0 < e; j0 += 32) {
+        unsigned mask = hit_mask(A.pos, pi, j0, min(e - j0, 32), cx, cy, cz, two_h, r2cut);
+        while (mask) {
+          const int j = j0 + __ffs(mask) - 1;
+          mask &= mask - 1u;
+          float3 d;
+          float r = sqrtf(pair_r2(pi, A.pos[j], cx, cy, cz, two_h, d));
+          float q = r * i_h;
+          float ker = kW(P->knorm, q);
+          float4 idmj = A.idm[j];
+          float4 vj = A.vel[j];
+          float volj = idmj.y / vj.w;
+          float wv = ker * volj;
+          Sym sj = ldsym(A.str_a, A.str_b, j);
+          rs = sadd(rs, smul(sj, wv));
+          rpos = f3add(rpos, f3mul(d, ker * idmj.y));
+          rker = rker + wv;
+          rv = make_float4(rv.x + vj.x * wv, rv.y + vj.y * wv, rv.z + vj.z * wv, rv.w + vj.w * wv);
+        }
+      }
+    }
+    b = bn;
+    e = en;
+  }
+  if (normal_role) {
+    float nn = f3norm(n);
+    if (nn > 0.f) A.cn[i] = make_float4(n.x / nn, n.y / nn, n.z / nn, 0.f);
+    return;
+  }
   if (rker > 0.f) {
     float4 nrm = A.normal[i];
     float fc = nrm.w;
@@ -234,40 +264,58 @@ __global__ void __launch_bounds__(256) k_corner(const DevParams* __restrict__ P,
   float4 cn = A.cn[i];
   float4 cw = A.cw[i];
   float dij = 999.f;
-  for (int cz = -1; cz <= 1; cz++)
-    for (int cy = -1; cy <= 1; cy++)
-      for (int cx = -1; cx <= 1; cx++) {
-        int jx = ci.x + cx, jy = ci.y + cy, jz = ci.z + cz;
-        if (!in_grid(P, jx, jy, jz)) continue;
-        int c = lin_key(P, jx, jy, jz);
-        for (int j0 = cbeg[c], e = cend[c]; j0 < e; j0 += 32) {
-          unsigned mask = hit_mask(A.pos, pi, j0, min(e - j0, 32), cx, cy, cz, two_h, r2cut);
-          while (mask) {
-            const int j = j0 + __ffs(mask) - 1;
-            mask &= mask - 1u;
-            int cj = A.codes[j].x;
-            if (cj < 10) continue;
-            float3 d;
-            float r = sqrtf(pair_r2(pi, A.pos[j], cx, cy, cz, two_h, d));
-            float q = r * i_h;
-            float dwdr = kG(P->kgrad, q);
-            float3 g = f3mul(d, dwdr);
-            if (r < dij && r > 0.f) {
-              float3 na = ld3(A.cn[j]);
-              float3 nm = f3div(d, r);
-              dij = r;
-              cv.y = fabsf(f3dot(d, na));
-              cv.w = float(cj);
-              float4 vj = A.vel[j];
-              cw = make_float4(vj.x, vj.y, vj.z, 0.f);
-              float cosc = fabsf(f3dot(nm, na));
-              if (cosc < 0.866f) cn = make_float4(nm.x, nm.y, nm.z, 0.f);
-              else cn = make_float4(na.x, na.y, na.z, 0.f);
-            }
-            cv.x = cv.x + fabsf((cn.x * g.x + cn.y * g.y + cn.z * g.z) * P->vol_def);
-          }
-        }
+  int b = 0, e = 0;  // wall range of cell k, fetched one cell ahead
+  if (in_grid(P, ci.x - 1, ci.y - 1, ci.z - 1)) {
+    const int c = lin_key(P, ci.x - 1, ci.y - 1, ci.z - 1);
+    b = cbeg[c];
+    e = cend[c];
+  }
+#pragma unroll 1
+  for (int k = 0; k < 27; k++) {
+    const int cz = k / 9 - 1, cy = (k / 3) % 3 - 1, cx = k % 3 - 1;
+    int bn = 0, en = 0;
+    if (k < 26) {
+      const int kn = k + 1;
+      const int nx = ci.x + kn % 3 - 1, ny = ci.y + (kn / 3) % 3 - 1, nz = ci.z + kn / 9 - 1;
+      if (in_grid(P, nx, ny, nz)) {
+        const int c = lin_key(P, nx, ny, nz);
+        bn = cbeg[c];
+        en = cend[c];
       }
+    }
+    for (int j0 = b; j0 < e; j0 += 32) {
+      unsigned mask = hit_mask(A.pos, pi, j0, min(e - j0, 32), cx, cy, cz, two_h, r2cut);
+      while (mask) {
+        const int j = j0 + __ffs(mask) - 1;
+        mask &= mask - 1u;
+        // all four loads of the hit are issued together, ahead of the branches
+        const int cj = A.codes[j].x;
+        const float4 pj = A.pos[j];
+        const float4 cnj = A.cn[j];
+        const float4 vj = A.vel[j];
+        if (cj < 10) continue;
+        float3 d;
+        float r = sqrtf(pair_r2(pi, pj, cx, cy, cz, two_h, d));
+        float q = r * i_h;
+        float dwdr = kG(P->kgrad, q);
+        float3 g = f3mul(d, dwdr);
+        if (r < dij && r > 0.f) {
+          float3 na = ld3(cnj);
+          float3 nm = f3div(d, r);
+          dij = r;
+          cv.y = fabsf(f3dot(d, na));
+          cv.w = float(cj);
+          cw = make_float4(vj.x, vj.y, vj.z, 0.f);
+          float cosc = fabsf(f3dot(nm, na));
+          if (cosc < 0.866f) cn = make_float4(nm.x, nm.y, nm.z, 0.f);
+          else cn = make_float4(na.x, na.y, na.z, 0.f);
+        }
+        cv.x = cv.x + fabsf((cn.x * g.x + cn.y * g.y + cn.z * g.z) * P->vol_def);
+      }
+    }
+    b = bn;
+    e = en;
+  }
   A.cvar[i] = cv;
   A.cn[i] = cn;
   A.cw[i] = cw;
@@ -848,7 +896,7 @@ void launch_rigid_body_update(hipStream_t s, DevParams* P) {
 }
 void launch_adami(hipStream_t s, const DevParams* P, int npb, Arrays A, const int* cb, const int* ce) {
   // 64-thread blocks: ~100k wall particles spread evenly over the CUs
-  if (npb > 0) hipLaunchKernelGGL(k_adami, dim3(blocks(npb, 64)), dim3(64), 0, s, P, A, cb, ce, 1);
+  if (npb > 0) hipLaunchKernelGGL(k_adami, dim3(blocks(2 * npb, 64)), dim3(64), 0, s, P, A, cb, ce, 1);
 }
 void launch_corner(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const int* cb, const int* ce,
                    const int* wall_near) {
