@@ -436,7 +436,7 @@ static int create_impl(const lq_params* cases, int n_cases, int device, lq_ctx**
   st |= dalloc(c, &B.acestn_a, n); st |= dalloc(c, &B.acestn_b, n);
   st |= dalloc(c, &B.s3, n); st |= dalloc(c, &B.fdef, n); st |= dalloc(c, &B.deferred, n);
   st |= dalloc(c, &B.str_tmp_a, n); st |= dalloc(c, &B.str_tmp_b, n);
-  st |= dalloc(c, &B.sr_a, n); st |= dalloc(c, &B.sr_b, n);
+  st |= dalloc(c, &B.rec, 4 * n);
   if (par.np < (1 << 25)) {  // pair-list entries hold the slot in 25 bits
     st |= dalloc(c, &B.plist, size_t(kPairListCap) * size_t(std::max(1, par.npm)));
     st |= dalloc(c, &B.pcount, size_t(std::max(1, par.npm)));

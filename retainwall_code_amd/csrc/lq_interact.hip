@@ -46,7 +46,7 @@ __device__ __forceinline__ void tile_home(DevParams* __restrict__ P, const Array
     const float4 vri = A.vel[i];
     const float3 vi = ld3(vri);
     const float rhoi = vri.w;
-    const float4 sria = B.sr_a[i], srib = B.sr_b[i];
+    const float4 sria = rec_sra(B, i), srib = rec_srb(B, i);
     const Sym sri = {sria.x, sria.y, sria.z, sria.w, srib.x, srib.y};
     const float tri = srib.w;
     const float artpi = tri > 0.f ? tri * P->art_p : 0.f;
@@ -125,8 +125,8 @@ __device__ __forceinline__ void tile_home(DevParams* __restrict__ P, const Array
           float3 d;
           const float r = sqrtf(pair_r2(pi, pj, cx, cy, cz, two_h, d));
           const float q = r * i_h;
-          pair_terms(K, S, d, r, q, vi, rhoi, sri, artpi, shift, bsel == 0, A.vel[slot], pj.w, B.sr_a[slot],
-                     B.sr_b[slot]);
+          pair_terms(K, S, d, r, q, vi, rhoi, sri, artpi, shift, bsel == 0, A.vel[slot], pj.w, rec_sra(B, slot),
+                     rec_srb(B, slot));
         }
       }
       const bool more = (j < jend) || (cstep < 54);

@@ -116,8 +116,10 @@ struct Buffers {
   int* deferred;     // compact list of deferred soil slots
   float4* str_tmp_a; // Shepard output scratch
   float2* str_tmp_b;
-  float4* sr_a;      // per-slot sigma/rho^2 (xx yy zz xy), written by k_pair_prep
-  float4* sr_b;      // per-slot sigma/rho^2 (xz yz), m/rho, trace
+  // Per-slot neighbour record written by k_pair_prep, 64 bytes so a pair's
+  // four loads share one cache line: pos, vel, then sigma/rho^2 as
+  // (xx yy zz xy) and (xz yz, m/rho, trace).
+  float4* rec;
   float* sur_x;      // [npm][16] surrogate inputs (normalised), per soil slot
   float* sur_y;      // [members][npm][8] surrogate outputs (normalised)
   unsigned short* sur_w;  // packed bf16 weights per member (MFMA operand layout)
@@ -131,6 +133,10 @@ struct Buffers {
   int* hist;         // radix histogram [256][nblocks]
   unsigned long long* wf_acc;  // per case: exact wall-force sum, 128-bit fixed point (lo, hi)
 };
+
+// sigma/rho^2 parts of slot j's neighbour record
+__device__ __forceinline__ float4 rec_sra(const Buffers& B, int j) { return B.rec[4 * size_t(j) + 2]; }
+__device__ __forceinline__ float4 rec_srb(const Buffers& B, int j) { return B.rec[4 * size_t(j) + 3]; }
 
 // ------------------------------------------------------------- helpers ----
 __device__ __forceinline__ float kW(float knorm, float q) {  // functions.h:142

@@ -74,7 +74,7 @@ __global__ void __launch_bounds__(256) k_interaction_plist(DevParams* __restrict
   const float4 vri = A.vel[i];
   const float3 vi = ld3(vri);
   const float rhoi = vri.w;
-  const float4 sria = B.sr_a[i], srib = B.sr_b[i];
+  const float4 sria = rec_sra(B, i), srib = rec_srb(B, i);
   const Sym sri = {sria.x, sria.y, sria.z, sria.w, srib.x, srib.y};
   const float tri = srib.w;
   const float artpi = tri > 0.f ? tri * P->art_p : 0.f;
@@ -101,7 +101,7 @@ __global__ void __launch_bounds__(256) k_interaction_plist(DevParams* __restrict
     if (r2 <= r2cut) {
       const float r = sqrtf(r2);
       const float q = r * i_h;
-      pair_terms(K, S, d, r, q, vi, rhoi, sri, artpi, shift, bound, A.vel[j], pj.w, B.sr_a[j], B.sr_b[j]);
+      pair_terms(K, S, d, r, q, vi, rhoi, sri, artpi, shift, bound, A.vel[j], pj.w, rec_sra(B, j), rec_srb(B, j));
     }
   }
   interaction_epilogue(P, A, B, i, S, vi);

@@ -370,15 +370,20 @@ __global__ void k_copy_soil_str(const DevParams* __restrict__ P, Arrays A, const
 // (particle_interaction.cu:52-60): sigma/rho^2, its trace and m/rho, with m
 // the Adami volume mass vol_def*rho for wall slots and the particle mass for
 // soil.  Computed once per slot here instead of once per pair; the values
-// are rounded exactly as the inline expressions.
+// are rounded exactly as the inline expressions.  They go into the slot's
+// 64-byte neighbour record next to its position and velocity, so the four
+// loads of a pair read one cache line.
 __global__ void __launch_bounds__(256) k_pair_prep(const DevParams* __restrict__ P, Arrays A, Buffers B, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float rho = A.vel[i].w;
   const float m = i < P->npb ? P->vol_def * rho : fabsf(A.pos[i].w);
   const Sym sr = sdiv(ldsym(A.str_a, A.str_b, i), rho * rho);
-  B.sr_a[i] = make_float4(sr.xx, sr.yy, sr.zz, sr.xy);
-  B.sr_b[i] = make_float4(sr.xz, sr.yz, m / rho, strace(sr));
+  float4* rc = B.rec + 4 * size_t(i);
+  rc[0] = A.pos[i];
+  rc[1] = A.vel[i];
+  rc[2] = make_float4(sr.xx, sr.yy, sr.zz, sr.xy);
+  rc[3] = make_float4(sr.xz, sr.yz, m / rho, strace(sr));
 }
 
 // Reference-order variant: each thread walks its 27 cells (cz, cy, cx;
@@ -397,7 +402,7 @@ __global__ void __launch_bounds__(256) k_interaction_ref(DevParams* __restrict__
   const float4 vri = A.vel[i];
   const float3 vi = ld3(vri);
   const float rhoi = vri.w;
-  const float4 sria = B.sr_a[i], srib = B.sr_b[i];
+  const float4 sria = rec_sra(B, i), srib = rec_srb(B, i);
   const Sym sri = {sria.x, sria.y, sria.z, sria.w, srib.x, srib.y};
   const float tri = srib.w;
   const float artpi = tri > 0.f ? tri * P->art_p : 0.f;
@@ -425,7 +430,7 @@ __global__ void __launch_bounds__(256) k_interaction_ref(DevParams* __restrict__
             if (!(r2 <= r2cut)) continue;
             float r = sqrtf(r2);
             float q = r * i_h;
-            pair_terms(K, S, d, r, q, vi, rhoi, sri, artpi, shift, bound, A.vel[j], pj.w, B.sr_a[j], B.sr_b[j]);
+            pair_terms(K, S, d, r, q, vi, rhoi, sri, artpi, shift, bound, A.vel[j], pj.w, rec_sra(B, j), rec_srb(B, j));
           }
         }
       }
@@ -457,8 +462,8 @@ struct NbrRec {
 };
 
 __device__ __forceinline__ NbrRec nl_load(const Arrays& A, const Buffers& B, int e) {
-  const int j = int(unsigned(e) >> 7);
-  return {A.pos[j], A.vel[j], B.sr_a[j], B.sr_b[j]};
+  const float4* r = B.rec + 4 * size_t(unsigned(e) >> 7);
+  return {r[0], r[1], r[2], r[3]};
 }
 
 __device__ __forceinline__ void nl_eval(const PairConst& K, IAcc& S, int e, const NbrRec& n, float3 pi, float two_h,
@@ -502,7 +507,7 @@ __global__ void __launch_bounds__(NL_WAVE * NL_WAVES, 5) k_interaction_nl(DevPar
   const float4 vri = A.vel[i];
   const float3 vi = ld3(vri);
   const float rhoi = vri.w;
-  const float4 sria = B.sr_a[i], srib = B.sr_b[i];
+  const float4 sria = rec_sra(B, i), srib = rec_srb(B, i);
   const Sym sri = {sria.x, sria.y, sria.z, sria.w, srib.x, srib.y};
   const float tri = srib.w;
   const float artpi = tri > 0.f ? tri * P->art_p : 0.f;
