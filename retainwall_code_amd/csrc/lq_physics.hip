@@ -450,7 +450,7 @@ __global__ void __launch_bounds__(256) k_interaction_ref(DevParams* __restrict__
 // An entry packs the slot j (bits 7..31), the cell offset (bits 0..5) and
 // the range (bit 6: 0 wall, 1 soil).
 constexpr int NL_WAVE = 64;
-constexpr int NL_WAVES = 2;
+constexpr int NL_WAVES = 1;
 constexpr int NL_CAP = 32;    // ring entries per lane (power of two)
 constexpr int NL_CHUNK = 32;  // candidates per hit mask
 static_assert((NL_CAP & (NL_CAP - 1)) == 0 && NL_CAP >= NL_CHUNK, "ring must hold a full chunk of hits");
