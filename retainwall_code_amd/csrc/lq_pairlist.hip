@@ -94,14 +94,15 @@ __global__ void __launch_bounds__(256) k_interaction_plist(DevParams* __restrict
     const int j = int(unsigned(e) >> 7);
     const int ox = (e & 3) - 1, oy = ((e >> 2) & 3) - 1, oz = ((e >> 4) & 3) - 1;
     const bool bound = (e & 64) == 0;
-    const float4 pj = A.pos[j];
+    const float4* rj = B.rec + 4 * size_t(j);  // the neighbour's 64-byte record
+    const float4 pj = rj[0];
     if (k + 1 < n) e = lst[size_t(k + 1) * stride];  // next entry in flight during this pair
     float3 d;
     const float r2 = pair_r2(pi, pj, ox, oy, oz, two_h, d);
     if (r2 <= r2cut) {
       const float r = sqrtf(r2);
       const float q = r * i_h;
-      pair_terms(K, S, d, r, q, vi, rhoi, sri, artpi, shift, bound, A.vel[j], pj.w, rec_sra(B, j), rec_srb(B, j));
+      pair_terms(K, S, d, r, q, vi, rhoi, sri, artpi, shift, bound, rj[1], pj.w, rj[2], rj[3]);
     }
   }
   interaction_epilogue(P, A, B, i, S, vi);
