@@ -150,7 +150,11 @@ def bench_surrogate(args):
     case = cases.config(args.case)
     plain = sim.open_case(case)
     sur = sim.open_case(case)
-    surrogate.start_log(sur, every=16, capacity=2_000_000)
+    # every 48th slot per call: 2 calls x warm-up steps x npm / 48 rows, all
+    # of them kept (no overflow), so the training set is reproducible
+    every = max(1, -(-2 * args.warmup * case.npm // 2_000_000))
+    every = max(every, 48)
+    surrogate.start_log(sur, every=every, capacity=2_000_000)
     plain.step(args.warmup)
     sur.step(args.warmup)
     sur.sync()

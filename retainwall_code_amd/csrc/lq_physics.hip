@@ -113,95 +113,94 @@ __global__ void k_rigid_body_update(DevParams* P) {
 
 // ------------------------------------------------------------- Adami ----
 // cuAdamiBoundaryMaterial + loops, adami_boundary.cu:26-158.
-// Two thread roles in one launch, each walking the 27 cells in reference
-// order: threads [0, npb) sum the soil terms of wall slot t (the Adami
-// extrapolation), threads [npb, 2 npb) the wall-normal integral of
-// contact-wall slot t - npb.  The two sums have disjoint inputs and outputs,
-// so splitting them changes no result while halving the chain of dependent
-// loads of a contact-wall thread; the next cell's range is fetched while
-// the current one is scanned.  The kernel is latency-bound (one thread per
-// wall slot fills about one wave per SIMD).
+// Each thread walks the 27 cells in reference order.  The soil extrapolation
+// sums and, for contact walls, the wall-normal integral share no inputs or
+// outputs.  With split set (few wall slots: about one wave per SIMD, so each
+// wave's chain of dependent loads sets the time) they run on two threads,
+// [0, npb) the soil terms of slot t and [npb, 2 npb) the normal of slot
+// t - npb, halving a contact-wall thread's chain; with many wall slots one
+// thread does both and each cell's ranges are read once.  Results are the
+// same either way.  The next cell's ranges are fetched while the current
+// one is scanned.
 __global__ void __launch_bounds__(64) k_adami(const DevParams* __restrict__ P, Arrays A,
-                                               const int* __restrict__ cbeg, const int* __restrict__ cend, int remap) {
+                                               const int* __restrict__ cbeg, const int* __restrict__ cend, int remap,
+                                               int split) {
   const int t = xcd_block(remap) * blockDim.x + threadIdx.x;
   const int npb = P->npb;
-  if (t >= 2 * npb) return;
-  const bool normal_role = t >= npb;
-  const int i = normal_role ? t - npb : t;
+  if (t >= (split ? 2 * npb : npb)) return;
+  const int i = t >= npb ? t - npb : t;
   const bool contact = A.codes[i].x >= 10;
-  if (normal_role && !contact) return;
+  const bool do_soil = !split || t < npb;
+  const bool do_normal = contact && (!split || t >= npb);
+  if (!do_soil && !do_normal) return;
   const float3 pi = ld3(A.pos[i]);
   const int4 ci = A.cellpos[i];
   const float two_h = P->two_h, i_h = P->i_h, r2cut = P->r2cut;
-  const int roff = normal_role ? 0 : P->ncell_material;  // wall ranges for the normal, soil ranges otherwise
+  const int ncm = P->ncell_material;
   float3 n = make_float3(0.f, 0.f, 0.f);
   Sym rs = {0, 0, 0, 0, 0, 0};
   float3 rpos = make_float3(0.f, 0.f, 0.f);
   float rker = 0.f;
   float4 rv = make_float4(0.f, 0.f, 0.f, 0.f);
-  int b = 0, e = 0;  // range of cell k, fetched one cell ahead
+  int bw = 0, ew = 0, bs = 0, es = 0;  // wall and soil ranges of cell k, fetched one cell ahead
   if (in_grid(P, ci.x - 1, ci.y - 1, ci.z - 1)) {
-    const int c = lin_key(P, ci.x - 1, ci.y - 1, ci.z - 1) + roff;
-    b = cbeg[c];
-    e = cend[c];
+    const int c = lin_key(P, ci.x - 1, ci.y - 1, ci.z - 1);
+    if (do_normal) { bw = cbeg[c]; ew = cend[c]; }
+    if (do_soil) { bs = cbeg[c + ncm]; es = cend[c + ncm]; }
   }
 #pragma unroll 1
   for (int k = 0; k < 27; k++) {
     const int cz = k / 9 - 1, cy = (k / 3) % 3 - 1, cx = k % 3 - 1;
-    int bn = 0, en = 0;
+    int bwn = 0, ewn = 0, bsn = 0, esn = 0;
     if (k < 26) {
       const int kn = k + 1;
       const int nx = ci.x + kn % 3 - 1, ny = ci.y + (kn / 3) % 3 - 1, nz = ci.z + kn / 9 - 1;
       if (in_grid(P, nx, ny, nz)) {
-        const int c = lin_key(P, nx, ny, nz) + roff;
-        bn = cbeg[c];
-        en = cend[c];
+        const int c = lin_key(P, nx, ny, nz);
+        if (do_normal) { bwn = cbeg[c]; ewn = cend[c]; }
+        if (do_soil) { bsn = cbeg[c + ncm]; esn = cend[c + ncm]; }
       }
     }
-    if (normal_role) {
-      for (int j0 = b; j0 < e; j0 += 32) {
-        unsigned mask = hit_mask(A.pos, pi, j0, min(e - j0, 32), cx, cy, cz, two_h, r2cut);
-        while (mask) {
-          const int j = j0 + __ffs(mask) - 1;
-          mask &= mask - 1u;
-          float3 d;
-          float r = sqrtf(pair_r2(pi, A.pos[j], cx, cy, cz, two_h, d));
-          float q = r * i_h;
-          float dwdr = kG(P->kgrad, q);
-          float3 g = f3mul(d, dwdr);
-          n = f3sub(n, f3mul(g, P->vol_def));
-        }
-      }
-    } else {
-      for (int j0 = b; j0 < e; j0 += 32) {
-        unsigned mask = hit_mask(A.pos, pi, j0, min(e - j0, 32), cx, cy, cz, two_h, r2cut);
-        while (mask) {
-          const int j = j0 + __ffs(mask) - 1;
-          mask &= mask - 1u;
-          float3 d;
-          float r = sqrtf(pair_r2(pi, A.pos[j], cx, cy, cz, two_h, d));
-          float q = r * i_h;
-          float ker = kW(P->knorm, q);
-          float4 idmj = A.idm[j];
-          float4 vj = A.vel[j];
-          float volj = idmj.y / vj.w;
-          float wv = ker * volj;
-          Sym sj = ldsym(A.str_a, A.str_b, j);
-          rs = sadd(rs, smul(sj, wv));
-          rpos = f3add(rpos, f3mul(d, ker * idmj.y));
-          rker = rker + wv;
-          rv = make_float4(rv.x + vj.x * wv, rv.y + vj.y * wv, rv.z + vj.z * wv, rv.w + vj.w * wv);
-        }
+    for (int j0 = bw; j0 < ew; j0 += 32) {  // normal integral over wall neighbours
+      unsigned mask = hit_mask(A.pos, pi, j0, min(ew - j0, 32), cx, cy, cz, two_h, r2cut);
+      while (mask) {
+        const int j = j0 + __ffs(mask) - 1;
+        mask &= mask - 1u;
+        float3 d;
+        float r = sqrtf(pair_r2(pi, A.pos[j], cx, cy, cz, two_h, d));
+        float q = r * i_h;
+        float dwdr = kG(P->kgrad, q);
+        float3 g = f3mul(d, dwdr);
+        n = f3sub(n, f3mul(g, P->vol_def));
       }
     }
-    b = bn;
-    e = en;
+    for (int j0 = bs; j0 < es; j0 += 32) {  // extrapolation from soil neighbours
+      unsigned mask = hit_mask(A.pos, pi, j0, min(es - j0, 32), cx, cy, cz, two_h, r2cut);
+      while (mask) {
+        const int j = j0 + __ffs(mask) - 1;
+        mask &= mask - 1u;
+        float3 d;
+        float r = sqrtf(pair_r2(pi, A.pos[j], cx, cy, cz, two_h, d));
+        float q = r * i_h;
+        float ker = kW(P->knorm, q);
+        float4 idmj = A.idm[j];
+        float4 vj = A.vel[j];
+        float volj = idmj.y / vj.w;
+        float wv = ker * volj;
+        Sym sj = ldsym(A.str_a, A.str_b, j);
+        rs = sadd(rs, smul(sj, wv));
+        rpos = f3add(rpos, f3mul(d, ker * idmj.y));
+        rker = rker + wv;
+        rv = make_float4(rv.x + vj.x * wv, rv.y + vj.y * wv, rv.z + vj.z * wv, rv.w + vj.w * wv);
+      }
+    }
+    bw = bwn; ew = ewn; bs = bsn; es = esn;
   }
-  if (normal_role) {
+  if (do_normal) {
     float nn = f3norm(n);
     if (nn > 0.f) A.cn[i] = make_float4(n.x / nn, n.y / nn, n.z / nn, 0.f);
-    return;
   }
+  if (!do_soil) return;
   if (rker > 0.f) {
     float4 nrm = A.normal[i];
     float fc = nrm.w;
@@ -900,8 +899,12 @@ void launch_rigid_body_update(hipStream_t s, DevParams* P) {
   hipLaunchKernelGGL(k_rigid_body_update, dim3(4), dim3(64), 0, s, P);
 }
 void launch_adami(hipStream_t s, const DevParams* P, int npb, Arrays A, const int* cb, const int* ce) {
-  // 64-thread blocks: ~100k wall particles spread evenly over the CUs
-  if (npb > 0) hipLaunchKernelGGL(k_adami, dim3(blocks(2 * npb, 64)), dim3(64), 0, s, P, A, cb, ce, 1);
+  // 64-thread blocks spread the wall slots evenly over the CUs; below 16
+  // waves per CU (256 CUs) the two sums go to separate threads
+  const int split = npb < 256 * 64 * 16 ? 1 : 0;
+  if (npb > 0)
+    hipLaunchKernelGGL(k_adami, dim3(blocks((split ? 2 : 1) * (long long)npb, 64)), dim3(64), 0, s, P, A, cb, ce, 1,
+                       split);
 }
 void launch_corner(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const int* cb, const int* ce,
                    const int* wall_near) {

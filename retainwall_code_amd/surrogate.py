@@ -106,10 +106,17 @@ def start_log(ctx, every=8, capacity=1_000_000):
 
 
 def read_log(ctx, max_rows=None):
+    """Logged (x, y) rows, in lexicographic order of (x, y).
+
+    Waves reserve log rows with an atomic counter, so the device order of the
+    rows depends on scheduling; sorting makes the returned data set (and what
+    is trained on it) reproducible whenever the log did not overflow."""
     cap = max_rows or 1_000_000
     x = np.zeros((cap, NIN), np.float32)
     y = np.zeros((cap, NOUT), np.float32)
     n = C.c_int()
     ctx._chk(ctx.lib.lq_surrogate_log_read(ctx.h, x.ctypes.data, y.ctypes.data, cap, C.byref(n)),
              "lq_surrogate_log_read")
-    return x[:n.value].copy(), y[:n.value].copy()
+    x, y = x[:n.value], y[:n.value]
+    order = np.lexsort(np.concatenate([x, y], axis=1).T[::-1])
+    return x[order].copy(), y[order].copy()
