@@ -237,8 +237,10 @@ int lq_predictor(lq_ctx* ctx);
 int lq_corrector(lq_ctx* ctx);
 
 /* One or more whole time steps (loquat.cu:548-561 without Saving): the
- * predictor and corrector sections in order.  use_graph != 0 captures one
- * step in a hipGraph and replays it.  No host synchronisation inside. */
+ * predictor and corrector sections in order.  use_graph != 0 captures two
+ * steps in a hipGraph once (again after a layout, variant or timing change)
+ * and replays it; an odd step is enqueued directly.  No host
+ * synchronisation inside. */
 int lq_step(lq_ctx* ctx, int nsteps, int use_graph);
 
 /* Wall earth-pressure probe, utilities.cu:578-670: F = 0.001 * sum fx [kN]
@@ -263,13 +265,19 @@ int lq_synchronize(lq_ctx* ctx);
 /* ---- instrumentation (no reference counterpart) --------------------------
  * When enabled, lq_step and the piecewise calls record HIP events around
  * each kernel class on the context stream; lq_kernel_times returns the
- * accumulated milliseconds and launch counts per class. */
+ * accumulated milliseconds and launch counts per class.  Graph replays are
+ * timed too: the captured graph holds the events as event-record nodes,
+ * pointed at fresh events before every replay. */
 enum lq_kernel_class {
   LQ_K_NEIGHBOR_SEARCH = 0, LQ_K_SORT_ARRAYS = 1, LQ_K_ADAMI = 2, LQ_K_CORNER = 3,
   LQ_K_INTERACTION = 4, LQ_K_CONTACT = 5, LQ_K_PREDICTOR = 6, LQ_K_CORRECTOR = 7,
   LQ_K_SHEPARD = 8, LQ_K_PAIR_LIST_BUILD = 9, LQ_K_CLASS_COUNT = 10
 };
 int lq_enable_timing(lq_ctx* ctx, int enable);
+/* Times only the classes whose bit (1 << lq_kernel_class) is set in mask
+ * (0 turns timing off) and resets the counters; fewer timed sections keep
+ * more of the graph replay's advantage over per-kernel launches. */
+int lq_enable_timing_classes(lq_ctx* ctx, unsigned mask);
 /* STREAM-style copy of `bytes` between two device scratch buffers, `reps`
  * launches, average milliseconds per launch (HBM bandwidth probe and the
  * known-size copy used to calibrate hardware byte counters). */

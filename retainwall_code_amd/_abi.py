@@ -124,7 +124,7 @@ EXPORTS = [
     "lq_sort_arrays_primary", "lq_sort_arrays_all", "lq_rigid_body_update", "lq_adami_boundary",
     "lq_find_corner_particle", "lq_stress_regularization", "lq_particle_interaction",
     "lq_predictor", "lq_corrector", "lq_step", "lq_wall_force", "lq_positions_absolute",
-    "lq_get_counts", "lq_synchronize", "lq_stress_saving_regularization", "lq_stream_copy", "lq_enable_timing", "lq_kernel_times", "lq_neighbor_lists", "lq_set_tile", "lq_set_interaction", "lq_set_pair_list", "lq_pair_list_status", "lq_autotune", "lq_interaction_cutoff_sq",
+    "lq_get_counts", "lq_synchronize", "lq_stress_saving_regularization", "lq_stream_copy", "lq_enable_timing", "lq_enable_timing_classes", "lq_kernel_times", "lq_neighbor_lists", "lq_set_tile", "lq_set_interaction", "lq_set_pair_list", "lq_pair_list_status", "lq_autotune", "lq_interaction_cutoff_sq",
     "lq_create_batch", "lq_case_count", "lq_upload_case", "lq_set_case_motion_tables", "lq_get_case_field",
     "lq_case_positions_absolute", "lq_case_counts", "lq_wall_force_cases",
     "lq_mlp_normalisation", "lq_mlp_train", "lq_mlp_loss_grad", "lq_mlp_forward", "lq_mlp_last_error",
@@ -162,6 +162,7 @@ def _proto(lib):
     lib.lq_stress_saving_regularization.argtypes = [vp, vp, C.c_size_t]
     lib.lq_get_counts.argtypes = [vp, P(i32), P(i32), P(i32)]
     lib.lq_enable_timing.argtypes = [vp, C.c_int]
+    lib.lq_enable_timing_classes.argtypes = [vp, C.c_uint]
     lib.lq_stream_copy.argtypes = [vp, C.c_size_t, C.c_int, P(C.c_double)]
     lib.lq_kernel_times.argtypes = [vp, P(C.c_double), P(C.c_int64), C.c_int]
     lib.lq_neighbor_lists.argtypes = [vp, C.c_int, vp, vp, C.c_size_t]

@@ -37,6 +37,7 @@ struct lq_ctx {
   long long host_step = 0;
   // instrumentation
   bool timing = false;
+  unsigned timing_mask = ~0u;  // kernel classes timed while timing is on
   std::vector<TimedPair> pending;
   std::vector<hipEvent_t> pool;
   double ms[LQ_K_CLASS_COUNT] = {0};
@@ -44,6 +45,18 @@ struct lq_ctx {
   // graph
   hipGraphExec_t graph = nullptr;
   long long layout_version = 0, graph_version = -1;
+  // A graph captured with timing on keeps its template (node handles) and,
+  // per timed section, the two event-record nodes; each replay points them
+  // at fresh events so every launch in a run of replays is timed.
+  bool graph_timed = false;
+  unsigned graph_mask = ~0u;
+  bool capturing = false;  // section events become event-record nodes
+  hipGraph_t graph_tmpl = nullptr;
+  struct TimedNodes {
+    int cls;
+    hipGraphNode_t a, b;
+  };
+  std::vector<TimedNodes> graph_sections;
   // cases (a plain context is a batch of one)
   int n_cases = 1;
   std::vector<lq_params> case_par;  // each case's own parameters
@@ -110,20 +123,27 @@ static hipEvent_t ev_get(lq_ctx* c) {
   hipEventCreate(&e);
   return e;
 }
+// During stream capture an event record is kept in the graph as a node only
+// with the external flag (a plain record just orders the capture).
+static void ev_record(lq_ctx* c, hipEvent_t e) {
+  if (c->capturing) hipEventRecordWithFlags(e, c->stream, hipEventRecordExternal);
+  else hipEventRecord(e, c->stream);
+}
 struct Section {
   lq_ctx* c;
   int cls;
   hipEvent_t a = nullptr;
   Section(lq_ctx* c_, int cls_) : c(c_), cls(cls_) {
-    if (c->timing) {
+    if (on()) {
       a = ev_get(c);
-      hipEventRecord(a, c->stream);
+      ev_record(c, a);
     }
   }
+  bool on() const { return c->timing && ((c->timing_mask >> cls) & 1u); }
   ~Section() {
-    if (c->timing) {
+    if (on()) {
       hipEvent_t b = ev_get(c);
-      hipEventRecord(b, c->stream);
+      ev_record(c, b);
       c->pending.push_back({cls, a, b});
     }
   }
@@ -503,6 +523,7 @@ extern "C" void lq_destroy(lq_ctx* c) {
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
   if (c->graph) hipGraphExecDestroy(c->graph);
+  if (c->graph_tmpl) hipGraphDestroy(c->graph_tmpl);
   for (auto& p : c->pending) { c->pool.push_back(p.a); c->pool.push_back(p.b); }
   for (auto e : c->pool) hipEventDestroy(e);
   for (void* p : c->allocs) hipFree(p);
@@ -1112,26 +1133,78 @@ static int enq_step(lq_ctx* c) {
   return 0;
 }
 
+// Captures two steps (the cur/alt buffer swap returns to its start).  With
+// timing on, the timed sections' events become event-record nodes; they are
+// matched to their sections here and re-pointed at fresh events per replay.
+static int capture_graph(lq_ctx* c) {
+  if (c->graph) { hipGraphExecDestroy(c->graph); c->graph = nullptr; }
+  if (c->graph_tmpl) { hipGraphDestroy(c->graph_tmpl); c->graph_tmpl = nullptr; }
+  c->graph_sections.clear();
+  const size_t first = c->pending.size();
+  hipGraph_t g;
+  HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  c->capturing = true;
+  enq_step(c);
+  enq_step(c);
+  c->capturing = false;
+  c->host_step -= 2;
+  HIPCHECK(c, hipStreamEndCapture(c->stream, &g));
+  HIPCHECK(c, hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0));
+  c->graph_version = c->layout_version;
+  c->graph_timed = c->timing;
+  c->graph_mask = c->timing_mask;
+  if (!c->timing) {
+    hipGraphDestroy(g);
+    return 0;
+  }
+  c->graph_tmpl = g;
+  size_t nn = 0;
+  HIPCHECK(c, hipGraphGetNodes(g, nullptr, &nn));
+  std::vector<hipGraphNode_t> nodes(nn);
+  HIPCHECK(c, hipGraphGetNodes(g, nodes.data(), &nn));
+  auto node_of = [&](hipEvent_t e) -> hipGraphNode_t {
+    for (auto n : nodes) {
+      hipGraphNodeType t;
+      hipEvent_t ev;
+      if (hipGraphNodeGetType(n, &t) == hipSuccess && t == hipGraphNodeTypeEventRecord &&
+          hipGraphEventRecordNodeGetEvent(n, &ev) == hipSuccess && ev == e)
+        return n;
+    }
+    return nullptr;
+  };
+  for (size_t k = first; k < c->pending.size(); k++) {
+    const TimedPair p = c->pending[k];
+    const hipGraphNode_t na = node_of(p.a), nb = node_of(p.b);
+    if (!na || !nb) return fail(c, LQ_ERR_HIP, "lq_step: timed section missing from the captured graph");
+    c->graph_sections.push_back({p.cls, na, nb});
+    c->pool.push_back(p.a);
+    c->pool.push_back(p.b);
+  }
+  c->pending.resize(first);
+  return 0;
+}
+
 extern "C" int lq_step(lq_ctx* c, int nsteps, int use_graph) {
   NEED_UPLOAD(c);
   if (nsteps < 0) return fail(c, LQ_ERR_ARG, "lq_step: nsteps < 0");
   hipSetDevice(c->device);
-  const bool graph_ok = use_graph && !c->timing && c->par.stress_shepard_filter == 0;
+  const bool graph_ok = use_graph && c->par.stress_shepard_filter == 0;
   int done = 0;
   if (graph_ok && nsteps >= 2) {
-    if (c->graph == nullptr || c->graph_version != c->layout_version) {
-      if (c->graph) { hipGraphExecDestroy(c->graph); c->graph = nullptr; }
-      hipGraph_t g;
-      HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      enq_step(c);
-      enq_step(c);  // two steps: the cur/alt buffer swap returns to its start
-      c->host_step -= 2;
-      HIPCHECK(c, hipStreamEndCapture(c->stream, &g));
-      HIPCHECK(c, hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0));
-      hipGraphDestroy(g);
-      c->graph_version = c->layout_version;
+    if (c->graph == nullptr || c->graph_version != c->layout_version || c->graph_timed != c->timing ||
+        (c->timing && c->graph_mask != c->timing_mask)) {
+      const int st = capture_graph(c);
+      if (st) return st;
     }
     while (nsteps - done >= 2) {
+      if (c->graph_timed) {
+        for (const auto& t : c->graph_sections) {
+          const hipEvent_t a = ev_get(c), b = ev_get(c);
+          HIPCHECK(c, hipGraphExecEventRecordNodeSetEvent(c->graph, t.a, a));
+          HIPCHECK(c, hipGraphExecEventRecordNodeSetEvent(c->graph, t.b, b));
+          c->pending.push_back({t.cls, a, b});
+        }
+      }
       HIPCHECK(c, hipGraphLaunch(c->graph, c->stream));
       c->host_step += 2;
       done += 2;
@@ -1258,8 +1331,17 @@ extern "C" int lq_enable_timing(lq_ctx* c, int enable) {
   if (!c) return LQ_ERR_ARG;
   resolve_timing(c);
   c->timing = enable != 0;
+  c->timing_mask = ~0u;
   for (int k = 0; k < LQ_K_CLASS_COUNT; k++) { c->ms[k] = 0; c->launches[k] = 0; }
   return LQ_OK;
+}
+
+extern "C" int lq_enable_timing_classes(lq_ctx* c, unsigned mask) {
+  if (!c) return LQ_ERR_ARG;
+  if (mask >> LQ_K_CLASS_COUNT) return fail(c, LQ_ERR_ARG, "lq_enable_timing_classes: unknown class bit");
+  const int st = lq_enable_timing(c, mask != 0);
+  c->timing_mask = mask;
+  return st;
 }
 
 extern "C" int lq_kernel_times(lq_ctx* c, double* ms, int64_t* launches, int n) {

@@ -163,6 +163,13 @@ class Context:
     def enable_timing(self, on=True):
         self._chk(self.lib.lq_enable_timing(self.h, 1 if on else 0), "lq_enable_timing")
 
+    def enable_timing_classes(self, names):
+        """Time only the named kernel classes (see _abi.KERNEL_CLASSES); resets the counters."""
+        mask = 0
+        for n in names:
+            mask |= 1 << _abi.KERNEL_CLASSES.index(n)
+        self._chk(self.lib.lq_enable_timing_classes(self.h, mask), "lq_enable_timing_classes")
+
     def stream_copy(self, nbytes, reps=20):
         ms = C.c_double()
         self._chk(self.lib.lq_stream_copy(self.h, nbytes, reps, C.byref(ms)), "lq_stream_copy")

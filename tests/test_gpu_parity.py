@@ -304,6 +304,43 @@ def test_graph_matches_eager():
     b.close()
 
 
+def test_timed_graph_replay():
+    """With timing on, graph replays carry their kernel events as event-record
+    nodes re-pointed per replay: every launch is counted and timed, and the
+    state equals eager stepping with timing on."""
+    case = cases.config("tiny")
+    say("[timed graph] case=tiny steps=2+6")
+    a = sim.open_case(case)
+    b = sim.open_case(case)
+    for c in (a, b):
+        c.enable_timing(True)
+    a.step(2, graph=False)
+    b.step(2, graph=True)  # capture
+    a.sync()
+    b.sync()
+    a.enable_timing(True)
+    b.enable_timing(True)
+    a.step(6, graph=False)
+    b.step(6, graph=True)  # three replays
+    ka, kb = a.kernel_times(), b.kernel_times()
+    say(f"  eager {ka['interaction']}  graph {kb['interaction']}")
+    for k in ["neighbor_search", "sort_arrays", "adami", "corner", "interaction", "contact", "predictor",
+              "corrector"]:
+        assert kb[k][1] == ka[k][1], k  # same launch counts
+        assert kb[k][0] > 0.0, k
+    assert ka["interaction"][1] == 12  # two calls per step
+    for k in ["pos", "vel", "str", "cmag", "cvar", "cellpos"]:
+        np.testing.assert_array_equal(bits(a.get(k)), bits(b.get(k)), err_msg=k)
+    # interaction only: the graph is captured again with fewer event nodes
+    b.enable_timing_classes(["interaction"])
+    b.step(4, graph=True)
+    kb = b.kernel_times()
+    assert kb["interaction"][1] == 8 and kb["interaction"][0] > 0.0
+    assert all(v == (0.0, 0) for k, v in kb.items() if k != "interaction")
+    a.close()
+    b.close()
+
+
 def test_particle_deletion(oracle_mod):
     """A soil particle leaving the cell grid is marked deleted by the corrector
     (predictor_corrector.cu:186-190) and dropped from npm by the next search
