@@ -993,11 +993,10 @@ static int enq_ns(lq_ctx* c) {
   launch_neighbor_search(c->stream, c->dP, c->par.np, c->nbits, c->par.ncell, c->B.cur, c->B, &k, &v);
   return 0;
 }
+// SortArraysPrimary in place: only the slots the sort moved are rewritten.
 static int enq_sort_primary(lq_ctx* c) {
   Section s(c, LQ_K_SORT_ARRAYS);
-  launch_gather(c->stream, c->dP, c->par.np, c->B.cur, c->B.alt, c->B.part_idx, c->B);
-  std::swap(c->B.cur, c->B.alt);
-  c->layout_version++;
+  launch_gather_moved(c->stream, c->dP, c->par.np, c->B.cur, c->B.alt, c->B.part_idx);
   return 0;
 }
 static int enq_sort_all(lq_ctx* c) {

@@ -246,6 +246,63 @@ __global__ void __launch_bounds__(256) k_gather(const DevParams* __restrict__ P,
   dst.stn_b[i] = src.stn_b[ss];
 }
 
+// The same gather applied in place to the slots the sort moved: between two
+// steps few particles change cell, so the sorted order is the identity on
+// most slots.  Phase 0 writes the gathered rows of slots with idx[i] != i
+// into the alternate set (exactly as k_gather would), phase 1 copies them
+// back; slots with idx[i] == i already hold their gathered row.
+__global__ void __launch_bounds__(256) k_gather_moved(const DevParams* __restrict__ P, Arrays cur, Arrays alt,
+                                                      const int* __restrict__ idx, int phase) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P->np) return;
+  const int s = idx[i];
+  if (s == i) return;
+  const bool st = i < P->npb + P->npm;
+  if (phase == 0) {
+    alt.codes[i] = cur.codes[s];
+    alt.cellpos[i] = cur.cellpos[s];
+    alt.orig_id[i] = cur.orig_id[s];
+    alt.case_id[i] = cur.case_id[s];
+    alt.pres[i] = cur.pres[s];
+    alt.pos[i] = cur.pos[s];
+    alt.pos_ori[i] = cur.pos_ori[s];
+    alt.vel[i] = cur.vel[s];
+    alt.idm[i] = cur.idm[s];
+    alt.cvar[i] = cur.cvar[s];
+    alt.cf[i] = cur.cf[s];
+    alt.cn[i] = cur.cn[s];
+    alt.cw[i] = cur.cw[s];
+    if (i < P->npb) alt.normal[i] = cur.normal[s];
+    if (st) {
+      alt.str_a[i] = cur.str_a[s];
+      alt.str_b[i] = cur.str_b[s];
+      alt.stn_a[i] = cur.stn_a[s];
+      alt.stn_b[i] = cur.stn_b[s];
+    }
+  } else {
+    cur.codes[i] = alt.codes[i];
+    cur.cellpos[i] = alt.cellpos[i];
+    cur.orig_id[i] = alt.orig_id[i];
+    cur.case_id[i] = alt.case_id[i];
+    cur.pres[i] = alt.pres[i];
+    cur.pos[i] = alt.pos[i];
+    cur.pos_ori[i] = alt.pos_ori[i];
+    cur.vel[i] = alt.vel[i];
+    cur.idm[i] = alt.idm[i];
+    cur.cvar[i] = alt.cvar[i];
+    cur.cf[i] = alt.cf[i];
+    cur.cn[i] = alt.cn[i];
+    cur.cw[i] = alt.cw[i];
+    if (i < P->npb) cur.normal[i] = alt.normal[i];
+    if (st) {
+      cur.str_a[i] = alt.str_a[i];
+      cur.str_b[i] = alt.str_b[i];
+      cur.stn_a[i] = alt.stn_a[i];
+      cur.stn_b[i] = alt.stn_b[i];
+    }
+  }
+}
+
 // The *_ini arrays of SortArraysAll are permuted in place through scratch.
 __global__ void __launch_bounds__(256) k_gather_ini(const DevParams* __restrict__ P, const int* __restrict__ idx,
                                                     const float4* __restrict__ pi, const float4* __restrict__ vi,
@@ -312,6 +369,12 @@ void launch_neighbor_search(hipStream_t s, DevParams* P, int np, int nbits, int 
 
 void launch_gather(hipStream_t s, const DevParams* P, int np, Arrays src, Arrays dst, const int* idx, const Buffers& B) {
   if (np > 0) hipLaunchKernelGGL(k_gather, dim3(blocks(np, 256)), dim3(256), 0, s, P, src, dst, idx, B, 0);
+}
+
+void launch_gather_moved(hipStream_t s, const DevParams* P, int np, Arrays cur, Arrays alt, const int* idx) {
+  if (np <= 0) return;
+  hipLaunchKernelGGL(k_gather_moved, dim3(blocks(np, 256)), dim3(256), 0, s, P, cur, alt, idx, 0);
+  hipLaunchKernelGGL(k_gather_moved, dim3(blocks(np, 256)), dim3(256), 0, s, P, cur, alt, idx, 1);
 }
 
 void launch_gather_ini(hipStream_t s, const DevParams* P, int np, const int* idx, Buffers& B, float4* t4a,

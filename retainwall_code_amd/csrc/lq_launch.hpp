@@ -63,6 +63,8 @@ void launch_gather_rows(hipStream_t s, const float* src, int ld, const int* idx,
 void launch_neighbor_search(hipStream_t s, DevParams* P, int np, int nbits, int ncell, Arrays A, Buffers& B,
                             const int** sorted_keys, const int** sorted_vals);
 void launch_gather(hipStream_t s, const DevParams* P, int np, Arrays src, Arrays dst, const int* idx, const Buffers& B);
+// The same permutation in place (alternate set as scratch), touching only moved slots.
+void launch_gather_moved(hipStream_t s, const DevParams* P, int np, Arrays cur, Arrays alt, const int* idx);
 void launch_gather_ini(hipStream_t s, const DevParams* P, int np, const int* idx, Buffers& B, float4* t4a,
                        float4* t4b, float4* t4c, float2* t2a, float4* t4d, float2* t2b);
 
