@@ -30,6 +30,7 @@ struct lq_ctx {
   std::vector<void*> allocs;
   std::string err;
   bool uploaded = false;
+  bool searched = false;  // a neighbour search has left sorted keys in B.part_cel
   int nbits = 0;
   int npm0 = 0;
   int tile_shape = 1, tile_tz = 7;
@@ -463,6 +464,9 @@ static int create_impl(const lq_params* cases, int n_cases, int device, lq_ctx**
   }
   st |= dalloc(c, &B.keys, n); st |= dalloc(c, &B.vals, n);
   st |= dalloc(c, &B.keys_alt, n); st |= dalloc(c, &B.vals_alt, n);
+  st |= dalloc(c, &B.skey[0], n); st |= dalloc(c, &B.skey[1], n); st |= dalloc(c, &B.mflag, n + 1);
+  st |= dalloc(c, &B.mkey, n); st |= dalloc(c, &B.mslot, n);
+  st |= dalloc(c, &B.mkey_alt, n); st |= dalloc(c, &B.mslot_alt, n);
   st |= dalloc(c, &B.cell_beg, size_t(par.ncell)); st |= dalloc(c, &B.cell_end, size_t(par.ncell));
   st |= dalloc(c, &B.wall_near, size_t(std::max(1, par.ncell_material)));
   st |= dalloc(c, &B.hist, size_t(256) * radix_tiles(par.np) + size_t(radix_tiles(par.np)) + 1);  // + scan tile sums
@@ -475,7 +479,7 @@ static int create_impl(const lq_params* cases, int n_cases, int device, lq_ctx**
   if (st) return bail(LQ_ERR_HIP);
   hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device);
   c->n_cu = std::max(1, c->n_cu);
-  B.part_cel = B.keys;
+  B.part_cel = B.skey[0];
   B.part_idx = B.vals;
   if (hipHostMalloc((void**)&c->wf_host, sizeof(unsigned long long) * 2 * n_cases) != hipSuccess)
     return bail(fail(c, LQ_ERR_HIP, "hipHostMalloc"));
@@ -987,10 +991,15 @@ extern "C" int lq_case_count(const lq_ctx* c) { return c ? c->n_cases : 0; }
   do { if (!(c)) return LQ_ERR_ARG; if (!(c)->uploaded) return fail((c), LQ_ERR_STATE, "call lq_upload first"); } while (0)
 
 static int enq_step_calc(lq_ctx* c) { launch_step_calc(c->stream, c->dP); return 0; }
+// After the first search the sort merges the few slots whose cell changed
+// into the previous order (the sorted keys alternate between two buffers).
 static int enq_ns(lq_ctx* c) {
   Section s(c, LQ_K_NEIGHBOR_SEARCH);
-  const int* k; const int* v;
-  launch_neighbor_search(c->stream, c->dP, c->par.np, c->nbits, c->par.ncell, c->B.cur, c->B, &k, &v);
+  Buffers& B = c->B;
+  const int* prev = c->searched ? B.part_cel : nullptr;
+  int* out = B.part_cel == B.skey[0] ? B.skey[1] : B.skey[0];
+  launch_neighbor_search(c->stream, c->dP, c->par.np, c->nbits, c->par.ncell, B.cur, B, prev, out);
+  c->searched = true;
   return 0;
 }
 // SortArraysPrimary in place: only the slots the sort moved are rewritten.

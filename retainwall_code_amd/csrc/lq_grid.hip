@@ -19,7 +19,7 @@ __device__ __forceinline__ int cell_key(const DevParams* P, int4 c) { return lin
 // cuGetParticleCellIndex, loquat_grid.cu:71-95.  Also flags whether any key
 // differs from the previous sorted key at the same slot.
 __global__ void k_hash(DevParams* __restrict__ P, Arrays A, int* __restrict__ keys, int* __restrict__ vals,
-                       const int* __restrict__ prev_sorted) {
+                       const int* __restrict__ prev_sorted, int* __restrict__ mflag) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P->np) return;
   int4 c = A.cellpos[i];
@@ -34,7 +34,11 @@ __global__ void k_hash(DevParams* __restrict__ P, Arrays A, int* __restrict__ ke
   }
   keys[i] = key;
   vals[i] = i;
-  if (prev_sorted != nullptr && key != prev_sorted[i]) P->sort_changed = 1;
+  if (prev_sorted != nullptr) {
+    const bool moved = key != prev_sorted[i];
+    if (moved) P->sort_changed = 1;
+    if (mflag) mflag[i] = moved ? 1 : 0;
+  }
 }
 
 __global__ void k_fill_int(int* __restrict__ a, int n, int v) {
@@ -190,6 +194,136 @@ __global__ void __launch_bounds__(RS_THREADS) k_radix_scatter(const int* __restr
   }
 }
 
+// ---- merge search -------------------------------------------------------
+// The stable sort by (key, slot) after a previous sort: slot i held key
+// prev[i] then (prev is sorted), and between two steps few particles change
+// cell.  Slots whose key is unchanged ("stayers") are already in (key, slot)
+// order; the others ("movers") are compacted in slot order, sorted stably by
+// key, and the two sorted sequences are merged by rank.  The result equals
+// the full radix sort for any inputs; it is fast when movers are few.
+
+// Movers in slot order: mflag holds the exclusive scan of the flags
+// (np + 1 entries, the count last).
+__global__ void k_mover_compact(const DevParams* __restrict__ P, const int* __restrict__ keys,
+                                const int* __restrict__ mscan, int* __restrict__ mkey, int* __restrict__ mslot) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P->np) return;
+  const int r = mscan[i];
+  if (mscan[i + 1] != r) {
+    mkey[r] = keys[i];
+    mslot[r] = i;
+  }
+}
+
+// Stable LSD radix sort of the M movers by key in one workgroup, over 8-bit
+// digits, tiles of 1024 ranked by wave ballots as in k_radix_scatter.  M is
+// read on the device; any M is sorted correctly (large M slowly).
+constexpr int MS_THREADS = 1024;
+__global__ void __launch_bounds__(MS_THREADS) k_mover_sort(const int* __restrict__ mtotal, int* k0, int* v0, int* k1,
+                                                           int* v1, int nbits) {
+  __shared__ int cnt[256];
+  __shared__ int run[256];
+  __shared__ int wcnt[MS_THREADS / 64][256];
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  const int M = *mtotal;
+  const unsigned long long lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  int *kin = k0, *vin = v0, *kout = k1, *vout = v1;
+  for (int shift = 0; shift < nbits; shift += 8) {
+    if (t < 256) cnt[t] = 0;
+    __syncthreads();
+    for (int i = t; i < M; i += MS_THREADS) atomicAdd(&cnt[(kin[i] >> shift) & 255], 1);
+    __syncthreads();
+    if (t == 0) {
+      int sum = 0;
+      for (int d = 0; d < 256; d++) {
+        run[d] = sum;
+        sum += cnt[d];
+      }
+    }
+    __syncthreads();
+    for (int base = 0; base < M; base += MS_THREADS) {
+      for (int k = t; k < (MS_THREADS / 64) * 256; k += MS_THREADS) wcnt[k >> 8][k & 255] = 0;
+      __syncthreads();
+      const int idx = base + t;
+      const bool valid = idx < M;
+      const int key = valid ? kin[idx] : 0;
+      const int val = valid ? vin[idx] : 0;
+      const int d = (key >> shift) & 255;
+      unsigned long long peers = __ballot(valid);
+#pragma unroll
+      for (int b = 0; b < 8; b++) {
+        const bool bit = (d >> b) & 1;
+        const unsigned long long m = __ballot(bit);
+        peers &= bit ? m : ~m;
+      }
+      const int rank = __popcll(peers & lt);
+      if (valid && rank == 0) wcnt[w][d] = __popcll(peers);
+      __syncthreads();
+      if (valid) {
+        int pos = run[d] + rank;
+        for (int ww = 0; ww < w; ww++) pos += wcnt[ww][d];
+        kout[pos] = key;
+        vout[pos] = val;
+      }
+      __syncthreads();
+      if (t < 256) {
+        int add = 0;
+        for (int ww = 0; ww < MS_THREADS / 64; ww++) add += wcnt[ww][t];
+        run[t] += add;
+      }
+      __syncthreads();
+    }
+    int* tk = kin; kin = kout; kout = tk;
+    int* tv = vin; vin = vout; vout = tv;
+  }
+}
+
+// Final position of every slot: a stayer i goes after the stayers before it
+// and the movers (key, slot) < (key_i, i); the t-th sorted mover goes after
+// t movers and the stayers (key, slot) < its own, found by binary search of
+// its key in prev (stayers' keys are their prev entries).
+__global__ void k_merge_place(const DevParams* __restrict__ P, const int* __restrict__ keys,
+                              const int* __restrict__ prev, const int* __restrict__ mscan,
+                              const int* __restrict__ smkey, const int* __restrict__ smslot, int* __restrict__ out_key,
+                              int* __restrict__ out_idx) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int np = P->np;
+  if (i >= np) return;
+  const int M = mscan[np];
+  if (mscan[i + 1] == mscan[i]) {
+    const int key = keys[i];
+    int lo = 0, hi = M;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      const int mk = smkey[mid];
+      if (mk < key || (mk == key && smslot[mid] < i)) lo = mid + 1;
+      else hi = mid;
+    }
+    const int pos = (i - mscan[i]) + lo;
+    out_key[pos] = key;
+    out_idx[pos] = i;
+  }
+  if (i < M) {
+    const int key = smkey[i], slot = smslot[i];
+    int lo = 0, hi = np;
+    while (lo < hi) {  // first prev >= key
+      const int mid = (lo + hi) >> 1;
+      if (prev[mid] < key) lo = mid + 1;
+      else hi = mid;
+    }
+    int up = lo, uh = np;
+    while (up < uh) {  // first prev > key
+      const int mid = (up + uh) >> 1;
+      if (prev[mid] <= key) up = mid + 1;
+      else uh = mid;
+    }
+    const int p = max(lo, min(slot, up));
+    const int pos = i + (p - mscan[p]);
+    out_key[pos] = key;
+    out_idx[pos] = slot;
+  }
+}
+
 // cuFindBegEndInCell, loquat_grid.cu:116-144 (half-open runs).
 __global__ void k_beg_end(const DevParams* __restrict__ P, const int* __restrict__ sorted, int* __restrict__ cbeg,
                           int* __restrict__ cend) {
@@ -330,36 +464,55 @@ void launch_fill_int(hipStream_t s, int* a, int n, int v) {
 
 int radix_tiles(int n) { return blocks(n, RS_TILE); }
 
-// Hash + stable radix sort + ranges + deletion bookkeeping.  Leaves the
-// sorted keys/values in *sorted_keys / *sorted_vals (pointers into B).
+// Hash + stable sort by (key, slot) + ranges + deletion bookkeeping.  With
+// prev (the sorted keys of the previous search, slot order unchanged since)
+// the sort is the merge search; without it a full radix sort.  The sorted
+// keys go to out_key (not prev), the permutation to B.vals.
 void launch_neighbor_search(hipStream_t s, DevParams* P, int np, int nbits, int ncell, Arrays A, Buffers& B,
-                            const int** sorted_keys, const int** sorted_vals) {
+                            const int* prev, int* out_key) {
   hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, s, P);
-  hipLaunchKernelGGL(k_hash, dim3(blocks(np, 256)), dim3(256), 0, s, P, A, B.keys, B.vals, (const int*)nullptr);
-  int* kin = B.keys; int* vin = B.vals;
-  int* kout = B.keys_alt; int* vout = B.vals_alt;
-  const int nb = radix_tiles(np);
-  for (int shift = 0; shift < nbits; shift += 8) {
-    hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(RS_THREADS), 0, s, (const int*)kin, np, shift, B.hist, nb);
-    {
-      const int nh = 256 * nb, nt = scan_tiles(nh);
-      int* sums = B.hist + nh;  // the histogram allocation carries the tile sums
-      hipLaunchKernelGGL(k_scan_tile_sums, dim3(nt), dim3(SC_THREADS), 0, s, (const int*)B.hist, nh, sums);
-      hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SC_THREADS), 0, s, sums, nt);
-      hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(SC_THREADS), 0, s, B.hist, nh, (const int*)sums);
+  hipLaunchKernelGGL(k_hash, dim3(blocks(np, 256)), dim3(256), 0, s, P, A, B.keys, B.vals, prev,
+                     prev ? B.mflag : nullptr);
+  if (prev) {
+    const int n = np + 1, nt = scan_tiles(n);
+    int* sums = B.hist;
+    hipLaunchKernelGGL(k_scan_tile_sums, dim3(nt), dim3(SC_THREADS), 0, s, (const int*)B.mflag, n, sums);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SC_THREADS), 0, s, sums, nt);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(SC_THREADS), 0, s, B.mflag, n, (const int*)sums);
+    hipLaunchKernelGGL(k_mover_compact, dim3(blocks(np, 256)), dim3(256), 0, s, (const DevParams*)P,
+                       (const int*)B.keys, (const int*)B.mflag, B.mkey, B.mslot);
+    hipLaunchKernelGGL(k_mover_sort, dim3(1), dim3(MS_THREADS), 0, s, (const int*)(B.mflag + np), B.mkey, B.mslot,
+                       B.mkey_alt, B.mslot_alt, nbits);
+    const bool odd = ((nbits + 7) / 8) & 1;
+    hipLaunchKernelGGL(k_merge_place, dim3(blocks(np, 256)), dim3(256), 0, s, (const DevParams*)P,
+                       (const int*)B.keys, prev, (const int*)B.mflag, (const int*)(odd ? B.mkey_alt : B.mkey),
+                       (const int*)(odd ? B.mslot_alt : B.mslot), out_key, B.vals);
+    B.part_idx = B.vals;
+  } else {
+    int* kin = B.keys; int* vin = B.vals;
+    int* kout = B.keys_alt; int* vout = B.vals_alt;
+    const int nb = radix_tiles(np);
+    for (int shift = 0; shift < nbits; shift += 8) {
+      hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(RS_THREADS), 0, s, (const int*)kin, np, shift, B.hist, nb);
+      {
+        const int nh = 256 * nb, nt = scan_tiles(nh);
+        int* sums = B.hist + nh;  // the histogram allocation carries the tile sums
+        hipLaunchKernelGGL(k_scan_tile_sums, dim3(nt), dim3(SC_THREADS), 0, s, (const int*)B.hist, nh, sums);
+        hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SC_THREADS), 0, s, sums, nt);
+        hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(SC_THREADS), 0, s, B.hist, nh, (const int*)sums);
+      }
+      hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(RS_THREADS), 0, s, (const int*)kin, (const int*)vin, kout,
+                         vout, np, shift, (const int*)B.hist, nb);
+      std::swap(kin, kout);
+      std::swap(vin, vout);
     }
-    hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(RS_THREADS), 0, s, (const int*)kin, (const int*)vin, kout, vout,
-                       np, shift, (const int*)B.hist, nb);
-    std::swap(kin, kout);
-    std::swap(vin, vout);
+    hipMemcpyAsync(out_key, kin, size_t(np) * sizeof(int), hipMemcpyDeviceToDevice, s);
+    B.part_idx = vin;
   }
-  B.part_cel = kin;
-  B.part_idx = vin;
-  *sorted_keys = kin;
-  *sorted_vals = vin;
+  B.part_cel = out_key;
   launch_fill_int(s, B.cell_beg, ncell, -1);
   launch_fill_int(s, B.cell_end, ncell, -1);
-  hipLaunchKernelGGL(k_beg_end, dim3(blocks(np, 256)), dim3(256), 0, s, P, (const int*)kin, B.cell_beg, B.cell_end);
+  hipLaunchKernelGGL(k_beg_end, dim3(blocks(np, 256)), dim3(256), 0, s, P, (const int*)out_key, B.cell_beg, B.cell_end);
   if (ncell > 1) {
     const int ncm = (ncell - 1) / 2;
     hipLaunchKernelGGL(k_wall_near, dim3(blocks(ncm, 256)), dim3(256), 0, s, P, (const int*)B.cell_beg, B.wall_near);

@@ -128,6 +128,9 @@ struct Buffers {
   int* pcount;       // pair-list length per soil slot
   // grid / sort
   int *keys, *vals, *keys_alt, *vals_alt, *part_cel, *part_idx;
+  // merge search: sorted keys of the last two searches (part_cel is one of
+  // them), mover flags scanned in place (np + 1), compacted movers
+  int *skey[2], *mflag, *mkey, *mslot, *mkey_alt, *mslot_alt;
   int *cell_beg, *cell_end;
   int* wall_near;    // per cell: a wall particle lies in its 27-cell neighbourhood
   int* hist;         // radix histogram [256][nblocks]

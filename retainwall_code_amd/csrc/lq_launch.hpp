@@ -60,8 +60,10 @@ void launch_gelu_back(hipStream_t s, float* dA, const float* Z, long long n);
 void launch_colsum(hipStream_t s, const float* dZ, int M, int N, float* db);
 void launch_adam(hipStream_t s, float* p, const float* g, float* m, float* v, int n, float lr, float c1, float c2);
 void launch_gather_rows(hipStream_t s, const float* src, int ld, const int* idx, int rows, int cols, float* dst);
+// prev: sorted keys of the previous search (merge search), or nullptr (full
+// radix sort).  Sorted keys go to out_key; sets B.part_cel and B.part_idx.
 void launch_neighbor_search(hipStream_t s, DevParams* P, int np, int nbits, int ncell, Arrays A, Buffers& B,
-                            const int** sorted_keys, const int** sorted_vals);
+                            const int* prev, int* out_key);
 void launch_gather(hipStream_t s, const DevParams* P, int np, Arrays src, Arrays dst, const int* idx, const Buffers& B);
 // The same permutation in place (alternate set as scratch), touching only moved slots.
 void launch_gather_moved(hipStream_t s, const DevParams* P, int np, Arrays cur, Arrays alt, const int* idx);

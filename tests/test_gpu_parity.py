@@ -82,6 +82,37 @@ def test_grid_bitexact(name, shuffle, oracle_mod):
     ctx.close()
 
 
+def test_merge_search_many_movers(oracle_mod):
+    """After a first search the sort merges the slots whose key changed into
+    the previous order.  Reversing the slot order of wall rows and of soil
+    rows on the device makes nearly every slot a mover (the one-workgroup
+    mover sort runs over several tiles); the result must still be the
+    stable sort, bit for bit, and a following step must match the oracle."""
+    case = cases.config("tiny")
+    say("[merge search] case=tiny, reversed slots")
+    ctx = sim.open_case(case)  # the upload's search leaves sorted keys
+    g = ctx.snapshot(STATE)
+    nb, n = case.npb, case.np
+    perm = np.concatenate([np.arange(nb)[::-1], nb + np.arange(n - nb)[::-1]])
+    for k in STATE:
+        if k == "normal":
+            ctx.set(k, g[k][perm[:nb]] if len(g[k]) == nb else g[k][perm])
+        else:
+            ctx.set(k, g[k][perm])
+    o = oracle_from(ctx, oracle_mod, np.float32, case)
+    ctx.neighbor_search()
+    ctx.sync()
+    o.neighbor_search()
+    for k in ["part_cel", "part_idx", "cell_beg", "cell_end"]:
+        np.testing.assert_array_equal(ctx.get(k), o[k], err_msg=k)
+    ctx.sort_arrays_primary()
+    ctx.sync()
+    o.sort_arrays_primary()
+    for k in ["codes", "cellpos", "orig_id", "pos", "vel", "str"]:
+        np.testing.assert_array_equal(bits(ctx.get(k)), bits(o[k]), err_msg=k)
+    ctx.close()
+
+
 @pytest.mark.parametrize("name", ["tiny", "small"])
 def test_neighbor_lists_bitexact(name, oracle_mod):
     case = cases.config(name)
