@@ -1141,7 +1141,8 @@ static int enq_step(lq_ctx* c) {
   return 0;
 }
 
-// Captures two steps (the cur/alt buffer swap returns to its start).  With
+// Captures two steps (the two sorted-key buffers alternate per search, so
+// after two steps every pointer is back at its start).  With
 // timing on, the timed sections' events become event-record nodes; they are
 // matched to their sections here and re-pointed at fresh events per replay.
 static int capture_graph(lq_ctx* c) {

@@ -106,7 +106,7 @@ struct Arrays {
 
 struct Buffers {
   Arrays cur;   // current state
-  Arrays alt;   // gather destination (swapped with cur after each sort)
+  Arrays alt;   // scratch rows of the in-place sort; SortArraysAll gathers into it and swaps
   float4 *pos_ini, *vel_ini, *str_ini_a, *stn_ini_a;
   float2 *str_ini_b, *stn_ini_b;
   float4 *accad, *spin, *acestn_a;
