@@ -294,6 +294,27 @@ def test_whole_step(name, oracle_mod):
     ctx.close()
 
 
+def test_steps_with_shepard(oracle_mod):
+    """Shepard stress filtering every second step (a15 in the step driver,
+    loquat.cu:548-561): two steps, one of them filtered, match the FP32
+    oracle's steps (the filtered stresses feed that step's interaction)."""
+    case = cases.config("tiny", shepard=2)
+    say("[shepard steps] case=tiny, filter every 2 steps")
+    ctx = sim.open_case(case)
+    ctx.step(3)
+    ctx.sync()
+    o32 = oracle_from(ctx, oracle_mod, np.float32, case)
+    o32.run(2)  # steps 3 and 4: the filter runs at step 4
+    ctx.step(2)
+    ctx.sync()
+    soil = slice(case.npb, case.np)
+    for k in ["pos", "vel", "str", "idm"]:
+        e = relerr(ctx.get(k)[soil], o32[k][soil])
+        say(f"  {k} relerr vs FP32 oracle {e:.2e}")
+        assert e <= 1e-5, k
+    ctx.close()
+
+
 def test_multistep_and_wall_force(oracle_mod):
     case = cases.config("tiny")
     n, every = 40, 10
