@@ -327,6 +327,9 @@ def main():
                 "valu_tflops": round(flops_call / (avg_ms * 1e-3) / 1e12, 2),
                 "valu_frac": round(flops_call / (avg_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
                 "flops_per_launch": flops_call}
+        if traffic:  # measured HBM bytes per launch at the measured launch time
+            roof["traffic_GBps"] = round(traffic / (avg_ms * 1e-3) / 1e9, 1)
+            roof["traffic_frac"] = round(traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.case)
