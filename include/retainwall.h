@@ -267,7 +267,8 @@ int lq_synchronize(lq_ctx* ctx);
  * each kernel class on the context stream; lq_kernel_times returns the
  * accumulated milliseconds and launch counts per class.  Graph replays are
  * timed too: the captured graph holds the events as event-record nodes,
- * pointed at fresh events before every replay. */
+ * pointed at fresh events before every replay.  Once 8192 event pairs are
+ * waiting, lq_step synchronises between steps to fold them into the totals. */
 enum lq_kernel_class {
   LQ_K_NEIGHBOR_SEARCH = 0, LQ_K_SORT_ARRAYS = 1, LQ_K_ADAMI = 2, LQ_K_CORNER = 3,
   LQ_K_INTERACTION = 4, LQ_K_CONTACT = 5, LQ_K_PREDICTOR = 6, LQ_K_CORRECTOR = 7,

@@ -149,6 +149,10 @@ struct Section {
     }
   }
 };
+// Folds the pending event pairs into the per-class totals (synchronises the
+// stream).  lq_step also does it between replays once kTimingPending pairs
+// wait, so a long timed run holds a bounded number of events.
+constexpr size_t kTimingPending = 8192;
 static void resolve_timing(lq_ctx* c) {
   if (c->pending.empty()) return;
   hipStreamSynchronize(c->stream);
@@ -1206,6 +1210,7 @@ extern "C" int lq_step(lq_ctx* c, int nsteps, int use_graph) {
       if (st) return st;
     }
     while (nsteps - done >= 2) {
+      if (c->pending.size() > kTimingPending) resolve_timing(c);
       if (c->graph_timed) {
         for (const auto& t : c->graph_sections) {
           const hipEvent_t a = ev_get(c), b = ev_get(c);
@@ -1219,7 +1224,10 @@ extern "C" int lq_step(lq_ctx* c, int nsteps, int use_graph) {
       done += 2;
     }
   }
-  for (; done < nsteps; done++) enq_step(c);
+  for (; done < nsteps; done++) {
+    if (c->pending.size() > kTimingPending) resolve_timing(c);
+    enq_step(c);
+  }
   return check_launch(c);
 }
 
