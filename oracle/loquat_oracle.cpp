@@ -191,6 +191,15 @@ template <class T> static int step_T(Par<T>& P, MotionTab<T>* M, State<T>& S, do
     sig[0] = s.xx; sig[1] = s.yy; sig[2] = s.zz; sig[3] = s.xy; sig[4] = s.xz; sig[5] = s.yz;          \
     *mcrt = m; return 0;                                                                               \
   }                                                                                                    \
+  extern "C" int or_constitutive_##SFX(int model_type, T dt, T rho, const T* w, const T* D, T* sig,   \
+                                       const T* mp, T* mcrt, int* yld) {                               \
+    sym6<T> s = {sig[0], sig[1], sig[2], sig[3], sig[4], sig[5]};                                      \
+    sym6<T> d = {D[0], D[1], D[2], D[3], D[4], D[5]};                                                  \
+    T m = *mcrt; bool y = false;                                                                       \
+    constitutive(model_type, dt, rho, V3<T>(w[0], w[1], w[2]), d, s, mp, m, y);                        \
+    sig[0] = s.xx; sig[1] = s.yy; sig[2] = s.zz; sig[3] = s.xy; sig[4] = s.xz; sig[5] = s.yz;          \
+    *mcrt = m; *yld = y ? 1 : 0; return 0;                                                             \
+  }                                                                                                    \
   extern "C" double or_wall_force_##SFX(const lq_params* p, or_state* s, const float* box) {          \
     Par<T> P; load_par(p, P); State<T> S = view<T>(s); return wall_force(P, S, box);                   \
   }                                                                                                    \

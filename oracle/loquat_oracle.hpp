@@ -731,6 +731,123 @@ template <class T> static void hypo_cs(T dt, T rho, v3<T> w, sym6<T> D, sym6<T>&
   sig = sym6<T>{nd6.xx + sph + coh, nd6.yy + sph + coh, nd6.zz + sph + coh, nd6.xy, nd6.xz, nd6.yz};
 }
 
+// ------------------------------- DP, mu(I)-DP, basic hypoplasticity ----
+// The Jaumann term sym(W S - S W), W = SkewTensor2FullTensor(w), as hypo_cs
+// writes it.
+template <class T> static sym6<T> jaumann(v3<T> w, sym6<T> s) {
+  const T Sxx = s.xx, Syy = s.yy, Szz = s.zz, Sxy = s.xy, Sxz = s.xz, Syz = s.yz;
+  full9<T> WS = {
+      -w.z * Sxy + w.y * Sxz, -w.z * Syy + w.y * Syz, -w.z * Syz + w.y * Szz,
+      w.z * Sxx - w.x * Sxz, w.z * Sxy - w.x * Syz, w.z * Sxz - w.x * Szz,
+      -w.y * Sxx + w.x * Sxy, -w.y * Sxy + w.x * Syy, -w.y * Sxz + w.x * Syz};
+  full9<T> SW = {
+      Sxy * w.z - Sxz * w.y, -Sxx * w.z + Sxz * w.x, Sxx * w.y - Sxy * w.x,
+      Syy * w.z - Syz * w.y, -Sxy * w.z + Syz * w.x, Sxy * w.y - Syy * w.x,
+      Syz * w.z - Szz * w.y, -Sxz * w.z + Szz * w.x, Sxz * w.y - Syz * w.x};
+  return sym6<T>{WS.xx - SW.xx, WS.yy - SW.yy, WS.zz - SW.zz, T(0.5) * ((WS.xy - SW.xy) + (WS.yx - SW.yx)),
+                 T(0.5) * ((WS.xz - SW.xz) + (WS.zx - SW.zx)), T(0.5) * ((WS.yz - SW.yz) + (WS.zy - SW.zy))};
+}
+
+// Elastic trial stress shared by both Drucker-Prager models:
+// sigma + (Jaumann + 2G D + lambda tr(D) I) dt, lambda = K - 2G/3.
+template <class T> static sym6<T> elastic_trial(T dt, v3<T> w, sym6<T> D, sym6<T> sig, T K, T G) {
+  const T lame = K - T(2) / T(3) * G;
+  sym6<T> rate = jaumann(w, sig) + D * (T(2) * G);
+  const T l = lame * trace(D);
+  rate.xx = rate.xx + l; rate.yy = rate.yy + l; rate.zz = rate.zz + l;
+  return sig + rate * dt;
+}
+
+template <class T> static sym6<T> add_iso(sym6<T> s, T p) {
+  s.xx = s.xx + p; s.yy = s.yy + p; s.zz = s.zz + p;
+  return s;
+}
+
+// cuMatModDP, constitutive_model.h:38-84: Drucker-Prager, elastic-perfectly
+// plastic with a tension cut-off; mp = {K, G, k_phi, k_psi, k_c, ...}
+// (GetMaterial_DP_EPP, GenCase.cu:16-25).  yld is set by a shear return.
+template <class T> static void dp_epp(T dt, v3<T> w, sym6<T> D, sym6<T>& sig, const T* mp, bool& yld) {
+  const T K = mp[0], G = mp[1], kphi = mp[2], kpsi = mp[3], kc = mp[4];
+  const sym6<T> pre = elastic_trial(dt, w, D, sig, K, G);
+  T sph = trace(pre) / T(3);
+  sym6<T> dev = add_iso(pre, -(trace(pre) / T(3)));
+  const T tenlt = kphi == T(0) ? T(0) : kc / kphi;
+  if (sph - tenlt >= T(0)) sph = tenlt;
+  const T tau = std::sqrt(T(0.5) * ddot(dev, dev));
+  const T shr = tau - (-sph * kphi + kc);
+  if (shr > T(0)) {
+    const T dlam = shr / (G + K * kphi * kpsi);
+    sph = sph - K * kpsi * dlam;
+    const T ratio = tau < T(1e-10) ? T(1) : (kc - kphi * sph) / tau;
+    dev = dev * ratio;
+    yld = true;
+  }
+  sig = add_iso(dev, sph);
+}
+
+// cuMatModMuIDP, constitutive_model.h:90-130: Drucker-Prager with the mu(I)
+// friction law; mp = {K, G, mu_s, mu_2, I0, rho_g, d, lambda, ...}
+// (GetMaterial_MuI_DP_EPP, GenCase.cu:27-40).
+template <class T> static void mui_dp(T dt, v3<T> w, sym6<T> D, sym6<T>& sig, const T* mp) {
+  const T K = mp[0], G = mp[1], mus = mp[2], dmu = mp[3] - mp[2], I0 = mp[4], rhog = mp[5], dg = mp[6];
+  const T lam = mp[7];
+  const sym6<T> pre = elastic_trial(dt, w, D, sig, K, G);
+  T sph = trace(pre) / T(3);
+  sym6<T> dev = add_iso(pre, -(trace(pre) / T(3)));
+  const sym6<T> dD = add_iso(D, -(trace(D) / T(3)));
+  const T gamma = std::sqrt(T(2) * ddot(dD, dD));
+  if (sph > T(0)) sph = T(0);
+  const T p = -sph;
+  const T mu = mus + dmu * gamma * dg / (I0 * std::sqrt(p / rhog) + dg * std::sqrt(gamma * gamma + lam * lam));
+  const T tau = std::sqrt(T(0.5) * ddot(dev, dev));
+  if (tau + sph * mu > T(0)) {
+    const T ratio = tau < T(1e-10) ? T(1) : (T(0) - mu * sph) / tau;
+    dev = dev * ratio;
+  }
+  sig = add_iso(dev, sph);
+}
+
+// cuMatModHypo (basic overload), constitutive_model.h:136-202; mp = {Ei, Vi,
+// phi_c, c, p_ref, ...} (GetMaterial_HypoBasic, GenCase.cu:42-50).
+template <class T> static void hypo_basic(T dt, v3<T> w, sym6<T> D, sym6<T>& sig, const T* mp) {
+  const T Ei = mp[0], Vi = mp[1], phic = mp[2], coh = mp[3], pref = mp[4];
+  const T C1 = -Ei / (T(3) * pref * (T(1) + Vi));
+  const T a = T(3) / (std::sqrt(T(2)) * std::sin(phic));
+  const T C2 = C1 * ((T(3) * Vi + a * std::sqrt(T(1) + Vi * Vi)) / (T(1) - Vi) - a * a / T(3));
+  const T C3 = a * a * C1;
+  const T C4 = a * C1;
+  sym6<T> s = sig;
+  if (coh > T(0)) { s.xx = s.xx - coh; s.yy = s.yy - coh; s.zz = s.zz - coh; }
+  const sym6<T> jau = jaumann(w, s);
+  const T I1 = trace(s);
+  if (I1 > T(-4)) { sig = sym6<T>{-1, -1, -2, 0, 0, 0}; return; }
+  const sym6<T> dev = add_iso(s, -(I1 / T(3)));
+  const T trs = trace(s), trd = trace(D), trsd = ddot(s, D), nd = std::sqrt(ddot(D, D));
+  const sym6<T> rate = jau + D * (C1 * trs) + s * (C2 * trd) + s * (C3 * trsd / trs) + (s + dev) * (C4 * nd);
+  const sym6<T> ns = sig + rate * dt;
+  const T sph = trace(ns) / T(3);
+  if (sph > T(0)) { sig = sym6<T>{T(-1) + coh, T(-1) + coh, T(-2) + coh, 0, 0, 0}; return; }
+  sym6<T> nd6 = add_iso(ns, -sph);
+  const T kphi = T(3) / (std::sqrt(T(2)) * a);
+  const T tau = std::sqrt(T(0.5) * ddot(nd6, nd6));
+  if (tau - (-sph * kphi) > T(0)) {
+    const T ratio = tau < T(1e-10) ? T(1) : (-kphi * sph) / tau;
+    nd6 = nd6 * ratio;
+  }
+  sig = add_iso(nd6, sph + coh);
+}
+
+// The model dispatch of cuPredictorMaterial / cuCorrectorMaterial
+// (predictor_corrector.cu:72-75, 169-172); RIGID and unknown types leave
+// the stress unchanged.
+template <class T> static void constitutive(int model_type, T dt, T rho, v3<T> w, sym6<T> D, sym6<T>& sig, const T* mp,
+                                           T& Mcrt, bool& yld) {
+  if (model_type == LQ_DP_EPP) dp_epp(dt, w, D, sig, mp, yld);
+  else if (model_type == LQ_MUI_DP_EPP) mui_dp(dt, w, D, sig, mp);
+  else if (model_type == LQ_HYPO_BASIC) hypo_basic(dt, w, D, sig, mp);
+  else if (model_type == LQ_HYPO_CS) hypo_cs(dt, rho, w, D, sig, mp, Mcrt);
+}
+
 // ---------------------------------------------------- a12: predictor ----
 // LoquatSph::Predictor + cuPredictorMaterial, predictor_corrector.cu:52-103.
 template <class T> static void predictor(const Par<T>& P, State<T>& S) {
@@ -753,9 +870,10 @@ template <class T> static void predictor(const Par<T>& P, State<T>& S) {
     st6(S.stn, i, ld6(S.stn, i) + D * hdt);
     sym6<T> sig = ld6(S.str, i);
     T Mcrt = 0;
-    if (S.codes[4 * i + 2] == LQ_HYPO_CS) hypo_cs(hdt, rho, ld3(S.spin, i), D, sig, mp, Mcrt);
+    bool yld = false;
+    constitutive(S.codes[4 * i + 2], hdt, rho, ld3(S.spin, i), D, sig, mp, Mcrt, yld);
     st6(S.str, i, sig);
-    S.idm[4 * i] = T(100);
+    if (!yld) S.idm[4 * i] = T(100);  // a Drucker-Prager yield keeps the old value (:78-83)
   }
 }
 
@@ -822,9 +940,10 @@ template <class T> static int corrector_material(const Par<T>& P, State<T>& S) {
     st6(S.stn, i, ld6(S.stn_ini, i) + D * P.dt);
     sym6<T> sig = ld6(S.str_ini, i);
     T Mcrt = 0;
-    if (S.codes[4 * i + 2] == LQ_HYPO_CS) hypo_cs(P.dt, rho, ld3(S.spin, i), D, sig, mp, Mcrt);
+    bool yld = false;
+    constitutive(S.codes[4 * i + 2], P.dt, rho, ld3(S.spin, i), D, sig, mp, Mcrt, yld);
     st6(S.str, i, sig);
-    S.idm[4 * i] = T(100);
+    if (!yld) S.idm[4 * i] = T(100);  // :175-180
     S.idm[4 * i + 3] = Mcrt;
     v3<int> c = cell_of(S.cellpos, i);
     if (!in_grid(P, c)) { S.cellpos[4 * i + 3] = P.ncell_delete_m; S.idm[4 * i] = T(999); left++; }

@@ -54,6 +54,8 @@ def lib():
             getattr(L, f"or_neighbor_search_{sfx}").argtypes = [P(_abi.Params), P(OrState), P(C.c_int)]
             getattr(L, f"or_rigid_body_update_{sfx}").argtypes = [P(_abi.Params), P(_abi.MotionTable)]
             getattr(L, f"or_hypo_cs_{sfx}").argtypes = [T, T, P(T), P(T), P(T), P(T), P(T)]
+            getattr(L, f"or_constitutive_{sfx}").argtypes = [C.c_int, T, T, P(T), P(T), P(T), P(T), P(T),
+                                                             P(C.c_int)]
             getattr(L, f"or_wall_force_{sfx}").argtypes = [P(_abi.Params), P(OrState), P(C.c_float)]
             getattr(L, f"or_wall_force_{sfx}").restype = C.c_double
             getattr(L, f"or_neighbor_lists_{sfx}").argtypes = [P(_abi.Params), P(OrState), C.c_int, C.c_void_p,
@@ -208,6 +210,22 @@ def hypo_cs(dt, rho, w, D, sig, mp, dtype=np.float64):
     mc = T(0)
     getattr(lib(), f"or_hypo_cs_{sfx}")(T(dt), T(rho), wv, Dv, sv, mv, C.byref(mc))
     return np.array(sv[:], dtype=dtype), mc.value
+
+
+def constitutive(model_type, dt, rho, w, D, sig, mp, dtype=np.float64):
+    """One stress update by the model dispatch of cuPredictorMaterial
+    (predictor_corrector.cu:72-75): DP, mu(I)-DP, basic or critical-state
+    hypoplasticity.  Returns (sigma, M_crt, yielded)."""
+    T = C.c_double if np.dtype(dtype) == np.float64 else C.c_float
+    sfx = "f64" if T is C.c_double else "f32"
+    wv = (T * 3)(*w)
+    Dv = (T * 6)(*D)
+    sv = (T * 6)(*sig)
+    mv = (T * 15)(*mp)
+    mc = T(0)
+    y = C.c_int(0)
+    getattr(lib(), f"or_constitutive_{sfx}")(int(model_type), T(dt), T(rho), wv, Dv, sv, mv, C.byref(mc), C.byref(y))
+    return np.array(sv[:], dtype=dtype), mc.value, bool(y.value)
 
 
 def kernel(knorm, q):

@@ -41,6 +41,44 @@ def hypo_cs_sand(rho0=1600.0, phi_c_deg=30.0, e0=0.70, h_s=5.8e9, n=0.28, e_d0=0
             rho0, rho0 * 0.8, rho0 * 1.2]
 
 
+def dp_soil(E=2.0e7, V=0.3, phi_deg=30.0, psi_deg=0.0, cohesion=0.0, rho0=1600.0):
+    """Drucker-Prager row, GetMaterial_DP_EPP (GenCase.cu:16-25): bulk and
+    shear modulus, then k_phi, k_psi and k_c of the cone fitted to
+    Mohr-Coulomb (plane strain); angles in degrees here."""
+    tphi, tpsi = math.tan(math.radians(phi_deg)), math.tan(math.radians(psi_deg))
+    row = [0.0] * 15
+    row[0] = E / (3.0 * (1.0 - 2.0 * V))
+    row[1] = E / (2.0 * (1.0 + V))
+    row[2] = 3.0 * tphi / math.sqrt(9.0 + 12.0 * tphi * tphi)
+    row[3] = 3.0 * tpsi / math.sqrt(9.0 + 12.0 * tpsi * tpsi)
+    row[4] = 3.0 * cohesion / math.sqrt(9.0 + 12.0 * tphi * tphi)
+    row[13], row[14] = rho0 * 0.8, rho0 * 1.2
+    return row
+
+
+def mui_dp_soil(E=2.0e7, V=0.3, phi1_deg=30.0, phi2_deg=40.0, I0=0.3, rho_g=2650.0, d_grain=0.001,
+                lam=1.0e-3, rho0=1600.0):
+    """mu(I) Drucker-Prager row, GetMaterial_MuI_DP_EPP (GenCase.cu:27-40):
+    moduli, mu_s = tan(phi1), mu_2 = tan(phi2), I0, grain density, grain
+    diameter and the regularisation lambda."""
+    row = [0.0] * 15
+    row[0] = E / (3.0 * (1.0 - 2.0 * V))
+    row[1] = E / (2.0 * (1.0 + V))
+    row[2] = math.tan(math.radians(phi1_deg))
+    row[3] = math.tan(math.radians(phi2_deg))
+    row[4], row[5], row[6], row[7] = I0, rho_g, d_grain, lam
+    row[13], row[14] = rho0 * 0.8, rho0 * 1.2
+    return row
+
+
+def hypo_basic_soil(Ei=5.0e7, Vi=0.3, phi_c_deg=30.0, cohesion=0.0, p_ref=1.0e5, rho0=1600.0):
+    """Basic hypoplastic row, GetMaterial_HypoBasic (GenCase.cu:42-50)."""
+    row = [0.0] * 15
+    row[0], row[1], row[2], row[3], row[4] = Ei, Vi, math.radians(phi_c_deg), cohesion, p_ref
+    row[13], row[14] = rho0 * 0.8, rho0 * 1.2
+    return row
+
+
 def _splitmix64(x):
     x = (x + np.uint64(0x9E3779B97F4A7C15)) & np.uint64(0xFFFFFFFFFFFFFFFF)
     z = x
@@ -111,7 +149,7 @@ def retaining_wall(box_length=0.4, box_width=0.2, box_height=0.3, dr=0.005, hdr=
                    fix_wall_height=0.04, wall_extra=0.05, rho0=1600.0, material=None,
                    cs=65.0, cfl=0.1, alpha_visc=1.5, art_p=0.01, gravity_z=-9.81, K0=0.5,
                    motion=("translate", -0.001), jitter=0.0, void_range=None, seed=0,
-                   time_max=1.0, shepard=0, name="retaining_wall") -> Case:
+                   time_max=1.0, shepard=0, model_type=_abi.HYPO_CS, name="retaining_wall") -> Case:
     """Build a GenCase case-1 style wall case.
 
     box_* are the backfill dimensions (x away from the wall, y along it, z up).
@@ -120,6 +158,9 @@ def retaining_wall(box_length=0.4, box_width=0.2, box_height=0.3, dr=0.005, hdr=
     perturbation in units of dr (soil and walls alike).  void_range=(lo, hi)
     draws each soil particle's initial void ratio e_i and sets its density to
     rho0 (1 + e0) / (1 + e_i) (constitutive_model.h:259 read backwards).
+    model_type is the soil's constitutive model (codes.z, types.h:51); pass
+    the matching parameter row as material (dp_soil, mui_dp_soil,
+    hypo_basic_soil) for any model other than the default HypoCS.
     """
     dr = f32(dr)
     hdr = f32(hdr)
@@ -164,7 +205,7 @@ def retaining_wall(box_length=0.4, box_width=0.2, box_height=0.3, dr=0.005, hdr=
     normal[blk == 1] = (1, 0, 0, -1)
     normal[blk == 10] = (1, 0, 0, 1)
     normal[blk == 11] = (-1, 0, 0, 1)
-    codes[npb:] = (6, _abi.NONFIX, _abi.HYPO_CS, 0)
+    codes[npb:] = (6, _abi.NONFIX, int(model_type), 0)
     pos = np.zeros((n, 4), np.float32)
     pos[:npb, 0], pos[:npb, 1], pos[:npb, 2] = x[bidx], y[bidx], z[bidx]
     pos[npb:, 0], pos[npb:, 1], pos[npb:, 2] = x[sidx], y[sidx], z[sidx]

@@ -356,4 +356,102 @@ __device__ __forceinline__ void hypo_cs_apply(const DevParams* __restrict__ P, i
   sig = Sym{nd6.xx + sph + coh, nd6.yy + sph + coh, nd6.zz + sph + coh, nd6.xy, nd6.xz, nd6.yz};
 }
 
+// ------------------------------- DP, mu(I)-DP, basic hypoplasticity ----
+__device__ __forceinline__ Sym add_iso(Sym s, float p) { return Sym{s.xx + p, s.yy + p, s.zz + p, s.xy, s.xz, s.yz}; }
+
+// Elastic trial stress of both Drucker-Prager models:
+// sigma + (Jaumann + 2G D + lambda tr(D) I) dt with lambda = K - 2G/3.
+__device__ __forceinline__ Sym elastic_trial(float dt, float3 w, Sym D, const Sym& sig, float K, float G) {
+  const float lame = K - 2.0f / 3.0f * G;
+  return sadd(sig, smul(add_iso(sadd(jaumann(w, sig), smul(D, 2.0f * G)), lame * strace(D)), dt));
+}
+
+// cuMatModDP, constitutive_model.h:38-84: Drucker-Prager, elastic-perfectly
+// plastic with a tension cut-off.  mp = {K, G, k_phi, k_psi, k_c, ...}
+// (GetMaterial_DP_EPP, GenCase.cu:16-25).  Returns true on a shear return.
+__device__ __forceinline__ bool dp_epp(const float* __restrict__ mp, float dt, float3 w, Sym D, Sym& sig) {
+  const float K = mp[0], G = mp[1], kphi = mp[2], kpsi = mp[3], kc = mp[4];
+  const Sym pre = elastic_trial(dt, w, D, sig, K, G);
+  float sph = strace(pre) / 3.0f;
+  Sym dev = add_iso(pre, -(strace(pre) / 3.0f));
+  const float tenlt = kphi == 0.f ? 0.f : kc / kphi;
+  if (sph - tenlt >= 0.f) sph = tenlt;
+  const float tau = sqrtf(0.5f * sddot(dev, dev));
+  const float shr = tau - (-sph * kphi + kc);
+  bool yld = false;
+  if (shr > 0.f) {
+    const float dlam = shr / (G + K * kphi * kpsi);
+    sph = sph - K * kpsi * dlam;
+    const float ratio = tau < 1e-10f ? 1.0f : (kc - kphi * sph) / tau;
+    dev = smul(dev, ratio);
+    yld = true;
+  }
+  sig = add_iso(dev, sph);
+  return yld;
+}
+
+// cuMatModMuIDP, constitutive_model.h:90-130: Drucker-Prager with the mu(I)
+// friction law.  mp = {K, G, mu_s, mu_2, I0, rho_g, d, lambda, ...}
+// (GetMaterial_MuI_DP_EPP, GenCase.cu:27-40).
+__device__ __forceinline__ void mui_dp(const float* __restrict__ mp, float dt, float3 w, Sym D, Sym& sig) {
+  const float K = mp[0], G = mp[1], mus = mp[2], dmu = mp[3] - mp[2], I0 = mp[4], rhog = mp[5], dg = mp[6];
+  const float lam = mp[7];
+  const Sym pre = elastic_trial(dt, w, D, sig, K, G);
+  float sph = strace(pre) / 3.0f;
+  Sym dev = add_iso(pre, -(strace(pre) / 3.0f));
+  const Sym dD = add_iso(D, -(strace(D) / 3.0f));
+  const float gamma = sqrtf(2.0f * sddot(dD, dD));
+  if (sph > 0.f) sph = 0.f;
+  const float p = -sph;
+  const float mu = mus + dmu * gamma * dg / (I0 * sqrtf(p / rhog) + dg * sqrtf(gamma * gamma + lam * lam));
+  const float tau = sqrtf(0.5f * sddot(dev, dev));
+  if (tau + sph * mu > 0.f) {
+    const float ratio = tau < 1e-10f ? 1.0f : (0.0f - mu * sph) / tau;
+    dev = smul(dev, ratio);
+  }
+  sig = add_iso(dev, sph);
+}
+
+// cuMatModHypo (basic overload), constitutive_model.h:136-202.  mp = {Ei,
+// Vi, phi_c, c, p_ref, ...} (GetMaterial_HypoBasic, GenCase.cu:42-50); the
+// constants C1..C4 are the critical-state model's (same formulas).
+__device__ __forceinline__ void hypo_basic(const DevParams* __restrict__ P, int model, float dt, float3 w, Sym D,
+                                           Sym& sig) {
+  const float coh = P->model_par[model][3];
+  const HypoConst& hc = P->hypo[model];
+  Sym s = sig;
+  if (coh > 0.f) { s.xx = s.xx - coh; s.yy = s.yy - coh; s.zz = s.zz - coh; }
+  const Sym jau = jaumann(w, s);
+  const float I1 = strace(s);
+  if (I1 > -4.0f) { sig = Sym{-1.f, -1.f, -2.f, 0.f, 0.f, 0.f}; return; }
+  const Sym dev = add_iso(s, -(I1 / 3.0f));
+  const float trs = strace(s), trd = strace(D), trsd = sddot(s, D), nd = sqrtf(sddot(D, D));
+  const Sym rate = sadd(sadd(sadd(sadd(jau, smul(D, hc.C1 * trs)), smul(s, hc.C2 * trd)), smul(s, hc.C3 * trsd / trs)),
+                        smul(sadd(s, dev), hc.C4 * nd));
+  const Sym ns = sadd(sig, smul(rate, dt));
+  const float sph = strace(ns) / 3.0f;
+  if (sph > 0.f) { sig = Sym{-1.0f + coh, -1.0f + coh, -2.0f + coh, 0.f, 0.f, 0.f}; return; }
+  Sym nd6 = add_iso(ns, -sph);
+  const float kphi = 3.0f / (sqrtf(2.0f) * hc.sq2sin);
+  const float tau = sqrtf(0.5f * sddot(nd6, nd6));
+  if (tau - (-sph * kphi) > 0.f) {
+    const float ratio = tau < 1e-10f ? 1.0f : (-kphi * sph) / tau;
+    nd6 = smul(nd6, ratio);
+  }
+  sig = add_iso(nd6, sph + coh);
+}
+
+// The model dispatch of cuPredictorMaterial / cuCorrectorMaterial
+// (predictor_corrector.cu:72-75, 169-172) for every model but the
+// critical-state one, which the callers handle (exact or surrogate).
+// Returns true when a Drucker-Prager update yielded.
+__device__ __forceinline__ bool constitutive_other(const DevParams* __restrict__ P, int4 cd, float dt, float3 w, Sym D,
+                                                   Sym& sig) {
+  const float* mp = P->model_par[cd.w];
+  if (cd.z == LQ_DP_EPP) return dp_epp(mp, dt, w, D, sig);
+  if (cd.z == LQ_MUI_DP_EPP) mui_dp(mp, dt, w, D, sig);
+  else if (cd.z == LQ_HYPO_BASIC) hypo_basic(P, cd.w, dt, w, D, sig);
+  return false;
+}
+
 }  // namespace lq

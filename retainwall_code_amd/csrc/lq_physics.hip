@@ -690,13 +690,20 @@ __device__ __forceinline__ float predictor_slot(const DevParams* __restrict__ P,
   stsym(A.stn_a, A.stn_b, i, sadd(stn, smul(D, hdt)));
   Sym sig = {sa.x, sa.y, sa.z, sa.w, sb.x, sb.y};
   float Mcrt = 0.f;
-  if (cd.z == LQ_HYPO_CS && !P->sur_on) hypo_cs(P, cd.w, hdt, rho, ld3(B.spin[i]), D, sig, Mcrt);
+  bool yld = false;
+  if (cd.z == LQ_HYPO_CS) {
+    if (!P->sur_on) hypo_cs(P, cd.w, hdt, rho, ld3(B.spin[i]), D, sig, Mcrt);
+  } else {
+    yld = constitutive_other(P, cd, hdt, ld3(B.spin[i]), D, sig);
+  }
   stsym(A.str_a, A.str_b, i, sig);
   A.pos[i] = p;
   A.vel[i] = v;
-  float4 id = A.idm[i];
-  id.x = 100.f;
-  A.idm[i] = id;
+  if (!yld) {  // a Drucker-Prager yield keeps the old value (predictor_corrector.cu:78-83)
+    float4 id = A.idm[i];
+    id.x = 100.f;
+    A.idm[i] = id;
+  }
   const float dx = p.x - p0.x, dy = p.y - p0.y, dz = p.z - p0.z;
   return dx * dx + dy * dy + dz * dz;
 }
@@ -783,10 +790,15 @@ __global__ void __launch_bounds__(256) k_corrector_material(const DevParams* __r
   stsym(A.stn_a, A.stn_b, i, sadd(stn0, smul(D, dt)));
   Sym sig = ldsym(B.str_ini_a, B.str_ini_b, i);
   float Mcrt = 0.f;
-  if (cd.z == LQ_HYPO_CS && !P->sur_on) hypo_cs(P, cd.w, dt, rho, ld3(B.spin[i]), D, sig, Mcrt);
+  bool yld = false;
+  if (cd.z == LQ_HYPO_CS) {
+    if (!P->sur_on) hypo_cs(P, cd.w, dt, rho, ld3(B.spin[i]), D, sig, Mcrt);
+  } else {
+    yld = constitutive_other(P, cd, dt, ld3(B.spin[i]), D, sig);
+  }
   stsym(A.str_a, A.str_b, i, sig);
   float4 id = A.idm[i];
-  id.x = 100.f;
+  if (!yld) id.x = 100.f;  // predictor_corrector.cu:175-180
   id.w = Mcrt;
   if (!in_case_grid(P, A.case_id, i, ci)) { ci.w = P->ncell_delete_m; id.x = 999.f; }
   A.idm[i] = id;
