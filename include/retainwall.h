@@ -180,7 +180,10 @@ int lq_abi_sizes(size_t* sizes, int n);
 int lq_initialize_parameters(lq_params* par);
 
 /* LoquatSph ctor + AllocateGpuMemory + GridConfig (loquat.cu:29-69, 193-220;
- * loquat_grid.cu:59-68).  par must already be initialised. */
+ * loquat_grid.cu:59-68).  par must already be initialised.  With
+ * periodbound set the interaction variant must be REFERENCE_ORDER or
+ * NEIGHBOR_LIST (the variants that run the periodic image pass); batch
+ * contexts reject periodbound. */
 int lq_create(const lq_params* par, int device, lq_ctx** out);
 void lq_destroy(lq_ctx* ctx);
 const char* lq_last_error(const lq_ctx* ctx);
@@ -292,7 +295,8 @@ int lq_kernel_times(lq_ctx* ctx, double* ms, int64_t* launches, int n);
 int lq_set_tile(lq_ctx* ctx, int shape, int tz);
 /* Switch the interaction variant of a live context; xcd_remap != 0 makes
  * the particle-parallel variants hand each of the 8 XCDs one contiguous run
- * of particle blocks (hardware places block b on XCD b % 8).  No reference
+ * of particle blocks (hardware places block b on XCD b % 8).  A periodic
+ * context accepts REFERENCE_ORDER and NEIGHBOR_LIST only.  No reference
  * counterpart. */
 int lq_set_interaction(lq_ctx* ctx, int variant, int xcd_remap);
 /* PAIR_LIST capacity (entries per soil slot, 1..160; default 160) and skin

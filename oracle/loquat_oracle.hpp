@@ -101,6 +101,8 @@ template <class T> struct Par {
   T time, dt;
   int n_motion;
   Motion<T> mv[LQ_RIGID_MOTION_MAX];
+  int periodbound;
+  T pl[4], pr[4];  // periodic planes a x + b y + c z + d (planeleft, planeright)
 };
 
 // Copy the float parameter block (already initialised by the reference rule
@@ -139,6 +141,8 @@ template <class T> static void load_par(const lq_params* p, Par<T>& P) {
   P.art_p = p->artificial_pressure_coefficient_material;
   P.time = p->time; P.dt = p->dt;
   P.n_motion = p->n_motion;
+  P.periodbound = p->periodbound;
+  for (int k = 0; k < 4; k++) { P.pl[k] = p->planeleft[k]; P.pr[k] = p->planeright[k]; }
   for (int k = 0; k < LQ_RIGID_MOTION_MAX; k++) {
     const lq_rigid_motion& m = p->mv[k];
     Motion<T>& o = P.mv[k];
@@ -191,6 +195,52 @@ template <class T> static inline int linear_key(const Par<T>& P, v3<int> c) {
 }
 template <class T> static inline bool in_grid(const Par<T>& P, v3<int> c) {
   return c.x >= 0 && c.y >= 0 && c.z >= 0 && c.x < P.cnx && c.y < P.cny && c.z < P.cnz;
+}
+
+// ------------------------------------------------ periodic boundaries ----
+// Dist2Plane (functions.h:134-136): signed distance of a point to the plane
+// a x + b y + c z + d = 0.
+template <class T> static inline T plane_dist(v3<T> x, const T* pl) {
+  return (x.x * pl[0] + x.y * pl[1] + x.z * pl[2] + pl[3]) / std::sqrt(pl[0] * pl[0] + pl[1] * pl[1] + pl[2] * pl[2]);
+}
+template <class T> static inline v3<T> abs_pos(const Par<T>& P, v3<int> c, v3<T> p) {
+  return V3(p.x + T(c.x) * P.two_h, p.y + T(c.y) * P.two_h, p.z + T(c.z) * P.two_h);
+}
+// Cell and relative position of an absolute position: int3(x 0.5/h)
+// truncated, x - cell 2h.
+template <class T> static inline void to_cell(const Par<T>& P, v3<T> a, v3<int>& c, v3<T>& p) {
+  const T scale = T(0.5) * P.i_h;
+  c = {int(a.x * scale), int(a.y * scale), int(a.z * scale)};
+  p = V3(a.x - T(c.x) * P.two_h, a.y - T(c.y) * P.two_h, a.z - T(c.z) * P.two_h);
+}
+// The periodic image used by the *LoopPeriod helpers (adami_boundary.cu:
+// 73-87, field_smoothing.cu:58-72, particle_interaction.cu:243-258): a
+// particle closer than 2h to either plane is shifted by the period along
+// that plane's normal (left takes precedence) and its 27-cell loop is run
+// again from there.  The interaction's shift length is d_l + d_r as
+// written; Adami and Shepard take |d_l + d_r|.
+template <class T> static bool period_image(const Par<T>& P, v3<int> c, v3<T> p, bool abs_len, v3<int>& cpb,
+                                            v3<T>& ppb) {
+  const v3<T> a = abs_pos(P, c, p);
+  const bool left = std::fabs(plane_dist(a, P.pl)) < P.two_h;
+  const bool right = std::fabs(plane_dist(a, P.pr)) < P.two_h;
+  if (!left && !right) return false;
+  T len = P.pl[3] + P.pr[3];
+  if (abs_len) len = std::fabs(len);
+  const T* n = left ? P.pl : P.pr;
+  to_cell(P, V3(a.x + n[0] * len, a.y + n[1] * len, a.z + n[2] * len), cpb, ppb);
+  return true;
+}
+// cuCheckOutDomainPeriod (predictor_corrector.cu:26-40): a particle behind
+// either plane re-enters through the other one.
+template <class T> static void wrap_period(const Par<T>& P, v3<int>& c, v3<T>& p) {
+  const v3<T> a = abs_pos(P, c, p);
+  const bool left = plane_dist(a, P.pl) < T(0);
+  const bool right = plane_dist(a, P.pr) < T(0);
+  if (!left && !right) return;
+  const T len = std::fabs(P.pl[3] + P.pr[3]);
+  const T* n = left ? P.pl : P.pr;
+  to_cell(P, V3(a.x + n[0] * len, a.y + n[1] * len, a.z + n[2] * len), c, p);
 }
 
 // ------------------------------------------------------------- a2: frames --
@@ -351,6 +401,7 @@ template <class T> static void adami(const Par<T>& P, State<T>& S) {
     v3<T> rpos = {0, 0, 0};
     T rker = 0;
     T rv[4] = {0, 0, 0, 0};
+    auto scan = [&](v3<int> ci, v3<T> pi) {
     for (int cz = -1; cz <= 1; cz++)
       for (int cy = -1; cy <= 1; cy++)
         for (int cx = -1; cx <= 1; cx++) {
@@ -382,6 +433,11 @@ template <class T> static void adami(const Par<T>& P, State<T>& S) {
             }
           }
         }
+    };
+    scan(ci, pi);
+    v3<int> cpb;
+    v3<T> ppb;
+    if (P.periodbound && period_image(P, ci, pi, true, cpb, ppb)) scan(cpb, ppb);
     T nn = norm(n);
     if (nn > T(0)) {
       S.cmag[9 * i + 3] = n.x / nn; S.cmag[9 * i + 4] = n.y / nn; S.cmag[9 * i + 5] = n.z / nn;
@@ -478,6 +534,7 @@ template <class T> static void shepard(const Par<T>& P, State<T>& S) {
     v3<int> ci = cell_of(S.cellpos, i);
     sym6<T> acc = {0, 0, 0, 0, 0, 0};
     T sk = 0;
+    auto scan = [&](v3<int> ci, v3<T> pi) {
     for (int cz = -1; cz <= 1; cz++)
       for (int cy = -1; cy <= 1; cy++)
         for (int cx = -1; cx <= 1; cx++) {
@@ -497,6 +554,11 @@ template <class T> static void shepard(const Par<T>& P, State<T>& S) {
             acc = acc + ld6(S.str, j) * wv;
           }
         }
+    };
+    scan(ci, pi);
+    v3<int> cpb;
+    v3<T> ppb;
+    if (P.periodbound && period_image(P, ci, pi, true, cpb, ppb)) scan(cpb, ppb);
     if (sk != T(0)) acc = acc / sk;
     out[idx] = acc;
   }
@@ -527,6 +589,7 @@ template <class T> static void interaction(const Par<T>& P, State<T>& S) {
     T accden = 0;
     v3<T> as = {0, 0, 0}, av = {0, 0, 0}, aa = {0, 0, 0};
     T gv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // xx xy xz yx yy yz zx zy zz
+    auto scan = [&](v3<int> ci, v3<T> pi) {
     for (int cz = -1; cz <= 1; cz++)
       for (int cy = -1; cy <= 1; cy++)
         for (int cx = -1; cx <= 1; cx++) {
@@ -585,6 +648,11 @@ template <class T> static void interaction(const Par<T>& P, State<T>& S) {
             }
           }
         }
+    };
+    scan(ci, pi);
+    v3<int> cpb;
+    v3<T> ppb;
+    if (P.periodbound && period_image(P, ci, pi, false, cpb, ppb)) scan(cpb, ppb);
     v3<T> damp = vi * P.damping;
     v3<T> s3 = (as + av) + aa;
     T mi = S.idm[4 * i + 1];
@@ -935,6 +1003,13 @@ template <class T> static int corrector_material(const Par<T>& P, State<T>& S) {
       d[k] = rebin_d(S.pos[4 * i + k], P.two_h);
       S.pos[4 * i + k] = S.pos[4 * i + k] - T(d[k]) * P.two_h;
       S.cellpos[4 * i + k] += d[k];
+    }
+    if (P.periodbound) {  // :158
+      v3<int> c = cell_of(S.cellpos, i);
+      v3<T> pr = ld3(S.pos, i);
+      wrap_period(P, c, pr);
+      st3(S.pos, i, pr);
+      S.cellpos[4 * i] = c.x; S.cellpos[4 * i + 1] = c.y; S.cellpos[4 * i + 2] = c.z;
     }
     sym6<T> D = ld6(S.acestn, i);
     st6(S.stn, i, ld6(S.stn_ini, i) + D * P.dt);

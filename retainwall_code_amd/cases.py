@@ -149,7 +149,8 @@ def retaining_wall(box_length=0.4, box_width=0.2, box_height=0.3, dr=0.005, hdr=
                    fix_wall_height=0.04, wall_extra=0.05, rho0=1600.0, material=None,
                    cs=65.0, cfl=0.1, alpha_visc=1.5, art_p=0.01, gravity_z=-9.81, K0=0.5,
                    motion=("translate", -0.001), jitter=0.0, void_range=None, seed=0,
-                   time_max=1.0, shepard=0, model_type=_abi.HYPO_CS, name="retaining_wall") -> Case:
+                   time_max=1.0, shepard=0, model_type=_abi.HYPO_CS, periodic_y=False,
+                   name="retaining_wall") -> Case:
     """Build a GenCase case-1 style wall case.
 
     box_* are the backfill dimensions (x away from the wall, y along it, z up).
@@ -161,6 +162,8 @@ def retaining_wall(box_length=0.4, box_width=0.2, box_height=0.3, dr=0.005, hdr=
     model_type is the soil's constitutive model (codes.z, types.h:51); pass
     the matching parameter row as material (dp_soil, mui_dp_soil,
     hypo_basic_soil) for any model other than the default HypoCS.
+    periodic_y makes the case periodic along y (the wall-parallel direction,
+    which has no walls): planes y = 0 and y = npy dr, one lattice period.
     """
     dr = f32(dr)
     hdr = f32(hdr)
@@ -239,7 +242,7 @@ def retaining_wall(box_length=0.4, box_width=0.2, box_height=0.3, dr=0.005, hdr=
     mt = motion_table(mode, speed, rp)
     dt = float(f32(cfl) * (f32(dr * hdr) / f32(cs)))
     params = {
-        "dim": 3, "periodbound": 0, "n_motion": 1, "np": n, "npb": npb, "npm": npm,
+        "dim": 3, "periodbound": int(bool(periodic_y)), "n_motion": 1, "np": n, "npb": npb, "npm": npm,
         "np_delete_m": 0, "domain_size": [float(dom_x), float(dom_y), float(dom_z)],
         "domain_min": [0.0, 0.0, 0.0], "dr": float(dr), "hdr": float(hdr),
         "gravity": [0.0, 0.0, gravity_z], "rho0_m": float(rho0), "damping": 0.0, "n_model": 1,
@@ -249,6 +252,9 @@ def retaining_wall(box_length=0.4, box_width=0.2, box_height=0.3, dr=0.005, hdr=
         "artificial_pressure_coefficient_material": art_p, "time_max": time_max, "time": 0.0,
         "dt": dt, "output_frequency": time_max / 100.0, "output_number": 0, "output_format": 2,
     }
+    if periodic_y:  # planeleft n = +y through y = 0, planeright n = -y through y = npy dr
+        params["planeleft"] = [0.0, 1.0, 0.0, 0.0]
+        params["planeright"] = [0.0, -1.0, 0.0, float(f32(npy * float(dr)))]
     # wall-force probe box (utilities.cu:619-621 with the case's own numbers)
     box = (float(fwh + bt + f32(3.0) * dr), float(bt + fwh), float(bt + fwh + f32(1.0)))
     return Case(params, codes, pos, vel, idm, stress, np.zeros((n, 6), np.float32), normal,

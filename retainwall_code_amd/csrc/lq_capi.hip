@@ -276,6 +276,9 @@ static void fill_static(const lq_params& p, DevParams& d) {
   d.band = p.dr * 1.0001f;      // particle_interaction.cu:147
   d.half_dr = p.dr / 2;         // particle_interaction.cu:157
   d.mu = 0.58108f;              // particle_interaction.cu:188-189
+  d.periodbound = p.periodbound ? 1 : 0;
+  d.plane_l = make_float4(p.planeleft[0], p.planeleft[1], p.planeleft[2], p.planeleft[3]);
+  d.plane_r = make_float4(p.planeright[0], p.planeright[1], p.planeright[2], p.planeright[3]);
   d.npm = p.npm; d.np_delete_m = p.np_delete_m;
   d.time = p.time; d.dt = p.dt; d.time_d = p.time; d.step = 0;
   int maxkey = p.ncell_delete_m, nb = 0;
@@ -383,7 +386,12 @@ static int create_impl(const lq_params* cases, int n_cases, int device, lq_ctx**
   for (int k = 0; k < n_cases; k++) {
     const lq_params& p = cases[k];
     if (p.np <= 0 || p.npb < 0 || p.npm < 0 || p.np != p.npb + p.npm) return LQ_ERR_ARG;  // loquat_io.cu:148-151
-    if (p.dim != 3 || p.periodbound) return LQ_ERR_ARG;
+    if (p.dim != 3) return LQ_ERR_ARG;
+    // periodic planes are one case's absolute coordinates: single-case
+    // contexts only, with the interaction variants that run the image pass
+    if (p.periodbound && (n_cases > 1 || p.interaction_variant == LQ_INTERACTION_TILE ||
+                          p.interaction_variant == LQ_INTERACTION_PAIR_LIST))
+      return LQ_ERR_ARG;
     if (p.key_mode != LQ_KEY_LINEAR) return LQ_ERR_ARG;
     if (p.n_motion < 0 || p.n_motion > LQ_RIGID_MOTION_MAX) return LQ_ERR_ARG;
     np_tot += p.np;
@@ -1045,14 +1053,14 @@ static void enq_interaction_kernel(lq_ctx* c) {
   if (c->par.interaction_variant == LQ_INTERACTION_PAIR_LIST) {
     launch_interaction_plist(c->stream, c->dP, c->npm0, c->B.cur, c->B, c->B.cell_beg, c->B.cell_end, c->xcd_remap);
     launch_interaction(c->stream, c->dP, c->npm0, c->B.cur, c->B, c->B.cell_beg, c->B.cell_end,
-                       LQ_INTERACTION_PAIR_LIST, c->xcd_remap);  // guarded cell-scan fallback
+                       LQ_INTERACTION_PAIR_LIST, c->xcd_remap, false);  // guarded cell-scan fallback
   } else if (c->par.interaction_variant == LQ_INTERACTION_TILE) {
     if (c->npm0 > 0)
       launch_interaction_tile(c->stream, c->dP, c->par.cell_num[0], c->par.cell_num[1], c->par.cell_num[2], c->B.cur,
                               c->B, c->B.cell_beg, c->B.cell_end, c->tile_shape, c->tile_tz);
   } else {
     launch_interaction(c->stream, c->dP, c->npm0, c->B.cur, c->B, c->B.cell_beg, c->B.cell_end,
-                       c->par.interaction_variant, c->xcd_remap);
+                       c->par.interaction_variant, c->xcd_remap, c->par.periodbound != 0);
   }
 }
 static void enq_plist_build(lq_ctx* c) {
@@ -1513,6 +1521,8 @@ extern "C" int lq_set_interaction(lq_ctx* c, int variant, int xcd_remap) {
     return fail(c, LQ_ERR_ARG, "lq_set_interaction: variant out of range");
   if (variant == LQ_INTERACTION_PAIR_LIST && !c->B.plist)
     return fail(c, LQ_ERR_ARG, "lq_set_interaction: pair lists need fewer than 2^25 particles");
+  if (c->par.periodbound && (variant == LQ_INTERACTION_TILE || variant == LQ_INTERACTION_PAIR_LIST))
+    return fail(c, LQ_ERR_ARG, "lq_set_interaction: periodic boundaries need REFERENCE_ORDER or NEIGHBOR_LIST");
   c->par.interaction_variant = variant;
   c->xcd_remap = xcd_remap ? 1 : 0;
   c->layout_version++;  // re-capture graphs
@@ -1577,9 +1587,9 @@ extern "C" int lq_autotune(lq_ctx* c, int reps, int* variant_out, int* shape_out
   for (int remap = 0; remap < 2; remap++) {
     cands.push_back({LQ_INTERACTION_REFERENCE_ORDER, c->tile_shape, c->tile_tz, remap});
     cands.push_back({LQ_INTERACTION_NEIGHBOR_LIST, c->tile_shape, c->tile_tz, remap});
-    if (c->B.plist) cands.push_back({LQ_INTERACTION_PAIR_LIST, c->tile_shape, c->tile_tz, remap});
+    if (c->B.plist && !c->par.periodbound) cands.push_back({LQ_INTERACTION_PAIR_LIST, c->tile_shape, c->tile_tz, remap});
   }
-  for (int sh = 0; sh < interaction_tile_shapes(); sh++) {
+  for (int sh = 0; sh < (c->par.periodbound ? 0 : interaction_tile_shapes()); sh++) {
     const int waves = interaction_tile_waves(sh);
     const int base = std::max(1, (waves * 64 * 9) / (10 * 16));  // ~90% of the lanes at ~16 soil per cell
     for (int tz = std::max(1, base - 2); tz <= std::min(interaction_tile_maxtz(sh), base + 2); tz++)

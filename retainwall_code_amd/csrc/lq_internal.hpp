@@ -53,7 +53,7 @@ struct DevCase {
 struct DevParams {
   // ---- static (set at create) ----
   int np, npb, cnx, cny, cnz, ncell, ncell_material, ncell_delete_m;
-  int key_mode, shepard, viscosity_type, n_motion, n_model, nbits, pad0, pad1;
+  int key_mode, shepard, viscosity_type, n_motion, n_model, nbits, periodbound, pad1;
   float h, i_h, dr, vol_def, bt, eta, two_h, r2cut;
   float gx, gy, gz, damping, knorm, kgrad, kernel_zero, kernel_dr;
   float art_exp, alpha_cs, art_p, cfl, cs, band, half_dr, mu;
@@ -63,6 +63,7 @@ struct DevParams {
   float r2list, disp2_lim;
   int plist_cap, plist_stride;
   int n_cases, pad_c0, pad_c1, pad_c2;
+  float4 plane_l, plane_r;  // periodic planes a x + b y + c z + d = 0 (planeleft, planeright)
   const float (*model_par)[LQ_MATERIAL_SIZE_MAX];  // [n_model] material rows (device table)
   const HypoConst* hypo;                           // [n_model]
   const DevCase* cases;                            // [n_cases]
@@ -157,6 +158,51 @@ __device__ __forceinline__ bool in_grid(const DevParams* P, int cx, int cy, int 
   return cx >= 0 && cy >= 0 && cz >= 0 && cx < P->cnx && cy < P->cny && cz < P->cnz;
 }
 __device__ __forceinline__ void set_err(DevParams* P, int code) { atomicCAS(&P->err, 0, code); }
+
+// ------------------------------------------------ periodic boundaries ----
+// Dist2Plane (functions.h:134-136): signed distance to a x + b y + c z + d = 0.
+__device__ __forceinline__ float plane_dist(float3 x, float4 pl) {
+  return (x.x * pl.x + x.y * pl.y + x.z * pl.z + pl.w) / sqrtf(pl.x * pl.x + pl.y * pl.y + pl.z * pl.z);
+}
+// Cell and relative position of an absolute position (int3(x 0.5/h)
+// truncated, x - cell 2h), single-case grids.
+__device__ __forceinline__ void to_cell(const DevParams* P, float3 a, int4& c, float3& p) {
+  const float scale = 0.5f * P->i_h, two_h = P->two_h;
+  c.x = int(a.x * scale); c.y = int(a.y * scale); c.z = int(a.z * scale);
+  p = make_float3(a.x - float(c.x) * two_h, a.y - float(c.y) * two_h, a.z - float(c.z) * two_h);
+}
+__device__ __forceinline__ float3 abs_from_cell(const DevParams* P, int4 c, float3 p) {
+  const float two_h = P->two_h;
+  return make_float3(p.x + float(c.x) * two_h, p.y + float(c.y) * two_h, p.z + float(c.z) * two_h);
+}
+// The periodic image of the *LoopPeriod helpers (particle_interaction.cu:
+// 243-258, adami_boundary.cu:73-87, field_smoothing.cu:58-72): a slot closer
+// than 2h to either plane is shifted by the period along that plane's
+// normal (left first) and its 27-cell loop runs again from there.  The
+// interaction takes d_l + d_r as the period, Adami and Shepard |d_l + d_r|.
+__device__ __forceinline__ bool period_image(const DevParams* P, int4 c, float3 p, bool abs_len, int4& cpb,
+                                             float3& ppb) {
+  const float3 a = abs_from_cell(P, c, p);
+  const bool left = fabsf(plane_dist(a, P->plane_l)) < P->two_h;
+  const bool right = fabsf(plane_dist(a, P->plane_r)) < P->two_h;
+  if (!left && !right) return false;
+  float len = P->plane_l.w + P->plane_r.w;
+  if (abs_len) len = fabsf(len);
+  const float4 n = left ? P->plane_l : P->plane_r;
+  to_cell(P, make_float3(a.x + n.x * len, a.y + n.y * len, a.z + n.z * len), cpb, ppb);
+  return true;
+}
+// cuCheckOutDomainPeriod (predictor_corrector.cu:26-40): a slot behind
+// either plane re-enters through the other one (cell.w kept).
+__device__ __forceinline__ void wrap_period(const DevParams* P, int4& c, float3& p) {
+  const float3 a = abs_from_cell(P, c, p);
+  const bool left = plane_dist(a, P->plane_l) < 0.f;
+  const bool right = plane_dist(a, P->plane_r) < 0.f;
+  if (!left && !right) return;
+  const float len = fabsf(P->plane_l.w + P->plane_r.w);
+  const float4 n = left ? P->plane_l : P->plane_r;
+  to_cell(P, make_float3(a.x + n.x * len, a.y + n.y * len, a.z + n.z * len), c, p);
+}
 // Cell coordinates of a particle in its own case's grid.
 __device__ __forceinline__ int3 case_cell(const DevParams* P, const int* case_id, int i, int4 c) {
   const DevCase& k = P->cases[case_id[i]];

@@ -24,7 +24,7 @@ void launch_interaction_plist(hipStream_t s, DevParams* P, int npm_max, Arrays A
                               const int* ce, int xcd_remap);
 void launch_pair_prep(hipStream_t s, const DevParams* P, int n, Arrays A, const Buffers& B);
 void launch_interaction(hipStream_t s, DevParams* P, int npm_max, Arrays A, const Buffers& B, const int* cb,
-                        const int* ce, int variant, int xcd_remap);
+                        const int* ce, int variant, int xcd_remap, bool periodic);
 void launch_contact(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const Buffers& B, const int* cb,
                     const int* ce);
 void launch_predictor(hipStream_t s, DevParams* P, int np, Arrays A, const Buffers& B, int track_disp);

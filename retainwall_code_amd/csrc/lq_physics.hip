@@ -142,60 +142,68 @@ __global__ void __launch_bounds__(64) k_adami(const DevParams* __restrict__ P, A
   float3 rpos = make_float3(0.f, 0.f, 0.f);
   float rker = 0.f;
   float4 rv = make_float4(0.f, 0.f, 0.f, 0.f);
-  int bw = 0, ew = 0, bs = 0, es = 0;  // wall and soil ranges of cell k, fetched one cell ahead
-  if (in_grid(P, ci.x - 1, ci.y - 1, ci.z - 1)) {
-    const int c = lin_key(P, ci.x - 1, ci.y - 1, ci.z - 1);
-    if (do_normal) { bw = cbeg[c]; ew = cend[c]; }
-    if (do_soil) { bs = cbeg[c + ncm]; es = cend[c + ncm]; }
-  }
+  // the 27-cell loop around (ci, pi); the periodic image pass
+  // (adami_boundary.cu:73-87, 103) runs it again from the shifted position
+  auto scan = [&](int4 ci, float3 pi) {
+    int bw = 0, ew = 0, bs = 0, es = 0;  // wall and soil ranges of cell k, fetched one cell ahead
+    if (in_grid(P, ci.x - 1, ci.y - 1, ci.z - 1)) {
+      const int c = lin_key(P, ci.x - 1, ci.y - 1, ci.z - 1);
+      if (do_normal) { bw = cbeg[c]; ew = cend[c]; }
+      if (do_soil) { bs = cbeg[c + ncm]; es = cend[c + ncm]; }
+    }
 #pragma unroll 1
-  for (int k = 0; k < 27; k++) {
-    const int cz = k / 9 - 1, cy = (k / 3) % 3 - 1, cx = k % 3 - 1;
-    int bwn = 0, ewn = 0, bsn = 0, esn = 0;
-    if (k < 26) {
-      const int kn = k + 1;
-      const int nx = ci.x + kn % 3 - 1, ny = ci.y + (kn / 3) % 3 - 1, nz = ci.z + kn / 9 - 1;
-      if (in_grid(P, nx, ny, nz)) {
-        const int c = lin_key(P, nx, ny, nz);
-        if (do_normal) { bwn = cbeg[c]; ewn = cend[c]; }
-        if (do_soil) { bsn = cbeg[c + ncm]; esn = cend[c + ncm]; }
+    for (int k = 0; k < 27; k++) {
+      const int cz = k / 9 - 1, cy = (k / 3) % 3 - 1, cx = k % 3 - 1;
+      int bwn = 0, ewn = 0, bsn = 0, esn = 0;
+      if (k < 26) {
+        const int kn = k + 1;
+        const int nx = ci.x + kn % 3 - 1, ny = ci.y + (kn / 3) % 3 - 1, nz = ci.z + kn / 9 - 1;
+        if (in_grid(P, nx, ny, nz)) {
+          const int c = lin_key(P, nx, ny, nz);
+          if (do_normal) { bwn = cbeg[c]; ewn = cend[c]; }
+          if (do_soil) { bsn = cbeg[c + ncm]; esn = cend[c + ncm]; }
+        }
       }
-    }
-    for (int j0 = bw; j0 < ew; j0 += 32) {  // normal integral over wall neighbours
-      unsigned mask = hit_mask(A.pos, pi, j0, min(ew - j0, 32), cx, cy, cz, two_h, r2cut);
-      while (mask) {
-        const int j = j0 + __ffs(mask) - 1;
-        mask &= mask - 1u;
-        float3 d;
-        float r = sqrtf(pair_r2(pi, A.pos[j], cx, cy, cz, two_h, d));
-        float q = r * i_h;
-        float dwdr = kG(P->kgrad, q);
-        float3 g = f3mul(d, dwdr);
-        n = f3sub(n, f3mul(g, P->vol_def));
+      for (int j0 = bw; j0 < ew; j0 += 32) {  // normal integral over wall neighbours
+        unsigned mask = hit_mask(A.pos, pi, j0, min(ew - j0, 32), cx, cy, cz, two_h, r2cut);
+        while (mask) {
+          const int j = j0 + __ffs(mask) - 1;
+          mask &= mask - 1u;
+          float3 d;
+          float r = sqrtf(pair_r2(pi, A.pos[j], cx, cy, cz, two_h, d));
+          float q = r * i_h;
+          float dwdr = kG(P->kgrad, q);
+          float3 g = f3mul(d, dwdr);
+          n = f3sub(n, f3mul(g, P->vol_def));
+        }
       }
-    }
-    for (int j0 = bs; j0 < es; j0 += 32) {  // extrapolation from soil neighbours
-      unsigned mask = hit_mask(A.pos, pi, j0, min(es - j0, 32), cx, cy, cz, two_h, r2cut);
-      while (mask) {
-        const int j = j0 + __ffs(mask) - 1;
-        mask &= mask - 1u;
-        float3 d;
-        float r = sqrtf(pair_r2(pi, A.pos[j], cx, cy, cz, two_h, d));
-        float q = r * i_h;
-        float ker = kW(P->knorm, q);
-        float4 idmj = A.idm[j];
-        float4 vj = A.vel[j];
-        float volj = idmj.y / vj.w;
-        float wv = ker * volj;
-        Sym sj = ldsym(A.str_a, A.str_b, j);
-        rs = sadd(rs, smul(sj, wv));
-        rpos = f3add(rpos, f3mul(d, ker * idmj.y));
-        rker = rker + wv;
-        rv = make_float4(rv.x + vj.x * wv, rv.y + vj.y * wv, rv.z + vj.z * wv, rv.w + vj.w * wv);
+      for (int j0 = bs; j0 < es; j0 += 32) {  // extrapolation from soil neighbours
+        unsigned mask = hit_mask(A.pos, pi, j0, min(es - j0, 32), cx, cy, cz, two_h, r2cut);
+        while (mask) {
+          const int j = j0 + __ffs(mask) - 1;
+          mask &= mask - 1u;
+          float3 d;
+          float r = sqrtf(pair_r2(pi, A.pos[j], cx, cy, cz, two_h, d));
+          float q = r * i_h;
+          float ker = kW(P->knorm, q);
+          float4 idmj = A.idm[j];
+          float4 vj = A.vel[j];
+          float volj = idmj.y / vj.w;
+          float wv = ker * volj;
+          Sym sj = ldsym(A.str_a, A.str_b, j);
+          rs = sadd(rs, smul(sj, wv));
+          rpos = f3add(rpos, f3mul(d, ker * idmj.y));
+          rker = rker + wv;
+          rv = make_float4(rv.x + vj.x * wv, rv.y + vj.y * wv, rv.z + vj.z * wv, rv.w + vj.w * wv);
+        }
       }
+      bw = bwn; ew = ewn; bs = bsn; es = esn;
     }
-    bw = bwn; ew = ewn; bs = bsn; es = esn;
-  }
+  };
+  scan(ci, pi);
+  int4 cpb;
+  float3 ppb;
+  if (P->periodbound && period_image(P, ci, pi, true, cpb, ppb)) scan(cpb, ppb);
   if (do_normal) {
     float nn = f3norm(n);
     if (nn > 0.f) A.cn[i] = make_float4(n.x / nn, n.y / nn, n.z / nn, 0.f);
@@ -333,25 +341,31 @@ __global__ void __launch_bounds__(256) k_shepard(const DevParams* __restrict__ P
   const float r2cut = P->r2cut;
   Sym acc = {0, 0, 0, 0, 0, 0};
   float sk = 0.f;
-  for (int cz = -1; cz <= 1; cz++)
-    for (int cy = -1; cy <= 1; cy++)
-      for (int cx = -1; cx <= 1; cx++) {
-        int jx = ci.x + cx, jy = ci.y + cy, jz = ci.z + cz;
-        if (!in_grid(P, jx, jy, jz)) continue;
-        int c = lin_key(P, jx, jy, jz) + P->ncell_material;
-        for (int j = cbeg[c], e = cend[c]; j < e; j++) {
-          float3 d;
-          const float r2 = pair_r2(pi, A.pos[j], cx, cy, cz, P->two_h, d);
-          if (!(r2 <= r2cut)) continue;
-          float r = sqrtf(r2);
-          float q = r * P->i_h;
-          float ker = kW(P->knorm, q);
-          float vol = A.idm[j].y / A.vel[j].w;
-          float wv = ker * vol;
-          sk = sk + wv;
-          acc = sadd(acc, smul(ldsym(A.str_a, A.str_b, j), wv));
+  auto scan = [&](int4 ci, float3 pi) {
+    for (int cz = -1; cz <= 1; cz++)
+      for (int cy = -1; cy <= 1; cy++)
+        for (int cx = -1; cx <= 1; cx++) {
+          int jx = ci.x + cx, jy = ci.y + cy, jz = ci.z + cz;
+          if (!in_grid(P, jx, jy, jz)) continue;
+          int c = lin_key(P, jx, jy, jz) + P->ncell_material;
+          for (int j = cbeg[c], e = cend[c]; j < e; j++) {
+            float3 d;
+            const float r2 = pair_r2(pi, A.pos[j], cx, cy, cz, P->two_h, d);
+            if (!(r2 <= r2cut)) continue;
+            float r = sqrtf(r2);
+            float q = r * P->i_h;
+            float ker = kW(P->knorm, q);
+            float vol = A.idm[j].y / A.vel[j].w;
+            float wv = ker * vol;
+            sk = sk + wv;
+            acc = sadd(acc, smul(ldsym(A.str_a, A.str_b, j), wv));
+          }
         }
-      }
+  };
+  scan(ci, pi);
+  int4 cpb;
+  float3 ppb;
+  if (P->periodbound && period_image(P, ci, pi, true, cpb, ppb)) scan(cpb, ppb);  // field_smoothing.cu:58-72, 85
   if (sk != 0.f) acc = sdiv(acc, sk);
   stsym(oa, ob, i, acc);
 }
@@ -413,26 +427,33 @@ __global__ void __launch_bounds__(256) k_interaction_ref(DevParams* __restrict__
   S.as = S.av = S.aa = make_float3(0.f, 0.f, 0.f);
 #pragma unroll
   for (int k = 0; k < 9; k++) S.gv[k] = 0.f;
-  for (int cz = -1; cz <= 1; cz++)
-    for (int cy = -1; cy <= 1; cy++)
-      for (int cx = -1; cx <= 1; cx++) {
-        int jx = ci.x + cx, jy = ci.y + cy, jz = ci.z + cz;
-        if (!in_grid(P, jx, jy, jz)) continue;
-        int cb = lin_key(P, jx, jy, jz);
-        for (int pass = 0; pass < 2; pass++) {
-          const bool bound = pass == 0;
-          int c = bound ? cb : cb + P->ncell_material;
-          for (int j = cbeg[c], e = cend[c]; j < e; j++) {
-            float3 d;
-            const float4 pj = A.pos[j];
-            const float r2 = pair_r2(pi, pj, cx, cy, cz, two_h, d);
-            if (!(r2 <= r2cut)) continue;
-            float r = sqrtf(r2);
-            float q = r * i_h;
-            pair_terms(K, S, d, r, q, vi, rhoi, sri, artpi, shift, bound, A.vel[j], pj.w, rec_sra(B, j), rec_srb(B, j));
+  auto scan = [&](int4 ci, float3 pi) {
+    for (int cz = -1; cz <= 1; cz++)
+      for (int cy = -1; cy <= 1; cy++)
+        for (int cx = -1; cx <= 1; cx++) {
+          int jx = ci.x + cx, jy = ci.y + cy, jz = ci.z + cz;
+          if (!in_grid(P, jx, jy, jz)) continue;
+          int cb = lin_key(P, jx, jy, jz);
+          for (int pass = 0; pass < 2; pass++) {
+            const bool bound = pass == 0;
+            int c = bound ? cb : cb + P->ncell_material;
+            for (int j = cbeg[c], e = cend[c]; j < e; j++) {
+              float3 d;
+              const float4 pj = A.pos[j];
+              const float r2 = pair_r2(pi, pj, cx, cy, cz, two_h, d);
+              if (!(r2 <= r2cut)) continue;
+              float r = sqrtf(r2);
+              float q = r * i_h;
+              pair_terms(K, S, d, r, q, vi, rhoi, sri, artpi, shift, bound, A.vel[j], pj.w, rec_sra(B, j),
+                         rec_srb(B, j));
+            }
           }
         }
-      }
+  };
+  scan(ci, pi);
+  int4 cpb;
+  float3 ppb;
+  if (P->periodbound && period_image(P, ci, pi, false, cpb, ppb)) scan(cpb, ppb);  // particle_interaction.cu:243-258
   interaction_epilogue(P, A, B, i, S, vi);
 }
 
@@ -490,37 +511,15 @@ __device__ __forceinline__ void nl_drain(const PairConst& K, IAcc& S, const Arra
   }
 }
 
-// At most 96 VGPRs (5 waves per SIMD, 20 per CU, which the 8 KB of ring per
-// wave also allows): the kernel is bound by its neighbour gathers and needs
-// the waves to keep them in flight.
-__global__ void __launch_bounds__(NL_WAVE * NL_WAVES, 5) k_interaction_nl(DevParams* __restrict__ P, Arrays A, Buffers B,
-                                                                      const int* __restrict__ cbeg,
-                                                                      const int* __restrict__ cend, int remap) {
-  __shared__ int s_list[NL_WAVES][NL_CAP][NL_WAVE];
-  const int lane = threadIdx.x & (NL_WAVE - 1);
-  const int idx = xcd_block(remap) * blockDim.x + threadIdx.x;
-  const bool active = idx < P->npm;
-  const int i = P->npb + (active ? idx : 0);
-  const float3 pi = ld3(A.pos[i]);
-  const int4 ci = A.cellpos[i];
-  const float4 vri = A.vel[i];
-  const float3 vi = ld3(vri);
-  const float rhoi = vri.w;
-  const float4 sria = rec_sra(B, i), srib = rec_srb(B, i);
-  const Sym sri = {sria.x, sria.y, sria.z, sria.w, srib.x, srib.y};
-  const float tri = srib.w;
-  const float artpi = tri > 0.f ? tri * P->art_p : 0.f;
-  const int shift = int(A.cvar[i].z);
+// The wave's scan of the 27 cells x {wall, soil} ranges around (ci, pi)
+// into the per-lane rings (see above).  Lanes with active unset have empty
+// ranges; the loop indices stay wave-uniform.
+__device__ __forceinline__ void nl_scan(const DevParams* __restrict__ P, const PairConst& K, IAcc& S, const Arrays& A,
+                                        const Buffers& B, const int* __restrict__ cbeg, const int* __restrict__ cend,
+                                        int (*lst)[NL_WAVE], int lane, int& head, int& cnt, int4 ci, float3 pi,
+                                        bool active, float3 vi, float rhoi, Sym sri, float artpi, int shift) {
   const float two_h = P->two_h, i_h = P->i_h, r2cut = P->r2cut;
   const int ncm = P->ncell_material;
-  const PairConst K = pair_const(P);
-  int (*lst)[NL_WAVE] = s_list[threadIdx.x / NL_WAVE];
-  IAcc S;
-  S.accden = 0.f;
-  S.as = S.av = S.aa = make_float3(0.f, 0.f, 0.f);
-#pragma unroll
-  for (int k = 0; k < 9; k++) S.gv[k] = 0.f;
-  int head = 0, cnt = 0;  // ring: oldest entry, number of entries
 #pragma unroll 1
   for (int r = 0; r < 54; r++) {
     const int cell = r >> 1, pass = r & 1;
@@ -564,7 +563,54 @@ __global__ void __launch_bounds__(NL_WAVE * NL_WAVES, 5) k_interaction_nl(DevPar
       j += m;
     }
   }
+}
+
+// At most 96 VGPRs (5 waves per SIMD, 20 per CU, which the 8 KB of ring per
+// wave also allows): the kernel is bound by its neighbour gathers and needs
+// the waves to keep them in flight.  PERIODIC adds the image pass of
+// cuParticleInteractionMaterialLoopPeriod (particle_interaction.cu:243-258)
+// after the rings are drained, so each particle still sums its own-position
+// pairs first and its image pairs after them, in reference order.
+template <bool PERIODIC>
+__global__ void __launch_bounds__(NL_WAVE * NL_WAVES, 5) k_interaction_nl(DevParams* __restrict__ P, Arrays A, Buffers B,
+                                                                      const int* __restrict__ cbeg,
+                                                                      const int* __restrict__ cend, int remap) {
+  __shared__ int s_list[NL_WAVES][NL_CAP][NL_WAVE];
+  const int lane = threadIdx.x & (NL_WAVE - 1);
+  const int idx = xcd_block(remap) * blockDim.x + threadIdx.x;
+  const bool active = idx < P->npm;
+  const int i = P->npb + (active ? idx : 0);
+  const float3 pi = ld3(A.pos[i]);
+  const int4 ci = A.cellpos[i];
+  const float4 vri = A.vel[i];
+  const float3 vi = ld3(vri);
+  const float rhoi = vri.w;
+  const float4 sria = rec_sra(B, i), srib = rec_srb(B, i);
+  const Sym sri = {sria.x, sria.y, sria.z, sria.w, srib.x, srib.y};
+  const float tri = srib.w;
+  const float artpi = tri > 0.f ? tri * P->art_p : 0.f;
+  const int shift = int(A.cvar[i].z);
+  const float two_h = P->two_h, i_h = P->i_h;
+  const PairConst K = pair_const(P);
+  int (*lst)[NL_WAVE] = s_list[threadIdx.x / NL_WAVE];
+  IAcc S;
+  S.accden = 0.f;
+  S.as = S.av = S.aa = make_float3(0.f, 0.f, 0.f);
+#pragma unroll
+  for (int k = 0; k < 9; k++) S.gv[k] = 0.f;
+  int head = 0, cnt = 0;  // ring: oldest entry, number of entries
+  nl_scan(P, K, S, A, B, cbeg, cend, lst, lane, head, cnt, ci, pi, active, vi, rhoi, sri, artpi, shift);
   nl_drain(K, S, A, B, lst, lane, head, cnt, pi, two_h, i_h, vi, rhoi, sri, artpi, shift);
+  if constexpr (PERIODIC) {
+    head = 0;
+    cnt = 0;
+    int4 cpb;
+    float3 ppb;
+    const bool img = active && period_image(P, ci, pi, false, cpb, ppb);
+    if (!img) { cpb = ci; ppb = pi; }
+    nl_scan(P, K, S, A, B, cbeg, cend, lst, lane, head, cnt, cpb, ppb, img, vi, rhoi, sri, artpi, shift);
+    nl_drain(K, S, A, B, lst, lane, head, cnt, ppb, two_h, i_h, vi, rhoi, sri, artpi, shift);
+  }
   if (active) interaction_epilogue(P, A, B, i, S, vi);
 }
 
@@ -785,6 +831,7 @@ __global__ void __launch_bounds__(256) k_corrector_material(const DevParams* __r
   int dx = rebin(p.x, two_h), dy = rebin(p.y, two_h), dz = rebin(p.z, two_h);
   p.x = p.x - float(dx) * two_h; p.y = p.y - float(dy) * two_h; p.z = p.z - float(dz) * two_h;
   ci.x += dx; ci.y += dy; ci.z += dz;
+  if (P->periodbound) wrap_period(P, ci, p);  // predictor_corrector.cu:158
   Sym D = ldsym(B.acestn_a, B.acestn_b, i);
   Sym stn0 = ldsym(B.stn_ini_a, B.stn_ini_b, i);
   stsym(A.stn_a, A.stn_b, i, sadd(stn0, smul(D, dt)));
@@ -937,14 +984,17 @@ void launch_pair_prep(hipStream_t s, const DevParams* P, int n, Arrays A, const 
   if (n > 0) hipLaunchKernelGGL(k_pair_prep, dim3(blocks(n, 256)), dim3(256), 0, s, P, A, B, n);
 }
 void launch_interaction(hipStream_t s, DevParams* P, int npm_max, Arrays A, const Buffers& B, const int* cb,
-                        const int* ce, int variant, int xcd_remap) {
+                        const int* ce, int variant, int xcd_remap, bool periodic) {
   if (npm_max <= 0) return;
   if (variant == LQ_INTERACTION_REFERENCE_ORDER)
     hipLaunchKernelGGL(k_interaction_ref, dim3(blocks(npm_max, 256)), dim3(256), 0, s, P, A, B, cb, ce,
                        xcd_remap, 0);
+  else if (variant == LQ_INTERACTION_NEIGHBOR_LIST && periodic)
+    hipLaunchKernelGGL(k_interaction_nl<true>, dim3(blocks(npm_max, NL_WAVE * NL_WAVES)), dim3(NL_WAVE * NL_WAVES),
+                       0, s, P, A, B, cb, ce, xcd_remap);
   else if (variant == LQ_INTERACTION_NEIGHBOR_LIST)
-    hipLaunchKernelGGL(k_interaction_nl, dim3(blocks(npm_max, NL_WAVE * NL_WAVES)), dim3(NL_WAVE * NL_WAVES), 0, s,
-                       P, A, B, cb, ce, xcd_remap);
+    hipLaunchKernelGGL(k_interaction_nl<false>, dim3(blocks(npm_max, NL_WAVE * NL_WAVES)), dim3(NL_WAVE * NL_WAVES),
+                       0, s, P, A, B, cb, ce, xcd_remap);
   else if (variant == LQ_INTERACTION_PAIR_LIST)  // fallback: runs only if the pair lists are not exact
     hipLaunchKernelGGL(k_interaction_ref, dim3(blocks(npm_max, 256)), dim3(256), 0, s, P, A, B, cb, ce,
                        xcd_remap, 1);
