@@ -42,8 +42,13 @@ def write_case(case: cases.Case, directory: str, name: str, time_max=None, outpu
     tm = p["time_max"] if time_max is None else time_max
     of = p["output_frequency"] if output_frequency is None else output_frequency
     items = [("dimension-of-problem", "dimension", p["dim"]), ("number-of-motion", "int", p["n_motion"]),
-             ("period-bound", "bool", p["periodbound"]), ("number-of-particles", "int", np_),
-             ("number-of-boundary-particles", "int", case.npb), ("number-of-material-particles", "int", case.npm)]
+             ("period-bound", "bool", p["periodbound"])]
+    if p["periodbound"]:  # GenCase.cu:134-143
+        for side in ("left", "right"):
+            for k, ab in enumerate("ABCD"):
+                items.append((f"plane{side}-{ab}", "scalar", repr(float(p["plane" + side][k]))))
+    items += [("number-of-particles", "int", np_),
+              ("number-of-boundary-particles", "int", case.npb), ("number-of-material-particles", "int", case.npm)]
     for k, ax in enumerate("xyz"):
         items.append((f"domain-size-{ax}", "scalar", repr(float(p["domain_size"][k]))))
     for k, ax in enumerate("xyz"):

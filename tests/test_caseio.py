@@ -25,6 +25,19 @@ def test_write_read_roundtrip(tmp_path):
     assert open(base + ".motion").read().split()[:2] == ["1", "0"]
 
 
+def test_periodic_planes_roundtrip(tmp_path):
+    """A periodic case writes its planes as GenCase.cu:134-143 does."""
+    case = cases.config("tiny", periodic_y=True)
+    base = caseio.write_case(case, str(tmp_path), "wall")
+    x = caseio.read_xml(base + ".xml")
+    assert x["period-bound"] in (1, True)
+    got = [x[f"planeleft-{k}"] for k in "ABCD"] + [x[f"planeright-{k}"] for k in "ABCD"]
+    want = list(case.params["planeleft"]) + list(case.params["planeright"])
+    assert got == pytest.approx(want)
+    plain = caseio.write_case(cases.config("tiny"), str(tmp_path), "plain")
+    assert "planeleft-A" not in open(plain + ".xml").read()
+
+
 def test_cli_missing_key_exits(tmp_path):
     """A missing XML key is fatal (loquat_io.cu:66-70)."""
     if not os.path.exists(CLI):
