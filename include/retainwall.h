@@ -208,6 +208,19 @@ int lq_get_params(lq_ctx* ctx, lq_params* out);
 int lq_get_time(lq_ctx* ctx, double* time, int64_t* step);
 int lq_set_time(lq_ctx* ctx, double time, int64_t step);
 
+/* Checkpoint / restart.  lq_checkpoint_write stores the whole device state
+ * a step reads (every per-slot array, the predictor snapshots and rates,
+ * cell ranges, imposed-motion tables, time, dt, step and particle counts)
+ * as raw bits; lq_checkpoint_read restores it into a context created with
+ * the same parameters (same shape, else LQ_ERR_ARG), after which steps
+ * continue bit for bit as the run that wrote it.  Replaces the reference's
+ * restart from N.dat + output_N.xml (loquat_io.cu:315-468, read back by
+ * LoadParameters/LoadParticles), which loses the contact state, the
+ * displacement record and the motion tables' progress.  The surrogate's
+ * networks are not part of a checkpoint. */
+int lq_checkpoint_write(lq_ctx* ctx, const char* path);
+int lq_checkpoint_read(lq_ctx* ctx, const char* path);
+
 /* ---- step pieces (each enqueues work on the context stream) -------------- */
 
 /* StepCalculation, loquat.cu:366-374: dt = cfl*h/cs, par.time = time. */

@@ -120,7 +120,8 @@ class HostState(C.Structure):
 EXPORTS = [
     "lq_abi_sizes", "lq_initialize_parameters", "lq_create", "lq_destroy", "lq_last_error",
     "lq_upload", "lq_set_motion_tables", "lq_get_motion_tables", "lq_get_field", "lq_set_field",
-    "lq_get_params", "lq_get_time", "lq_set_time", "lq_step_calculation", "lq_neighbor_search",
+    "lq_get_params", "lq_get_time", "lq_set_time", "lq_checkpoint_write", "lq_checkpoint_read",
+    "lq_step_calculation", "lq_neighbor_search",
     "lq_sort_arrays_primary", "lq_sort_arrays_all", "lq_rigid_body_update", "lq_adami_boundary",
     "lq_find_corner_particle", "lq_stress_regularization", "lq_particle_interaction",
     "lq_predictor", "lq_corrector", "lq_step", "lq_wall_force", "lq_positions_absolute",
@@ -151,6 +152,8 @@ def _proto(lib):
     lib.lq_get_params.argtypes = [vp, P(Params)]
     lib.lq_get_time.argtypes = [vp, P(C.c_double), P(C.c_int64)]
     lib.lq_set_time.argtypes = [vp, C.c_double, C.c_int64]
+    lib.lq_checkpoint_write.argtypes = [vp, C.c_char_p]
+    lib.lq_checkpoint_read.argtypes = [vp, C.c_char_p]
     for name in ["lq_step_calculation", "lq_neighbor_search", "lq_sort_arrays_primary",
                  "lq_sort_arrays_all", "lq_rigid_body_update", "lq_adami_boundary",
                  "lq_find_corner_particle", "lq_stress_regularization", "lq_particle_interaction",

@@ -1211,7 +1211,11 @@ extern "C" int lq_step(lq_ctx* c, int nsteps, int use_graph) {
   hipSetDevice(c->device);
   const bool graph_ok = use_graph && c->par.stress_shepard_filter == 0;
   int done = 0;
-  if (graph_ok && nsteps >= 2) {
+  if (graph_ok && !c->searched && nsteps >= 3) {
+    enq_step(c);  // the first search sorts in full; the captured steps then merge
+    done = 1;
+  }
+  if (graph_ok && nsteps - done >= 2) {
     if (c->graph == nullptr || c->graph_version != c->layout_version || c->graph_timed != c->timing ||
         (c->timing && c->graph_mask != c->timing_mask)) {
       const int st = capture_graph(c);
@@ -1667,5 +1671,134 @@ extern "C" int lq_neighbor_lists(lq_ctx* c, int max_nb, int32_t* counts, int32_t
   hipFree(dc);
   hipFree(di);
   if (e != hipSuccess) return fail(c, LQ_ERR_HIP, std::string("lq_neighbor_lists: ") + hipGetErrorString(e));
+  return LQ_OK;
+}
+
+// ------------------------------------------------------------ checkpoint ----
+// A checkpoint holds the whole device state a step reads: every per-slot
+// array of the current layout, the predictor snapshots and rates, the cell
+// ranges, the imposed-motion tables and the device-updated parameters, raw
+// (bit patterns), so that a run resumed from it continues bit for bit.  The
+// reference restarts from its .dat/.xml output (loquat_io.cu:315-468),
+// which drops the contact state, the displacement record and the tables'
+// progress.  The next search after a load runs the full sort (its result
+// equals the merge path's).
+namespace {
+struct CkptHeader {
+  char magic[8];
+  int32_t version, n_cases, np, npb, npm0, ncell, ncm, n_motion_rows;
+  int32_t dyn_npm, dyn_np_delete_m, dyn_err, dyn_sort_changed;
+  uint32_t dyn_max_disp2_bits;
+  int32_t dyn_plist_bad;
+  float dyn_time, dyn_dt;
+  double dyn_time_d;
+  int64_t dyn_step, host_step;
+};
+constexpr char kCkptMagic[8] = {'L', 'Q', 'C', 'K', 'P', 'T', 0, 1};
+struct CkptArray {
+  void* p;
+  size_t bytes;
+};
+}  // namespace
+
+static std::vector<CkptArray> ckpt_arrays(lq_ctx* c) {
+  const size_t n = size_t(c->par.np), nc = size_t(std::max(1, c->par.ncell));
+  const size_t ncm = size_t(std::max(1, c->par.ncell_material));
+  Arrays& A = c->B.cur;
+  Buffers& B = c->B;
+  std::vector<CkptArray> v = {
+      {A.codes, n * 16}, {A.cellpos, n * 16}, {A.orig_id, n * 4}, {A.case_id, n * 4}, {A.pres, n * 4},
+      {A.pos, n * 16}, {A.pos_ori, n * 16}, {A.vel, n * 16}, {A.idm, n * 16}, {A.normal, n * 16},
+      {A.str_a, n * 16}, {A.stn_a, n * 16}, {A.str_b, n * 8}, {A.stn_b, n * 8},
+      {A.cvar, n * 16}, {A.cf, n * 16}, {A.cn, n * 16}, {A.cw, n * 16},
+      {B.pos_ini, n * 16}, {B.vel_ini, n * 16}, {B.str_ini_a, n * 16}, {B.stn_ini_a, n * 16},
+      {B.str_ini_b, n * 8}, {B.stn_ini_b, n * 8}, {B.accad, n * 16}, {B.spin, n * 16},
+      {B.acestn_a, n * 16}, {B.acestn_b, n * 8},
+      {B.cell_beg, nc * 4}, {B.cell_end, nc * 4}, {B.wall_near, ncm * 4},
+      {c->d_mv, size_t(c->n_motion_rows) * sizeof(DevMotion)},
+      {c->d_tab, size_t(c->n_motion_rows) * sizeof(DevTable)}};
+  return v;
+}
+
+extern "C" int lq_checkpoint_write(lq_ctx* c, const char* path) {
+  NEED_UPLOAD(c);
+  if (!path) return fail(c, LQ_ERR_ARG, "lq_checkpoint_write: null path");
+  hipSetDevice(c->device);
+  DevParams d;
+  int st = pull_params(c, d);
+  if (st) return st;
+  CkptHeader h{};
+  std::memcpy(h.magic, kCkptMagic, 8);
+  h.version = 1;
+  h.n_cases = c->n_cases; h.np = c->par.np; h.npb = c->par.npb; h.npm0 = c->npm0;
+  h.ncell = c->par.ncell; h.ncm = c->par.ncell_material; h.n_motion_rows = c->n_motion_rows;
+  h.dyn_npm = d.npm; h.dyn_np_delete_m = d.np_delete_m; h.dyn_err = d.err; h.dyn_sort_changed = d.sort_changed;
+  h.dyn_max_disp2_bits = d.max_disp2_bits; h.dyn_plist_bad = d.plist_bad;
+  h.dyn_time = d.time; h.dyn_dt = d.dt; h.dyn_time_d = d.time_d; h.dyn_step = d.step;
+  h.host_step = c->host_step;
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(c, LQ_ERR_ARG, std::string("lq_checkpoint_write: cannot open ") + path);
+  bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1;
+  std::vector<char> buf;
+  for (const CkptArray& a : ckpt_arrays(c)) {
+    if (!ok) break;
+    buf.resize(a.bytes);
+    hipError_t e = hipMemcpyAsync(buf.data(), a.p, a.bytes, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+      std::fclose(f);
+      return fail(c, LQ_ERR_HIP, std::string("lq_checkpoint_write: ") + hipGetErrorString(e));
+    }
+    ok = std::fwrite(buf.data(), 1, a.bytes, f) == a.bytes;
+  }
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok) return fail(c, LQ_ERR_ARG, std::string("lq_checkpoint_write: short write to ") + path);
+  return LQ_OK;
+}
+
+extern "C" int lq_checkpoint_read(lq_ctx* c, const char* path) {
+  if (!c) return LQ_ERR_ARG;
+  if (!path) return fail(c, LQ_ERR_ARG, "lq_checkpoint_read: null path");
+  hipSetDevice(c->device);
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(c, LQ_ERR_ARG, std::string("lq_checkpoint_read: cannot open ") + path);
+  CkptHeader h{};
+  bool ok = std::fread(&h, sizeof(h), 1, f) == 1 && std::memcmp(h.magic, kCkptMagic, 8) == 0 && h.version == 1;
+  if (!ok) {
+    std::fclose(f);
+    return fail(c, LQ_ERR_ARG, "lq_checkpoint_read: not a checkpoint of this format");
+  }
+  if (h.n_cases != c->n_cases || h.np != c->par.np || h.npb != c->par.npb || h.npm0 != c->npm0 ||
+      h.ncell != c->par.ncell || h.ncm != c->par.ncell_material || h.n_motion_rows != c->n_motion_rows) {
+    std::fclose(f);
+    return fail(c, LQ_ERR_ARG, "lq_checkpoint_read: the checkpoint was written by a context of another shape");
+  }
+  std::vector<char> buf;
+  for (const CkptArray& a : ckpt_arrays(c)) {
+    buf.resize(a.bytes);
+    if (std::fread(buf.data(), 1, a.bytes, f) != a.bytes) { ok = false; break; }
+    hipError_t e = hipMemcpyAsync(a.p, buf.data(), a.bytes, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+      std::fclose(f);
+      return fail(c, LQ_ERR_HIP, std::string("lq_checkpoint_read: ") + hipGetErrorString(e));
+    }
+  }
+  std::fclose(f);
+  if (!ok) return fail(c, LQ_ERR_ARG, "lq_checkpoint_read: truncated file");
+  DevParams d;
+  int st = pull_params(c, d);
+  if (st) return st;
+  d.npm = h.dyn_npm; d.np_delete_m = h.dyn_np_delete_m; d.err = h.dyn_err; d.sort_changed = h.dyn_sort_changed;
+  d.max_disp2_bits = h.dyn_max_disp2_bits; d.plist_bad = h.dyn_plist_bad;
+  d.time = h.dyn_time; d.dt = h.dyn_dt; d.time_d = h.dyn_time_d; d.step = h.dyn_step;
+  HIPCHECK(c, hipMemcpyAsync(c->dP, &d, sizeof(DevParams), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  c->host_step = h.host_step;
+  c->searched = false;  // no previous sorted keys: the next search sorts in full
+  c->uploaded = true;
+  c->case_uploaded.assign(c->n_cases, 1);
+  c->n_uploaded = c->n_cases;
+  c->layout_version++;  // re-capture graphs
   return LQ_OK;
 }

@@ -104,6 +104,13 @@ class Context:
     def set_time(self, t, step):
         self._chk(self.lib.lq_set_time(self.h, t, step), "lq_set_time")
 
+    def checkpoint_write(self, path):
+        """Whole device state, raw, for a bit-exact restart (lq_checkpoint_write)."""
+        self._chk(self.lib.lq_checkpoint_write(self.h, str(path).encode()), "lq_checkpoint_write")
+
+    def checkpoint_read(self, path):
+        self._chk(self.lib.lq_checkpoint_read(self.h, str(path).encode()), "lq_checkpoint_read")
+
     # -------------------------------------------------------- step pieces --
     def step_calculation(self):
         self._chk(self.lib.lq_step_calculation(self.h), "lq_step_calculation")
