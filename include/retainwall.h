@@ -266,6 +266,18 @@ int lq_step(lq_ctx* ctx, int nsteps, int use_graph);
  * point, 2^-64 resolution, independent of order) and scaled once.
  * Synchronises. */
 int lq_wall_force(lq_ctx* ctx, const float box[3], double* force_kn);
+/* Wall pressure profile, GetValuesAlongLine (utilities.cu:224-367, written
+ * to stress_zz<N>.txt at each save).  The reference hard-codes the wall face
+ * x = 0.04 + bt - 0.005 t, the depth range [bt+0.04, bt+1.04] and
+ * int(1/dr) bins; here geom = {x_face, z_lo, z_hi, z_top} and nbins are
+ * arguments.  Soil slots (deleted ones included, as the reference loops to
+ * np) in the bands [x_face + (k-1) dr, x_face + k dr], k = 1..3, with
+ * z_lo <= z <= z_hi fall in bin int((z_top - z) / dr); per bin and band the
+ * last slot in slot order gives contact-force x / dr.  out f32[nbins][4] =
+ * (depth = bin dr where band 1 has a slot, p1, p2, p3); bins without a slot
+ * read 0 (the reference leaves them unwritten).  Single-case contexts.
+ * Synchronises. */
+int lq_wall_pressure_profile(lq_ctx* ctx, const float geom[4], int nbins, float* out, size_t bytes);
 
 /* Absolute positions (PositionFromRelative2Absolute, loquat_grid.cu:273-289)
  * into dst f32[np][4].  Synchronises. */

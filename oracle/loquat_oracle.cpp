@@ -203,6 +203,10 @@ template <class T> static int step_T(Par<T>& P, MotionTab<T>* M, State<T>& S, do
   extern "C" double or_wall_force_##SFX(const lq_params* p, or_state* s, const float* box) {          \
     Par<T> P; load_par(p, P); State<T> S = view<T>(s); return wall_force(P, S, box);                   \
   }                                                                                                    \
+  extern "C" int or_wall_profile_##SFX(const lq_params* p, or_state* s, const float* geom, int nbins,  \
+                                       double* out) {                                                  \
+    Par<T> P; load_par(p, P); State<T> S = view<T>(s); wall_profile(P, S, geom, nbins, out); return 0; \
+  }                                                                                                    \
   extern "C" int or_run_##SFX(lq_params* p, lq_motion_table* tabs, or_state* s, int nsteps,          \
                               double* time, int64_t* step, const float* box, int force_every,          \
                               double* forces, int* ns2_identity_all, double* seconds) {                \

@@ -1042,4 +1042,26 @@ template <class T> static double wall_force(const Par<T>& P, const State<T>& S, 
   return F;
 }
 
+// cuGetValuesAlongLine, utilities.cu:266-320, with the case constants as
+// arguments: geom = {x_face, z_lo, z_hi, z_top}; out[nbins][4] = depth
+// (set by band-1 writes), band-1..3 contact force x / dr of the last slot
+// (serial slot order) in each bin; unwritten entries stay 0.
+template <class T> static void wall_profile(const Par<T>& P, const State<T>& S, const float geom[4], int nbins,
+                                            double* out) {
+  for (int k = 0; k < 4 * nbins; k++) out[k] = 0.0;
+  const T dr = P.dr;
+  for (int i = P.npb; i < P.np; i++) {
+    const v3<T> a = abs_pos(P, cell_of(S.cellpos, i), ld3(S.pos, i));
+    if (!(a.z >= T(geom[1]) && a.z <= T(geom[2]))) continue;
+    const int bin = int((T(geom[3]) - a.z) / dr);
+    if (bin < 0 || bin >= nbins) continue;
+    for (int b = 0; b < 3; b++) {
+      if (a.x >= T(geom[0]) + dr * T(b) && a.x <= T(geom[0]) + dr * T(b + 1)) {
+        if (b == 0) out[4 * bin] = double(T(bin) * dr);
+        out[4 * bin + 1 + b] = double(S.cmag[9 * i] / dr);
+      }
+    }
+  }
+}
+
 }  // namespace oracle

@@ -58,6 +58,8 @@ def lib():
                                                              P(C.c_int)]
             getattr(L, f"or_wall_force_{sfx}").argtypes = [P(_abi.Params), P(OrState), P(C.c_float)]
             getattr(L, f"or_wall_force_{sfx}").restype = C.c_double
+            getattr(L, f"or_wall_profile_{sfx}").argtypes = [P(_abi.Params), P(OrState), P(C.c_float), C.c_int,
+                                                             C.c_void_p]
             getattr(L, f"or_neighbor_lists_{sfx}").argtypes = [P(_abi.Params), P(OrState), C.c_int, C.c_void_p,
                                                                C.c_void_p]
             getattr(L, f"or_run_{sfx}").argtypes = [P(_abi.Params), P(_abi.MotionTable), P(OrState), C.c_int,
@@ -181,6 +183,14 @@ class Oracle:
         s = self._st()
         b = (C.c_float * 3)(*box)
         return getattr(lib(), f"or_wall_force_{self.sfx}")(C.byref(self.p), C.byref(s), b)
+
+    def wall_profile(self, geom, nbins):
+        """cuGetValuesAlongLine restated (utilities.cu:266-320)."""
+        s = self._st()
+        g = (C.c_float * 4)(*geom)
+        out = np.zeros((nbins, 4), np.float64)
+        getattr(lib(), f"or_wall_profile_{self.sfx}")(C.byref(self.p), C.byref(s), g, int(nbins), out.ctypes.data)
+        return out
 
     def run(self, nsteps, box=(0, 0, 0), force_every=0):
         s = self._st()

@@ -1295,6 +1295,31 @@ extern "C" int lq_wall_force(lq_ctx* c, const float box[3], double* F) {
   return lq_wall_force_cases(c, box, F);
 }
 
+// GetValuesAlongLine (utilities.cu:224-367) with the case constants passed
+// in: out f32[nbins][4] = (depth, band-1, band-2, band-3 pressure).
+extern "C" int lq_wall_pressure_profile(lq_ctx* c, const float geom[4], int nbins, float* out, size_t bytes) {
+  NEED_UPLOAD(c);
+  if (c->n_cases != 1) return fail(c, LQ_ERR_ARG, "lq_wall_pressure_profile: single-case contexts only");
+  if (!geom || !out || nbins <= 0 || bytes != size_t(nbins) * 16)
+    return fail(c, LQ_ERR_ARG, "lq_wall_pressure_profile: args");
+  hipSetDevice(c->device);
+  int* last = nullptr;
+  float4* dout = nullptr;
+  HIPCHECK(c, hipMalloc(&last, size_t(nbins) * 3 * sizeof(int)));
+  hipError_t e = hipMalloc(&dout, size_t(nbins) * sizeof(float4));
+  if (e == hipSuccess) {
+    launch_wall_profile(c->stream, c->dP, c->par.np, c->par.npb, c->B.cur,
+                        make_float4(geom[0], geom[1], geom[2], geom[3]), nbins, last, dout);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  hipFree(last);
+  if (dout) hipFree(dout);
+  if (e != hipSuccess) return fail(c, LQ_ERR_HIP, std::string("lq_wall_pressure_profile: ") + hipGetErrorString(e));
+  return LQ_OK;
+}
+
 extern "C" int lq_positions_absolute(lq_ctx* c, float* dst, size_t bytes) {
   NEED_UPLOAD(c);
   if (!dst || bytes != size_t(c->par.np) * 16) return fail(c, LQ_ERR_ARG, "lq_positions_absolute: size");

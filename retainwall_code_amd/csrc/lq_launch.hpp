@@ -29,6 +29,8 @@ void launch_contact(hipStream_t s, const DevParams* P, int npm_max, Arrays A, co
                     const int* ce);
 void launch_predictor(hipStream_t s, DevParams* P, int np, Arrays A, const Buffers& B, int track_disp);
 void launch_corrector(hipStream_t s, DevParams* P, int npb, int npm_max, int n_motion, Arrays A, const Buffers& B);
+void launch_wall_profile(hipStream_t s, const DevParams* P, int np, int npb, Arrays A, float4 geom, int nbins, int* last,
+                         float4* out);
 void launch_wall_force(hipStream_t s, const DevParams* P, int np, int n_cases, Arrays A, const float* boxes,
                        unsigned long long* acc);
 void launch_neighbor_list(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const int* cb, const int* ce,

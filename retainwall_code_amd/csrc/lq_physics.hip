@@ -894,6 +894,43 @@ __global__ void __launch_bounds__(256) k_wall_force(const DevParams* __restrict_
   }
 }
 
+// -------------------------------------------- wall pressure profile ----
+// cuGetValuesAlongLine, utilities.cu:266-320: the soil slots in three
+// one-dr bands in front of the wall face x_face (bands closed at both ends)
+// and inside [z_lo, z_hi] are binned by depth num_z = int((z_top - z)/dr);
+// in each (band, bin) the reference's serial loop over slots keeps the last
+// slot's contact force x / dr, and the band-1 write also sets the depth
+// num_z dr.  Here every slot records itself in last[band][bin] with
+// atomicMax (the largest slot is the serial loop's last writer), and a
+// second kernel gathers the values.  geom = {x_face, z_lo, z_hi, z_top}.
+// Absolute positions are formed in FP32.
+__global__ void __launch_bounds__(256) k_profile_mark(const DevParams* __restrict__ P, Arrays A, float4 geom,
+                                                      int nbins, int* __restrict__ last) {
+  const int i = P->npb + int(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= P->np) return;
+  const float3 pa = abs_from_cell(P, A.cellpos[i], ld3(A.pos[i]));
+  const float dr = P->dr;
+  if (!(pa.z >= geom.y && pa.z <= geom.z)) return;
+  const int bin = int((geom.w - pa.z) / dr);
+  if (bin < 0 || bin >= nbins) return;
+#pragma unroll
+  for (int b = 0; b < 3; b++)
+    if (pa.x >= geom.x + dr * float(b) && pa.x <= geom.x + dr * float(b + 1)) atomicMax(&last[b * nbins + bin], i);
+}
+
+__global__ void __launch_bounds__(256) k_profile_write(const DevParams* __restrict__ P, Arrays A, int nbins,
+                                                       const int* __restrict__ last, float4* __restrict__ out) {
+  const int bin = blockIdx.x * blockDim.x + threadIdx.x;
+  if (bin >= nbins) return;
+  const float dr = P->dr;
+  float4 r = make_float4(0.f, 0.f, 0.f, 0.f);  // depth, band 1, band 2, band 3
+  const int j1 = last[bin], j2 = last[nbins + bin], j3 = last[2 * nbins + bin];
+  if (j1 >= 0) { r.x = float(bin) * dr; r.y = A.cf[j1].x / dr; }
+  if (j2 >= 0) r.z = A.cf[j2].x / dr;
+  if (j3 >= 0) r.w = A.cf[j3].x / dr;
+  out[bin] = r;
+}
+
 __global__ void k_fill_u64(unsigned long long* a, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = 0ull;
@@ -1016,6 +1053,13 @@ void launch_corrector(hipStream_t s, DevParams* P, int npb, int npm_max, int n_m
     hipLaunchKernelGGL(k_corrector_bound, dim3(blocks(npb, 256)), dim3(256), 0, s, P, A, B);
   if (npm_max > 0)
     hipLaunchKernelGGL(k_corrector_material, dim3(blocks(npm_max, 256)), dim3(256), 0, s, P, A, B);
+}
+void launch_wall_profile(hipStream_t s, const DevParams* P, int np, int npb, Arrays A, float4 geom, int nbins, int* last,
+                         float4* out) {
+  launch_fill_int(s, last, 3 * nbins, -1);
+  if (np > npb)
+    hipLaunchKernelGGL(k_profile_mark, dim3(blocks(np - npb, 256)), dim3(256), 0, s, P, A, geom, nbins, last);
+  hipLaunchKernelGGL(k_profile_write, dim3(blocks(nbins, 256)), dim3(256), 0, s, P, A, nbins, (const int*)last, out);
 }
 void launch_wall_force(hipStream_t s, const DevParams* P, int np, int n_cases, Arrays A, const float* boxes,
                        unsigned long long* acc) {

@@ -77,6 +77,7 @@ void LoquatSph::CopyDataFromDeviceToHost() {
   std::copy(nrm.begin(), nrm.begin() + size_t(par_host->npb) * 4, io->GetNormalMaterial().begin());
   lq_check(ctx, lq_stress_saving_regularization(ctx, io->GetStr().data(), np * 24), "StressSavingRegularization");
   lq_check(ctx, lq_get_field(ctx, LQ_F_STN, io->GetStn().data(), np * 24), "stn");
+  lq_check(ctx, lq_get_field(ctx, LQ_F_ACCAD, io->GetAccad().data(), np * 16), "accad");
 }
 
 void LoquatSph::Saving(std::ofstream& th) {
@@ -92,6 +93,22 @@ void LoquatSph::Saving(std::ofstream& th) {
   CopyDataFromDeviceToHost();
   io->SaveXml();
   io->SaveParticles();
+  {
+    // GetValuesAlongLine (utilities.cu:324-367) with the reference's case
+    // constants: wall face 0.04 + bt - 0.005 t, depths [bt + 0.04, bt + 1.04]
+    const float bt = par_host->bound_thickness;
+    const int nip = int(1.0f / par_host->dr);
+    const float geom[4] = {float(0.04f + bt - double(par_host->time) * 0.005), bt + 0.04f, bt + 0.04f + 1.0f,
+                           0.04f + 1.0f + bt};
+    std::vector<float> prof(size_t(std::max(nip, 1)) * 4);
+    if (nip > 0)
+      lq_check(ctx, lq_wall_pressure_profile(ctx, geom, nip, prof.data(), prof.size() * sizeof(float)),
+               "GetValuesAlongLine");
+    std::ofstream zz(project_name + "/stress_zz" + std::to_string(par_host->output_number) + ".txt");
+    for (int i = 0; i < nip; i++)
+      zz << prof[4 * size_t(i)] << " " << prof[4 * size_t(i) + 1] << " " << prof[4 * size_t(i) + 2] << " "
+         << prof[4 * size_t(i) + 3] << std::endl;
+  }
   double F = 0.0;
   lq_check(ctx, lq_wall_force(ctx, probe_box, &F), "GetValuesAtPoint");
   th << par_host->time << " " << F << std::endl;  // utilities.cu:669

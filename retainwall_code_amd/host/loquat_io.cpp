@@ -1,6 +1,7 @@
 // loquat_io.cpp -- XML / .dat / .motion input and plain-text output.
 #include "loquat_io.hpp"
 
+#include <algorithm>
 #include <cctype>
 #include <cstdlib>
 #include <cstring>
@@ -202,6 +203,7 @@ void LoquatIo::LoadParticles() {
   str_.assign(size_t(np) * 6, 0.f);
   stn_.assign(size_t(np) * 6, 0.f);
   normal_.assign(size_t(std::max(npb, 0)) * 4, 0.f);
+  accad_.assign(size_t(np) * 4, 0.f);
   std::string dat = project_name_ + ".dat";
   std::ifstream in(dat);
   if (!in) {
@@ -324,10 +326,15 @@ void LoquatIo::SaveXml() {
   o << "</parameters>\n";
 }
 
-// Plain .dat in the input format (boundary rows with normals, then soil
-// rows), so that N.dat + output_N.xml restart a run.  Deleted particles
-// (slots past npb+npm) are not written.
+// output_format 1: plain .dat in the input format (boundary rows with
+// normals, then soil rows), so that N.dat + output_N.xml restart a run.
+// Deleted particles (slots past npb+npm) are not written.  output_format 2:
+// ParaView .vtu files (SaveVtu).
 void LoquatIo::SaveParticles() {
+  if (par_.output_format == 2) {
+    SaveVtu();
+    return;
+  }
   std::ostringstream name;
   name << project_name_ << "/" << par_.output_number << ".dat";
   std::ofstream o(name.str());
@@ -353,4 +360,94 @@ void LoquatIo::SaveParticles() {
     }
     o << "\n";
   }
+}
+
+// loquat_io.cu:470-680: one ASCII UnstructuredGrid file <id>_<N>.vtu per
+// value of idmassidx.x (boundary block ids, 100 for soil, 999 for deleted),
+// the slots grouped by a stable sort on that id, one vertex cell per
+// particle.  Point data as the reference writes it: Material (codes.w),
+// Ori_ID (idmassidx.w), Velocity, Acceleration (accad, which holds the
+// accumulated displacement after a corrector), Density, Mass, Normal for
+// boundary groups (id < 100), Stress and Strain in xx yy zz xy yz xz order;
+// Points are the absolute positions.
+void LoquatIo::SaveVtu() {
+  const int np = par_.np, npb = par_.npb;
+  std::vector<int> key(static_cast<size_t>(np)), order(static_cast<size_t>(np));
+  for (int i = 0; i < np; i++) {
+    key[size_t(i)] = static_cast<int>(idmassidx_[4 * size_t(i)]);
+    order[size_t(i)] = i;
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[size_t(a)] < key[size_t(b)]; });
+  std::cout << "|";
+  for (int g0 = 0; g0 < np;) {
+    const int id = key[size_t(order[size_t(g0)])];
+    int g1 = g0;
+    while (g1 < np && key[size_t(order[size_t(g1)])] == id) g1++;
+    const int n = g1 - g0;
+    std::cout << id << ": " << n << "|";
+    std::ostringstream name;
+    name << project_name_ << "/" << id << "_" << par_.output_number << ".vtu";
+    std::ofstream o(name.str());
+    if (!o) {
+      std::cout << "Error! Cannot write file " << name.str() << " !" << std::endl;
+      g0 = g1;
+      continue;
+    }
+    auto each = [&](auto&& emit) {
+      for (int k = g0; k < g1; k++) emit(size_t(order[size_t(k)]));
+    };
+    auto vec3 = [&](const char* nm, const std::vector<float>& a, int width) {
+      o << "        <DataArray type=\"Float64\" Name=\"" << nm << "\" NumberOfComponents=\"3\">\n";
+      each([&](size_t i) { o << a[width * i] << " " << a[width * i + 1] << " " << a[width * i + 2] << " "; });
+      o << "</DataArray>\n";
+    };
+    auto scalar = [&](const char* nm, auto&& value) {
+      o << "        <DataArray type=\"Float64\" Name=\"" << nm << "\">\n";
+      each([&](size_t i) { o << value(i) << " "; });
+      o << "</DataArray>\n";
+    };
+    auto tensor = [&](const char* nm, const std::vector<float>& t) {
+      o << "        <DataArray type=\"Float64\" Name=\"" << nm << "\" NumberOfComponents=\"6\">\n";
+      each([&](size_t i) {
+        const float* v = &t[6 * i];
+        o << v[0] << " " << v[1] << " " << v[2] << " " << v[3] << " " << v[5] << " " << v[4] << " ";
+      });
+      o << "</DataArray>\n";
+    };
+    o << "<?xml version=\"1.0\"?>\n"
+      << "<VTKFile type=\"UnstructuredGrid\" version=\"0.1\" byte_order=\"LittleEndian\" "
+         "compressor=\"vtkZLibDataCompressor\">\n"
+      << "  <UnstructuredGrid GhostLevel=\"0\">\n"
+      << "    <Piece NumberOfPoints=\"" << n << "\" NumberOfCells=\"" << n << "\" Name=\"" << id << "\">\n"
+      << "      <PointData>\n";
+    scalar("Material", [&](size_t i) { return codes_[4 * i + 3]; });
+    scalar("Ori_ID", [&](size_t i) { return idmassidx_[4 * i + 3]; });
+    vec3("Velocity", velrhop_, 4);
+    vec3("Acceleration", accad_, 4);
+    scalar("Density", [&](size_t i) { return velrhop_[4 * i + 3]; });
+    scalar("Mass", [&](size_t i) { return idmassidx_[4 * i + 1]; });
+    if (id < 100) {  // MATERIAL_NUMBER, types.h:37
+      o << "        <DataArray type=\"Float64\" Name=\"Normal\" NumberOfComponents=\"3\">\n";
+      each([&](size_t i) {
+        const bool b = i < size_t(npb);
+        o << (b ? normal_[4 * i] : 0.f) << " " << (b ? normal_[4 * i + 1] : 0.f) << " " << (b ? normal_[4 * i + 2] : 0.f)
+          << " ";
+      });
+      o << "</DataArray>\n";
+    }
+    tensor("Stress", str_);
+    tensor("Strain", stn_);
+    o << "      </PointData>\n      <CellData>\n      </CellData>\n      <Points>\n";
+    vec3("Points", pospres_, 4);
+    o << "      </Points>\n      <Cells>\n"
+      << "        <DataArray type=\"Int64\" Name=\"connectivity\" format=\"ascii\">\n";
+    for (int j = 0; j < n; j++) o << j << " ";
+    o << "</DataArray>\n        <DataArray type=\"Int64\" Name=\"offsets\" format=\"ascii\"> \n";
+    for (int j = 0; j < n; j++) o << j + 1 << " ";
+    o << "</DataArray>\n        <DataArray type=\"Int64\" Name=\"types\" format=\"ascii\">\n";
+    for (int j = 0; j < n; j++) o << 1 << " ";
+    o << "</DataArray>\n      </Cells>\n    </Piece>\n  </UnstructuredGrid>\n</VTKFile>\n";
+    g0 = g1;
+  }
+  std::cout << std::endl;
 }

@@ -4,7 +4,8 @@
 // rows with 31 columns, then soil rows with 27) and <project>.motion, with the
 // same key names, column order and fatal-error behaviour as the reference
 // (loquat_io.cu:62-312).  Writes output_<N>.xml, <N>.dat (restartable, same
-// format as the input) and time_history.txt.
+// format as the input) or <id>_<N>.vtu files (output_format 2), and
+// time_history.txt.
 #pragma once
 #include <map>
 #include <string>
@@ -19,7 +20,7 @@ class LoquatIo {
   void LoadParameters();  // loquat_io.cu:75-202
   void LoadParticles();   // loquat_io.cu:205-312
   void SaveXml();         // loquat_io.cu:315-399
-  void SaveParticles();   // loquat_io.cu:430-468 (plain .dat, format 1)
+  void SaveParticles();   // loquat_io.cu:430-680 (.dat for format 1, .vtu for format 2)
 
   lq_params* GetParameters() { return &par_; }
   std::vector<int32_t>& GetCodes() { return codes_; }
@@ -29,6 +30,7 @@ class LoquatIo {
   std::vector<float>& GetStr() { return str_; }
   std::vector<float>& GetStn() { return stn_; }
   std::vector<float>& GetNormalMaterial() { return normal_; }
+  std::vector<float>& GetAccad() { return accad_; }
   std::vector<lq_motion_table>& GetMotionTables() { return motion_; }
   const std::string& ProjectName() const { return project_name_; }
 
@@ -41,11 +43,12 @@ class LoquatIo {
   int GetInt(const std::string& name) const;
   float GetScalar(const std::string& name) const;
   bool GetBool(const std::string& name) const;
+  void SaveVtu();  // loquat_io.cu:470-680
 
   std::string project_name_;
   std::map<std::string, Value> xml_;
   lq_params par_{};
   std::vector<int32_t> codes_;
-  std::vector<float> pospres_, velrhop_, idmassidx_, str_, stn_, normal_;
+  std::vector<float> pospres_, velrhop_, idmassidx_, str_, stn_, normal_, accad_;
   std::vector<lq_motion_table> motion_;
 };

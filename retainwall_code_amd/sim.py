@@ -162,6 +162,15 @@ class Context:
         self._chk(self.lib.lq_wall_force(self.h, b, C.byref(F)), "lq_wall_force")
         return F.value
 
+    def wall_pressure_profile(self, geom, nbins):
+        """(depth, p1, p2, p3) per depth bin (lq_wall_pressure_profile);
+        geom = (x_face, z_lo, z_hi, z_top)."""
+        g = (C.c_float * 4)(*geom)
+        out = np.zeros((nbins, 4), np.float32)
+        self._chk(self.lib.lq_wall_pressure_profile(self.h, g, nbins, out.ctypes.data, out.nbytes),
+                  "lq_wall_pressure_profile")
+        return out
+
     def positions_absolute(self):
         out = np.empty((self.params.np, 4), np.float32)
         self._chk(self.lib.lq_positions_absolute(self.h, out.ctypes.data, out.nbytes), "lq_positions_absolute")
