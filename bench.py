@@ -51,25 +51,34 @@ def candidate_counts(cellpos, beg, end, p, soil):
 
 
 def cpu_baseline(case_name, seconds_target=20.0):
-    """FP32 oracle (serial, 1 thread) on the same case for a bounded number of steps."""
+    """FP32 oracle on the host cores for a bounded number of whole steps of
+    the same case: its per-particle loops run on OpenMP threads (one per
+    core the run may use, OMP_NUM_THREADS when set), and one serial step is
+    timed beside it."""
     from retainwall_code_amd import cases
     from oracle import oracle as O
+    O.build()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    threads = max(1, min(threads, 64))
     case = cases.config(case_name)
     p = cases.make_params(case)
     o = O.Oracle(p, np.float32, cases.motion_tables(case)).load_host(case)
     o.abs2rel()
     o.neighbor_search()
     o.sort_arrays_primary()
+    O.set_threads(1)
+    serial = o.run(1)["seconds"]
+    O.set_threads(threads)
     r = o.run(1)
-    nsteps = 1
-    secs = r["seconds"]
+    nsteps, secs = 1, r["seconds"]
     if secs < seconds_target / 2:
         more = max(1, int(seconds_target / max(secs, 1e-3)) - 1)
         r = o.run(more)
         nsteps, secs = more, r["seconds"]
-    return {"value": case.npm * nsteps / secs, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{nsteps} full step(s) of the same {case.npm}-soil-particle case, FP32 oracle, serial",
-            "seconds": secs}
+    return {"value": case.npm * nsteps / secs, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{nsteps} full step(s) of the same {case.npm}-soil-particle case, FP32 oracle, "
+                      f"{threads} OpenMP threads",
+            "seconds": secs, "serial_value": case.npm / serial}
 
 
 def timed_steps(ctx, steps, graph):

@@ -7,6 +7,8 @@
 
 #include <chrono>
 
+#include <omp.h>
+
 using namespace oracle;
 
 extern "C" {
@@ -268,6 +270,11 @@ extern "C" int or_neighbor_lists_f32(const lq_params* p, or_state* s, int max_nb
 extern "C" int or_neighbor_lists_f64(const lq_params* p, or_state* s, int max_nb, int32_t* counts, int32_t* ids) {
   return neighbor_lists_T<double>(p, s, max_nb, counts, ids);
 }
+
+// Threads for the per-particle loops (results do not depend on it: every
+// parallel iteration writes only its own particle's rows).
+extern "C" void or_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+extern "C" int or_get_threads() { return omp_get_max_threads(); }
 
 // Wendland C2 value and gradient factor (functions.h:142-143) for tests.
 extern "C" double or_kernel_f64(double knorm, double q) { return kernelW<double>(knorm, q); }

@@ -311,6 +311,7 @@ template <class T> static int neighbor_search(Par<T>& P, State<T>& S, int* ident
 
 template <class T, class E> static void gather(std::vector<E>& tmp, E* a, int width, int n, const int32_t* idx) {
   tmp.assign(a, a + size_t(width) * n);
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < n; i++)
     for (int k = 0; k < width; k++) a[size_t(width) * i + k] = tmp[size_t(width) * idx[i] + k];
 }
@@ -392,6 +393,7 @@ template <class T> static void rigid_body_update(Par<T>& P, MotionTab<T>* M) {
 // ------------------------------------------------------------ a7: Adami --
 // cuAdamiBoundaryMaterial + loops, adami_boundary.cu:26-158.
 template <class T> static void adami(const Par<T>& P, State<T>& S) {
+#pragma omp parallel for schedule(dynamic, 256)
   for (int i = 0; i < P.npb; i++) {
     v3<T> pi = ld3(S.pos, i);
     v3<int> ci = cell_of(S.cellpos, i);
@@ -483,6 +485,7 @@ template <class T> static void adami(const Par<T>& P, State<T>& S) {
 // ----------------------------------------------------------- a9: corner --
 // cuFindCornerParticleMaterial + loop, adami_boundary.cu:209-292.
 template <class T> static void find_corner(const Par<T>& P, State<T>& S) {
+#pragma omp parallel for schedule(dynamic, 256)
   for (int idx = 0; idx < P.npm; idx++) {
     int i = P.npb + idx;
     v3<T> pi = ld3(S.pos, i);
@@ -528,6 +531,7 @@ template <class T> static void find_corner(const Par<T>& P, State<T>& S) {
 // cuStressRegularizationShepard + loop, field_smoothing.cu:26-101.
 template <class T> static void shepard(const Par<T>& P, State<T>& S) {
   std::vector<sym6<T>> out(P.npm);
+#pragma omp parallel for schedule(dynamic, 256)
   for (int idx = 0; idx < P.npm; idx++) {
     int i = P.npb + idx;
     v3<T> pi = ld3(S.pos, i);
@@ -574,6 +578,8 @@ template <class T> static void interaction(const Par<T>& P, State<T>& S) {
   const T W0 = P.kernel_zero, Wdr = P.kernel_dr;
   const T art_exp = W0 / Wdr;
   std::vector<Partial<T>> part(P.npm);
+  // each iteration writes only slot i's rows; neighbours are read-only here
+#pragma omp parallel for schedule(dynamic, 256)
   for (int idx = 0; idx < P.npm; idx++) {
     int i = P.npb + idx;
     S.deferred[i] = 0;
@@ -925,6 +931,7 @@ template <class T> static void predictor(const Par<T>& P, State<T>& S) {
   std::memcpy(S.str_ini, S.str, sizeof(T) * 6 * (P.npb + P.npm));
   std::memcpy(S.stn_ini, S.stn, sizeof(T) * 6 * (P.npb + P.npm));
   const T hdt = T(0.5) * P.dt;
+#pragma omp parallel for schedule(static)
   for (int idx = 0; idx < P.npm; idx++) {
     int i = P.npb + idx;
     if (S.codes[4 * i + 1] != LQ_NONFIX) continue;
@@ -950,6 +957,7 @@ static inline int rebin_d(double p, double two_h) { return int(p >= two_h) - int
 
 // cuCorrectorBound, predictor_corrector.cu:109-137.
 template <class T> static void corrector_bound(const Par<T>& P, State<T>& S) {
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < P.npb; i++) {
     if (S.codes[4 * i + 1] != LQ_NONFIX) continue;
     const Motion<T>& m = P.mv[S.codes[4 * i + 3]];
@@ -987,6 +995,7 @@ template <class T> static void corrector_bound(const Par<T>& P, State<T>& S) {
 // cuCorrectorMaterial, predictor_corrector.cu:139-199.
 template <class T> static int corrector_material(const Par<T>& P, State<T>& S) {
   int left = 0;
+#pragma omp parallel for schedule(static) reduction(+ : left)
   for (int idx = 0; idx < P.npm; idx++) {
     int i = P.npb + idx;
     if (S.codes[4 * i + 1] != LQ_NONFIX) continue;

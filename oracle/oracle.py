@@ -44,6 +44,8 @@ def lib():
         from retainwall_code_amd import _abi
         P = C.POINTER
         L.or_initialize_parameters.argtypes = [P(_abi.Params)]
+        L.or_set_threads.argtypes = [C.c_int]
+        L.or_set_threads.restype = None
         for fn in ("or_kernel_f64", "or_grad_f64"):
             getattr(L, fn).argtypes = [C.c_double, C.c_double]
             getattr(L, fn).restype = C.c_double
@@ -236,6 +238,12 @@ def constitutive(model_type, dt, rho, w, D, sig, mp, dtype=np.float64):
     y = C.c_int(0)
     getattr(lib(), f"or_constitutive_{sfx}")(int(model_type), T(dt), T(rho), wv, Dv, sv, mv, C.byref(mc), C.byref(y))
     return np.array(sv[:], dtype=dtype), mc.value, bool(y.value)
+
+
+def set_threads(n):
+    """OpenMP threads for the oracle's per-particle loops (results are the
+    same for any count)."""
+    lib().or_set_threads(int(n))
 
 
 def kernel(knorm, q):
