@@ -50,6 +50,17 @@ def candidate_counts(cellpos, beg, end, p, soil):
     return tot
 
 
+WORKLOADS = {
+    "c1": "c1: retaining wall 0.4x0.2x0.3 m backfill, dx=0.005 m, wall translating away at 1 mm/s",
+    "c2": "c2: retaining wall 1.0x0.5x0.6 m backfill, dx=0.0063 m, toe rotation 0.01 rad/s (active), "
+          "jitter 0.05dx, void ratio U(0.65,0.75)",
+    "c2_dx004": "c2 at dx=0.004 m: retaining wall 1.0x0.5x0.6 m backfill, toe rotation 0.01 rad/s (active), "
+                "jitter 0.05dx, void ratio U(0.65,0.75)",
+    "c3": "c3: retaining wall 2.0x1.0x1.0 m backfill, dx=0.004 m, 15 deg sloping top, 10 kPa surface strip "
+          "0.5-1.0 m behind the wall, wall translating away at 1 mm/s",
+}
+
+
 def cpu_baseline(case_name, seconds_target=20.0):
     """FP32 oracle on the host cores for a bounded number of whole steps of
     the same case: its per-particle loops run on OpenMP threads (one per
@@ -346,8 +357,7 @@ def main():
         "metric": "particle-steps/sec", "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"{args.case}: retaining wall 1.0x0.5x0.6 m backfill, dx=0.0063 m, toe rotation "
-                               f"0.01 rad/s (active), jitter 0.05dx, void ratio U(0.65,0.75)",
+        "config": {"workload": WORKLOADS.get(args.case, f"{args.case} (tests' case generator)"),
                    "soil_particles": case.npm, "boundary_particles": case.npb, "cells": int(p.ncell),
                    "dt": float(p.dt) if p.dt else case.params["dt"], "graph": bool(args.graph),
                    "interaction_variant": vname, "tile_shape": tile_shape, "tile_tz": tile_tz,

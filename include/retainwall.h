@@ -278,6 +278,13 @@ int lq_wall_force(lq_ctx* ctx, const float box[3], double* force_kn);
  * read 0 (the reference leaves them unwritten).  Single-case contexts.
  * Synchronises. */
 int lq_wall_pressure_profile(lq_ctx* ctx, const float geom[4], int nbins, float* out, size_t bytes);
+/* Strip surcharge on the free surface: every soil slot whose block id
+ * (codes.x) equals block carries a downward force pressure * dr^2 (the load
+ * on one lattice column), added with gravity in the interaction.  block -1
+ * removes it; contact-wall ids (>= 10) are rejected.  No reference
+ * counterpart: the 10 kPa strip load of the large sloping-backfill workload,
+ * whose generator marks the top particle of each column under the strip. */
+int lq_set_surcharge(lq_ctx* ctx, int block, float pressure);
 
 /* Absolute positions (PositionFromRelative2Absolute, loquat_grid.cu:273-289)
  * into dst f32[np][4].  Synchronises. */

@@ -274,6 +274,10 @@ extern "C" int or_neighbor_lists_f64(const lq_params* p, or_state* s, int max_nb
 // Threads for the per-particle loops (results do not depend on it: every
 // parallel iteration writes only its own particle's rows).
 extern "C" void or_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+extern "C" void or_set_surcharge(int block, double pressure) {
+  g_sur_block = block < 0 ? -1 : block;
+  g_sur_pressure = pressure;
+}
 extern "C" int or_get_threads() { return omp_get_max_threads(); }
 
 // Wendland C2 value and gradient factor (functions.h:142-143) for tests.

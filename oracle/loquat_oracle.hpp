@@ -103,7 +103,14 @@ template <class T> struct Par {
   Motion<T> mv[LQ_RIGID_MOTION_MAX];
   int periodbound;
   T pl[4], pr[4];  // periodic planes a x + b y + c z + d (planeleft, planeright)
+  int sur_block;   // soil block id carrying the strip surcharge, -1 for none
+  T sur_force;     // its downward force per slot (pressure * dr^2)
 };
+
+// Strip surcharge of the large sloping-backfill workload (no reference
+// counterpart), set by or_set_surcharge and read by load_par.
+inline int g_sur_block = -1;
+inline double g_sur_pressure = 0.0;
 
 // Copy the float parameter block (already initialised by the reference rule
 // loquat.cu:130-190, see lq_initialize_parameters) into T; for T=double the
@@ -142,6 +149,8 @@ template <class T> static void load_par(const lq_params* p, Par<T>& P) {
   P.time = p->time; P.dt = p->dt;
   P.n_motion = p->n_motion;
   P.periodbound = p->periodbound;
+  P.sur_block = g_sur_block;
+  P.sur_force = g_sur_block < 0 ? T(0) : T(float(g_sur_pressure) * p->dr * p->dr);
   for (int k = 0; k < 4; k++) { P.pl[k] = p->planeleft[k]; P.pr[k] = p->planeright[k]; }
   for (int k = 0; k < LQ_RIGID_MOTION_MAX; k++) {
     const lq_rigid_motion& m = p->mv[k];
@@ -195,6 +204,14 @@ template <class T> static inline int linear_key(const Par<T>& P, v3<int> c) {
 }
 template <class T> static inline bool in_grid(const Par<T>& P, v3<int> c) {
   return c.x >= 0 && c.y >= 0 && c.z >= 0 && c.x < P.cnx && c.y < P.cny && c.z < P.cnz;
+}
+
+// Gravity plus, on slots of the surcharge block, the strip load over the
+// slot's mass.
+template <class T> static inline v3<T> body_acc(const Par<T>& P, const int32_t* codes, int i, T mi) {
+  v3<T> g = P.g;
+  if (P.sur_block >= 0 && codes[4 * i] == P.sur_block) g.z = g.z - P.sur_force / mi;
+  return g;
 }
 
 // ------------------------------------------------ periodic boundaries ----
@@ -662,7 +679,8 @@ template <class T> static void interaction(const Par<T>& P, State<T>& S) {
     v3<T> damp = vi * P.damping;
     v3<T> s3 = (as + av) + aa;
     T mi = S.idm[4 * i + 1];
-    v3<T> acc0 = (s3 - damp) + P.g;
+    const v3<T> gi = body_acc(P, S.codes, i, mi);
+    v3<T> acc0 = (s3 - damp) + gi;
     // predicted velocity v + a dt (particle_interaction.cu:287)
     v3<T> vp = V3(mad(acc0.x, P.dt, vi.x), mad(acc0.y, P.dt, vi.y), mad(acc0.z, P.dt, vi.z));
     v3<T> nb = {S.cmag[9 * i + 3], S.cmag[9 * i + 4], S.cmag[9 * i + 5]};
@@ -687,7 +705,7 @@ template <class T> static void interaction(const Par<T>& P, State<T>& S) {
       }
     }
     part[idx] = {s3, damp, accden, mi};
-    v3<T> acc = ((s3 + ast) - damp) + P.g;
+    v3<T> acc = ((s3 + ast) - damp) + gi;
     S.accad[4 * i] = acc.x; S.accad[4 * i + 1] = acc.y; S.accad[4 * i + 2] = acc.z; S.accad[4 * i + 3] = accden;
     sym6<T> D = {gv[0], gv[4], gv[8], T(0.5) * (gv[1] + gv[3]), T(0.5) * (gv[2] + gv[6]), T(0.5) * (gv[5] + gv[7])};
     st6(S.acestn, i, D);
@@ -739,7 +757,7 @@ template <class T> static void interaction(const Par<T>& P, State<T>& S) {
     S.cvar[4 * o.i] = o.K;
     S.cmag[9 * o.i] = o.f.x; S.cmag[9 * o.i + 1] = o.f.y; S.cmag[9 * o.i + 2] = o.f.z;
     v3<T> ast = o.f * (T(1) / pt.mass);
-    v3<T> acc = ((pt.s3 + ast) - pt.damp) + P.g;
+    v3<T> acc = ((pt.s3 + ast) - pt.damp) + body_acc(P, S.codes, o.i, pt.mass);
     S.accad[4 * o.i] = acc.x; S.accad[4 * o.i + 1] = acc.y; S.accad[4 * o.i + 2] = acc.z;
   }
 }

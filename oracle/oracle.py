@@ -46,6 +46,8 @@ def lib():
         L.or_initialize_parameters.argtypes = [P(_abi.Params)]
         L.or_set_threads.argtypes = [C.c_int]
         L.or_set_threads.restype = None
+        L.or_set_surcharge.argtypes = [C.c_int, C.c_double]
+        L.or_set_surcharge.restype = None
         for fn in ("or_kernel_f64", "or_grad_f64"):
             getattr(L, fn).argtypes = [C.c_double, C.c_double]
             getattr(L, fn).restype = C.c_double
@@ -244,6 +246,12 @@ def set_threads(n):
     """OpenMP threads for the oracle's per-particle loops (results are the
     same for any count)."""
     lib().or_set_threads(int(n))
+
+
+def set_surcharge(block, pressure):
+    """Strip surcharge for the following oracle calls: slots of soil block
+    `block` carry pressure * dr^2 downward (block -1: none)."""
+    lib().or_set_surcharge(int(block), float(pressure))
 
 
 def kernel(knorm, q):

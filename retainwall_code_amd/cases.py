@@ -108,6 +108,7 @@ class Case:
     motion: list = field(default_factory=list)
     probe_box: tuple = (0.0, 0.0, 0.0)
     name: str = "case"
+    surcharge: tuple = None  # (soil block id, pressure Pa) of a strip load, or None
 
     @property
     def np(self):
@@ -149,8 +150,8 @@ def retaining_wall(box_length=0.4, box_width=0.2, box_height=0.3, dr=0.005, hdr=
                    fix_wall_height=0.04, wall_extra=0.05, rho0=1600.0, material=None,
                    cs=65.0, cfl=0.1, alpha_visc=1.5, art_p=0.01, gravity_z=-9.81, K0=0.5,
                    motion=("translate", -0.001), jitter=0.0, void_range=None, seed=0,
-                   time_max=1.0, shepard=0, model_type=_abi.HYPO_CS, periodic_y=False,
-                   name="retaining_wall") -> Case:
+                   time_max=1.0, shepard=0, model_type=_abi.HYPO_CS, periodic_y=False, slope_deg=0.0,
+                   surcharge=None, name="retaining_wall") -> Case:
     """Build a GenCase case-1 style wall case.
 
     box_* are the backfill dimensions (x away from the wall, y along it, z up).
@@ -164,6 +165,12 @@ def retaining_wall(box_length=0.4, box_width=0.2, box_height=0.3, dr=0.005, hdr=
     hypo_basic_soil) for any model other than the default HypoCS.
     periodic_y makes the case periodic along y (the wall-parallel direction,
     which has no walls): planes y = 0 and y = npy dr, one lattice period.
+    slope_deg tilts the backfill top, rising away from the wall, about the
+    middle of the backfill (the mean height stays box_height).  surcharge =
+    (pressure, x0, x1) puts a strip load on the free surface between x0 and
+    x1 behind the wall face: the top soil particle of each lattice column
+    there gets block id 7 and carries pressure * dr^2 (lq_set_surcharge), and
+    the initial stress below it includes the load (K0 laterally).
     """
     dr = f32(dr)
     hdr = f32(hdr)
@@ -174,7 +181,9 @@ def retaining_wall(box_length=0.4, box_width=0.2, box_height=0.3, dr=0.005, hdr=
     bl, bw, bh = f32(box_length), f32(box_width), f32(box_height)
     dom_x = bl + fwh + f32(2.0) * bt  # GenCase.cu:366
     dom_y = bw + f32(2.0) * bt
-    dom_z = f32(float(mwh + fwh) + 2.0 * float(bt))  # 2.0 is a double literal there
+    tan_s = math.tan(math.radians(slope_deg))
+    soil_top = float(bh) + tan_s * float(bl) / 2.0  # highest backfill point
+    dom_z = f32(float(max(float(mwh), soil_top) + fwh) + 2.0 * float(bt))  # 2.0 is a double literal there
     npx = max(int(dom_x / dr), 1)
     npy = max(int(dom_y / dr), 1)
     npz = int(dom_z / dr)
@@ -192,7 +201,11 @@ def retaining_wall(box_length=0.4, box_width=0.2, box_height=0.3, dr=0.005, hdr=
     left = (z > fwh + bt) & (z <= fwh + bt + mwh) & (x >= fwh) & (x <= fwh + bt)
     right = (z >= bt) & (z <= fwh + bt + mwh) & (x >= fwh + bl + bt) & (x <= fwh + bl + two * bt)
     bmask = bottom | bed | left | right
-    specimen = (z > bt) & (z <= bh + fwh + bt) & (x > fwh + bt) & (x <= fwh + bt + bl)
+    if slope_deg:
+        ztop = (float(bh + fwh + bt) + tan_s * ((x - float(fwh + bt)) - float(bl) / 2.0)).astype(np.float32)
+        specimen = (z > bt) & (z <= ztop) & (x > fwh + bt) & (x <= fwh + bt + bl)
+    else:
+        specimen = (z > bt) & (z <= bh + fwh + bt) & (x > fwh + bt) & (x <= fwh + bt + bl)
     bidx = np.nonzero(bmask)[0]
     sidx = np.nonzero(specimen)[0]
     npb, npm = len(bidx), len(sidx)
@@ -215,7 +228,23 @@ def retaining_wall(box_length=0.4, box_width=0.2, box_height=0.3, dr=0.005, hdr=
     rho0 = f32(rho0)
     # K0 geostatic stress from the untouched lattice height (GenCase.cu:483-485)
     zs = z[sidx]
-    szz = (rho0 * f32(gravity_z) * (-zs + bt + bh + fwh)).astype(np.float32)
+    if slope_deg:
+        szz = (rho0 * f32(gravity_z) * (-zs + ztop[sidx])).astype(np.float32)
+    else:
+        szz = (rho0 * f32(gravity_z) * (-zs + bt + bh + fwh)).astype(np.float32)
+    sur = None
+    if surcharge is not None:
+        q, sx0, sx1 = surcharge
+        xs = x[sidx] - float(fwh + bt)
+        strip = (xs >= sx0) & (xs <= sx1)
+        col = (np.round((x[sidx] - half) / dr).astype(np.int64) * npy
+               + np.round((y[sidx] - half) / dr).astype(np.int64))
+        top = np.full(col.max() + 1, -np.inf, np.float32)
+        np.maximum.at(top, col, zs)
+        surface = strip & (zs == top[col])
+        codes[npb:, 0][surface] = 7
+        szz = np.where(strip, szz - f32(q), szz).astype(np.float32)
+        sur = (7, float(q))
     stress = np.zeros((n, 6), np.float32)
     stress[npb:, 0] = szz * f32(K0)
     stress[npb:, 1] = szz * f32(K0)
@@ -258,7 +287,7 @@ def retaining_wall(box_length=0.4, box_width=0.2, box_height=0.3, dr=0.005, hdr=
     # wall-force probe box (utilities.cu:619-621 with the case's own numbers)
     box = (float(fwh + bt + f32(3.0) * dr), float(bt + fwh), float(bt + fwh + f32(1.0)))
     return Case(params, codes, pos, vel, idm, stress, np.zeros((n, 6), np.float32), normal,
-                [mt], box, name)
+                [mt], box, name, sur)
 
 
 def make_params(case: Case, key_mode=_abi.KEY_LINEAR, variant=_abi.INTERACTION_NEIGHBOR_LIST,
@@ -384,6 +413,9 @@ def config(name: str, **over) -> Case:
     elif name == "c2_dx004":  # literal dx 0.004 geometry (~5.0M soil particles)
         kw = dict(box_length=1.0, box_width=0.5, box_height=0.6, dr=0.004, motion=("rotate", -0.01),
                   jitter=0.05, void_range=(0.65, 0.75), seed=2024)
+    elif name == "c3":  # 2.0 x 1.0 x 1.0 m at dx 0.004 (~31M), 15 deg slope, 10 kPa strip, translation
+        kw = dict(box_length=2.0, box_width=1.0, box_height=1.0, dr=0.004, motion=("translate", -0.001),
+                  slope_deg=15.0, surcharge=(1.0e4, 0.5, 1.0), jitter=0.05, seed=3)
     elif name == "tiny":  # parity-test size (~6k particles)
         kw = dict(box_length=0.06, box_width=0.03, box_height=0.05, dr=0.0025, fix_wall_height=0.01,
                   wall_extra=0.01, motion=("translate", -0.001), jitter=0.05, seed=7)

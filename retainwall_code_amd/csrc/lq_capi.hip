@@ -277,6 +277,7 @@ static void fill_static(const lq_params& p, DevParams& d) {
   d.half_dr = p.dr / 2;         // particle_interaction.cu:157
   d.mu = 0.58108f;              // particle_interaction.cu:188-189
   d.periodbound = p.periodbound ? 1 : 0;
+  d.surcharge_block = -1;
   d.plane_l = make_float4(p.planeleft[0], p.planeleft[1], p.planeleft[2], p.planeleft[3]);
   d.plane_r = make_float4(p.planeright[0], p.planeright[1], p.planeright[2], p.planeright[3]);
   d.npm = p.npm; d.np_delete_m = p.np_delete_m;
@@ -1293,6 +1294,25 @@ extern "C" int lq_wall_force(lq_ctx* c, const float box[3], double* F) {
   if (!c) return LQ_ERR_ARG;
   if (c->n_cases != 1) return fail(c, LQ_ERR_ARG, "lq_wall_force: batched context, use lq_wall_force_cases");
   return lq_wall_force_cases(c, box, F);
+}
+
+// Strip surcharge: soil slots whose block id (codes.x) equals block carry a
+// downward force pressure * dr^2 (the load on one lattice column); block -1
+// removes it.
+extern "C" int lq_set_surcharge(lq_ctx* c, int block, float pressure) {
+  if (!c) return LQ_ERR_ARG;
+  if (block >= 10) return fail(c, LQ_ERR_ARG, "lq_set_surcharge: block ids >= 10 are contact walls");
+  hipSetDevice(c->device);
+  DevParams d;
+  int st = pull_params(c, d);
+  if (st) return st;
+  d.surcharge_block = block < 0 ? -1 : block;
+  d.surcharge_force = block < 0 ? 0.f : pressure * c->par.dr * c->par.dr;
+  HIPCHECK(c, hipMemcpyAsync(c->dP, &d, sizeof(DevParams), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  c->hp.surcharge_block = d.surcharge_block;
+  c->hp.surcharge_force = d.surcharge_force;
+  return LQ_OK;
 }
 
 // GetValuesAlongLine (utilities.cu:224-367) with the case constants passed

@@ -172,7 +172,7 @@ __device__ __forceinline__ void interaction_epilogue(DevParams* __restrict__ P, 
   float3 damp = f3mul(vi, P->damping);
   float3 s3 = f3add(f3add(S.as, S.av), S.aa);
   float mi = A.idm[i].y;
-  const float3 g = make_float3(P->gx, P->gy, P->gz);
+  const float3 g = body_acc(P, A.codes, i, mi);
   float3 acc0 = f3add(f3sub(s3, damp), g);
   const float dt = P->dt;
   float3 vp = make_float3(fmaf(acc0.x, dt, vi.x), fmaf(acc0.y, dt, vi.y), fmaf(acc0.z, dt, vi.z));

@@ -53,7 +53,7 @@ struct DevCase {
 struct DevParams {
   // ---- static (set at create) ----
   int np, npb, cnx, cny, cnz, ncell, ncell_material, ncell_delete_m;
-  int key_mode, shepard, viscosity_type, n_motion, n_model, nbits, periodbound, pad1;
+  int key_mode, shepard, viscosity_type, n_motion, n_model, nbits, periodbound, surcharge_block;
   float h, i_h, dr, vol_def, bt, eta, two_h, r2cut;
   float gx, gy, gz, damping, knorm, kgrad, kernel_zero, kernel_dr;
   float art_exp, alpha_cs, art_p, cfl, cs, band, half_dr, mu;
@@ -64,6 +64,7 @@ struct DevParams {
   int plist_cap, plist_stride;
   int n_cases, pad_c0, pad_c1, pad_c2;
   float4 plane_l, plane_r;  // periodic planes a x + b y + c z + d = 0 (planeleft, planeright)
+  float surcharge_force, pad_s0, pad_s1, pad_s2;  // downward force on each surcharge-block slot (-1 block: none)
   const float (*model_par)[LQ_MATERIAL_SIZE_MAX];  // [n_model] material rows (device table)
   const HypoConst* hypo;                           // [n_model]
   const DevCase* cases;                            // [n_cases]
@@ -158,6 +159,15 @@ __device__ __forceinline__ bool in_grid(const DevParams* P, int cx, int cy, int 
   return cx >= 0 && cy >= 0 && cz >= 0 && cx < P->cnx && cy < P->cny && cz < P->cnz;
 }
 __device__ __forceinline__ void set_err(DevParams* P, int code) { atomicCAS(&P->err, 0, code); }
+
+// Gravity plus, on slots of the surcharge block, the strip load's force
+// over the slot's mass (a body force like gravity; no reference
+// counterpart, the load of the large sloping-backfill workload).
+__device__ __forceinline__ float3 body_acc(const DevParams* P, const int4* codes, int i, float mi) {
+  float3 g = make_float3(P->gx, P->gy, P->gz);
+  if (P->surcharge_block >= 0 && codes[i].x == P->surcharge_block) g.z = g.z - P->surcharge_force / mi;
+  return g;
+}
 
 // ------------------------------------------------ periodic boundaries ----
 // Dist2Plane (functions.h:134-136): signed distance to a x + b y + c z + d = 0.

@@ -684,7 +684,7 @@ __global__ void __launch_bounds__(256) k_contact_redistribute(const DevParams* _
       const float3 ast = f3mul(fr, 1.0f / s3m.w);
       const float3 vi = ld3(A.vel[i]);
       const float3 damp = f3mul(vi, P->damping);
-      const float3 g = make_float3(P->gx, P->gy, P->gz);
+      const float3 g = body_acc(P, A.codes, i, s3m.w);
       const float3 acc = f3add(f3sub(f3add(make_float3(s3m.x, s3m.y, s3m.z), ast), damp), g);
       const float4 old = B.accad[i];
       B.accad[i] = make_float4(acc.x, acc.y, acc.z, old.w);

@@ -104,6 +104,10 @@ class Context:
     def set_time(self, t, step):
         self._chk(self.lib.lq_set_time(self.h, t, step), "lq_set_time")
 
+    def set_surcharge(self, block, pressure):
+        """Strip load: slots of soil block `block` carry pressure * dr^2 downward."""
+        self._chk(self.lib.lq_set_surcharge(self.h, int(block), float(pressure)), "lq_set_surcharge")
+
     def checkpoint_write(self, path):
         """Whole device state, raw, for a bit-exact restart (lq_checkpoint_write)."""
         self._chk(self.lib.lq_checkpoint_write(self.h, str(path).encode()), "lq_checkpoint_write")
@@ -321,6 +325,8 @@ def open_case(case, key_mode=_abi.KEY_LINEAR, variant=_abi.INTERACTION_NEIGHBOR_
     ctx = Context(p, device, lib)
     ctx.upload(case)
     ctx.set_motion(cases.motion_tables(case), len(case.motion))
+    if case.surcharge is not None:
+        ctx.set_surcharge(*case.surcharge)
     if initial_sort:
         ctx.neighbor_search()
         ctx.sort_arrays_primary()
