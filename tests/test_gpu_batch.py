@@ -90,3 +90,58 @@ def test_batch_rejects_mismatched_physics():
     p1.cs_m = p1.cs_m * 2.0
     with pytest.raises(sim.LqError, match="cs"):
         sim.BatchContext([p0, p1])
+
+
+def mixed_model_batch():
+    """Three cases with different soil models: Drucker-Prager, basic
+    hypoplasticity and critical-state hypoplasticity."""
+    cs = small_batch(3, seed=11)
+    rows = [(_abi.DP_EPP, cases.dp_soil(phi_deg=25.0, psi_deg=5.0, cohesion=50.0)),
+            (_abi.HYPO_BASIC, cases.hypo_basic_soil(phi_c_deg=32.0)), (_abi.HYPO_CS, None)]
+    out = []
+    for c, (mt, row) in zip(cs, rows):
+        if row is None:
+            out.append(c)
+            continue
+        c.codes[c.npb:, 2] = mt
+        c.params["model_par"] = [row] + [[0.0] * 15] * 9
+        out.append(c)
+    return out
+
+
+def test_batch_mixed_models_equal_single_cases():
+    cs = mixed_model_batch()
+    say(f"[batch models] {len(cs)} cases with DP, basic and critical-state soil")
+    batch = sim.open_batch(cs)
+    singles = [sim.open_case(c) for c in cs]
+    for step in range(3):
+        batch.step(1)
+        for s in singles:
+            s.step(1)
+        batch.sync()
+        for k, s in enumerate(singles):
+            compare_case(batch, k, s, step)
+    batch.close()
+    for s in singles:
+        s.close()
+
+
+def test_batch_checkpoint_restart(tmp_path):
+    cs = small_batch(3, seed=13)
+    path = tmp_path / "batch.ckpt"
+    say(f"[batch checkpoint] {len(cs)} cases")
+    a = sim.open_batch(cs)
+    a.step(3)
+    a.checkpoint_write(path)
+    a.step(4, graph=True)
+    a.sync()
+    ref = [{n: bits(a.get_case(k, n)) for n in STATE} for k in range(len(cs))]
+    a.close()
+    b = sim.open_batch(cs)
+    b.checkpoint_read(path)
+    b.step(4, graph=True)
+    b.sync()
+    for k in range(len(cs)):
+        for n in STATE:
+            np.testing.assert_array_equal(bits(b.get_case(k, n)), ref[k][n], err_msg=f"case {k} {n}")
+    b.close()
