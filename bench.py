@@ -269,8 +269,9 @@ def main():
     args = ap.parse_args()
     args.graph = not args.eager
     if args.gpus != 1:
-        print(json.dumps({"error": "this benchmark runs on exactly one GPU; multi-GPU runs are not part of "
-                                   "this build (the batched/sharded design is described in DESIGN.md)",
+        print(json.dumps({"error": "this benchmark runs on exactly one GPU: independent cases shard across "
+                                   "GPUs with no exchange and a single case would shard by y-slabs with a 2h "
+                                   "halo, but multi-GPU runs are not implemented",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
