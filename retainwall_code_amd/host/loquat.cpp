@@ -148,18 +148,35 @@ void LoquatSph::RunSimulation(int argc, char* argv[]) {
   grid->GridConfig(par_host);
   grid->NeighborSearch();
   lq_check(ctx, lq_sort_arrays_primary(ctx), "SortArraysPrimary");
-  std::printf("Saving initial state ...\n");
-  CopyDataFromDeviceToHost();
-  io->SaveXml();
-  io->SaveParticles();
-
+  const bool resume = argc >= 4;
   t_initial = t_interval = std::chrono::steady_clock::now();
-  time = double(par_host->time);
-  step = 0;
-  output_time = 0.0;
   step_interval = 0;
-  lq_check(ctx, lq_set_time(ctx, time, 0), "lq_set_time");
-  std::ofstream th(project_name + "/time_history.txt");
+  if (resume) {
+    // loquat <project> <gpu> <checkpoint>: the device state from the
+    // checkpoint, the time loop's own state from its .host companion
+    const std::string ck = argv[3];
+    lq_check(ctx, lq_checkpoint_read(ctx, ck.c_str()), "lq_checkpoint_read");
+    std::ifstream hin(ck + ".host");
+    int out_no = 0;
+    if (!(hin >> time >> output_time >> step >> out_no)) {
+      std::cout << "Error! Cannot read " << ck << ".host !" << std::endl;
+      std::exit(EXIT_FAILURE);
+    }
+    par_host->output_number = out_no;
+    par_host->time = float(time);
+    step_interval = step;
+    std::cout << "Resuming from " << ck << " at time " << time << ", step " << step << std::endl;
+  } else {
+    std::printf("Saving initial state ...\n");
+    CopyDataFromDeviceToHost();
+    io->SaveXml();
+    io->SaveParticles();
+    time = double(par_host->time);
+    step = 0;
+    output_time = 0.0;
+    lq_check(ctx, lq_set_time(ctx, time, 0), "lq_set_time");
+  }
+  std::ofstream th(project_name + "/time_history.txt", resume ? std::ios::app : std::ios::trunc);
   // dt = cfl * h / cs in float, as StepCalculation (loquat.cu:368-369)
   const float dt = par_host->cfl * (par_host->h / par_host->cs_m);
   const double tmax = double(par_host->time_max), freq = double(par_host->output_frequency);
@@ -193,5 +210,11 @@ void LoquatSph::RunSimulation(int argc, char* argv[]) {
   }
   lq_check(ctx, lq_synchronize(ctx), "synchronize");
   par_host->time = float(time);
+  // restart point: a later run with a larger time-max continues from here
+  const std::string ck = project_name + "/restart.ckpt";
+  lq_check(ctx, lq_checkpoint_write(ctx, ck.c_str()), "lq_checkpoint_write");
+  std::ofstream hout(ck + ".host");
+  hout.precision(17);
+  hout << time << " " << output_time << " " << step << " " << par_host->output_number << std::endl;
   SummaryToCalculation();
 }

@@ -1,7 +1,9 @@
 // loquat.hpp -- LoquatSph, the simulation driver (loquat.h:25-126).
 //
 // Same public surface as the reference: LoquatSph(name) and
-// RunSimulation(argc, argv) with the `loquat <project> [gpu_id]` CLI.  The
+// RunSimulation(argc, argv) with the `loquat <project> [gpu_id]` CLI; a third
+// argument resumes from a checkpoint (every run ends by writing
+// <project>/restart.ckpt and its .host companion with the time loop's state).  The
 // step runs on the GPU through libretainwall.so; between save points the host
 // issues whole batches of steps (hipGraph replay) with no per-step syncs.
 #pragma once

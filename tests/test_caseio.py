@@ -137,3 +137,28 @@ def test_cli_vtu_output(tmp_path):
         assert len(arrays["connectivity"]) == n and arrays["types"].max() == 1
     assert total == case.np
     assert not os.path.exists(os.path.join(base, "0.dat"))  # format 2 writes no .dat
+
+
+@pytest.mark.gpu
+def test_cli_resume_continues_run(tmp_path):
+    """`loquat <case> 0` ends by writing <case>/restart.ckpt; a second run
+    with a larger time-max and the checkpoint as third argument continues it:
+    the time history of the two runs equals an uninterrupted run's."""
+    case = cases.config("tiny")
+    dt = case.params["dt"]
+    kw = dict(output_frequency=7.5 * dt)
+    full = caseio.write_case(case, str(tmp_path / "full"), "wall", time_max=40 * dt * 0.9999, **kw)
+    part = caseio.write_case(case, str(tmp_path / "part"), "wall", time_max=20 * dt * 0.9999, **kw)
+    print("[cli resume] case=tiny, 40 steps vs 20 + 20", flush=True)
+    for base, extra in ((full, []), (part, [])):
+        r = subprocess.run([CLI, base, "0", *extra], capture_output=True, text=True, cwd=str(tmp_path), timeout=120)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    ck = os.path.join(part, "restart.ckpt")
+    assert os.path.exists(ck) and os.path.exists(ck + ".host")
+    caseio.write_case(case, str(tmp_path / "part"), "wall", time_max=40 * dt * 0.9999, **kw)  # longer run
+    r = subprocess.run([CLI, part, "0", ck], capture_output=True, text=True, cwd=str(tmp_path), timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    a = open(os.path.join(full, "time_history.txt")).read()
+    b = open(os.path.join(part, "time_history.txt")).read()
+    assert len(a.splitlines()) >= 4
+    assert a == b
