@@ -397,7 +397,7 @@ static int create_impl(const lq_params* cases, int n_cases, int device, lq_ctx**
     if (p.n_motion < 0 || p.n_motion > LQ_RIGID_MOTION_MAX) return LQ_ERR_ARG;
     np_tot += p.np;
   }
-  if (np_tot >= (1 << 26)) return LQ_ERR_ARG;  // neighbour-list slot encoding
+  if (np_tot >= (1 << 26)) return LQ_ERR_ARG;  // neighbour-list and tile entries hold the slot in 26 bits
   lq_ctx* c = new lq_ctx();
   c->device = device;
   c->n_cases = n_cases;

@@ -467,8 +467,11 @@ __global__ void __launch_bounds__(256) k_interaction_ref(DevParams* __restrict__
 // and drains the rings at the end.  Lanes thus stay busy on pair terms
 // gathered over many ranges, while each particle still sums its pairs in
 // reference order (a FIFO keeps the insertion order).
-// An entry packs the slot j (bits 7..31), the cell offset (bits 0..5) and
-// the range (bit 6: 0 wall, 1 soil).
+// An entry packs the slot j (bits 6..31, so slots below 2^26, the context
+// limit) and the cell offset (bits 0..5, two bits per axis).  Whether j is a
+// wall or a soil neighbour needs no bit: wall slots sort before every soil
+// slot, so j < npb exactly when j came from a wall range, i.e. when the
+// entry is below npb << 6.
 constexpr int NL_WAVE = 64;
 constexpr int NL_WAVES = 1;
 constexpr int NL_CAP = 32;    // ring entries per lane (power of two)
@@ -482,12 +485,13 @@ struct NbrRec {
 };
 
 __device__ __forceinline__ NbrRec nl_load(const Arrays& A, const Buffers& B, int e) {
-  const float4* r = B.rec + 4 * size_t(unsigned(e) >> 7);
+  const float4* r = B.rec + 4 * size_t(unsigned(e) >> 6);
   return {r[0], r[1], r[2], r[3]};
 }
 
 __device__ __forceinline__ void nl_eval(const PairConst& K, IAcc& S, int e, const NbrRec& n, float3 pi, float two_h,
-                                        float i_h, float3 vi, float rhoi, Sym sri, float artpi, int shift) {
+                                        float i_h, float3 vi, float rhoi, Sym sri, float artpi, int shift,
+                                        unsigned wall_end) {
   // cell offset o in {-1, 0, 1} per axis, stored as o + 1: (o + 1) * 2h - 2h
   // is exact, so these equal float(o) * 2h of pair_r2
   float3 d;
@@ -496,18 +500,18 @@ __device__ __forceinline__ void nl_eval(const PairConst& K, IAcc& S, int e, cons
   d.z = (pi.z - n.p.z) - fmaf(float((e >> 4) & 3), two_h, -two_h);
   const float r = sqrtf(fmaf(d.z, d.z, fmaf(d.y, d.y, d.x * d.x)));
   const float q = r * i_h;
-  pair_terms(K, S, d, r, q, vi, rhoi, sri, artpi, shift, (e & 64) == 0, n.v, n.p.w, n.a, n.b);
+  pair_terms(K, S, d, r, q, vi, rhoi, sri, artpi, shift, unsigned(e) < wall_end, n.v, n.p.w, n.a, n.b);
 }
 
 // Evaluate the lane's `pop` oldest ring entries in order.
 __device__ __forceinline__ void nl_drain(const PairConst& K, IAcc& S, const Arrays& A, const Buffers& B,
                                          const int (*lst)[NL_WAVE], int lane, int head, int pop, float3 pi,
                                          float two_h, float i_h, float3 vi, float rhoi, Sym sri, float artpi,
-                                         int shift) {
+                                         int shift, unsigned wall_end) {
 #pragma unroll 1
   for (int k = 0; k < pop; k++) {
     const int en = lst[(head + k) & (NL_CAP - 1)][lane];
-    nl_eval(K, S, en, nl_load(A, B, en), pi, two_h, i_h, vi, rhoi, sri, artpi, shift);
+    nl_eval(K, S, en, nl_load(A, B, en), pi, two_h, i_h, vi, rhoi, sri, artpi, shift, wall_end);
   }
 }
 
@@ -520,6 +524,7 @@ __device__ __forceinline__ void nl_scan(const DevParams* __restrict__ P, const P
                                         bool active, float3 vi, float rhoi, Sym sri, float artpi, int shift) {
   const float two_h = P->two_h, i_h = P->i_h, r2cut = P->r2cut;
   const int ncm = P->ncell_material;
+  const unsigned wall_end = unsigned(P->npb) << 6;
 #pragma unroll 1
   for (int r = 0; r < 54; r++) {
     const int cell = r >> 1, pass = r & 1;
@@ -531,7 +536,7 @@ __device__ __forceinline__ void nl_scan(const DevParams* __restrict__ P, const P
       j = cbeg[c];
       e = cend[c];
     }
-    const unsigned code = unsigned((cx + 1) | ((cy + 1) << 2) | ((cz + 1) << 4) | (pass << 6));
+    const unsigned code = unsigned((cx + 1) | ((cy + 1) << 2) | ((cz + 1) << 4));
     while (j < e) {
       const int m = min(e - j, NL_CHUNK);
       unsigned mask = 0u;
@@ -550,14 +555,14 @@ __device__ __forceinline__ void nl_scan(const DevParams* __restrict__ P, const P
         for (int b = NL_CHUNK; b; b >>= 1)
           if (__ballot(over >= need + b)) need += b;
         const int pop = min(cnt, need);
-        nl_drain(K, S, A, B, lst, lane, head, pop, pi, two_h, i_h, vi, rhoi, sri, artpi, shift);
+        nl_drain(K, S, A, B, lst, lane, head, pop, pi, two_h, i_h, vi, rhoi, sri, artpi, shift, wall_end);
         head = (head + pop) & (NL_CAP - 1);
         cnt -= pop;
       }
       while (mask) {
         const int t = __ffs(mask) - 1;
         mask &= mask - 1u;
-        lst[(head + cnt) & (NL_CAP - 1)][lane] = int((unsigned(j + t) << 7) | code);
+        lst[(head + cnt) & (NL_CAP - 1)][lane] = int((unsigned(j + t) << 6) | code);
         cnt++;
       }
       j += m;
@@ -599,8 +604,9 @@ __global__ void __launch_bounds__(NL_WAVE * NL_WAVES, 5) k_interaction_nl(DevPar
 #pragma unroll
   for (int k = 0; k < 9; k++) S.gv[k] = 0.f;
   int head = 0, cnt = 0;  // ring: oldest entry, number of entries
+  const unsigned wall_end = unsigned(P->npb) << 6;
   nl_scan(P, K, S, A, B, cbeg, cend, lst, lane, head, cnt, ci, pi, active, vi, rhoi, sri, artpi, shift);
-  nl_drain(K, S, A, B, lst, lane, head, cnt, pi, two_h, i_h, vi, rhoi, sri, artpi, shift);
+  nl_drain(K, S, A, B, lst, lane, head, cnt, pi, two_h, i_h, vi, rhoi, sri, artpi, shift, wall_end);
   if constexpr (PERIODIC) {
     head = 0;
     cnt = 0;
@@ -609,7 +615,7 @@ __global__ void __launch_bounds__(NL_WAVE * NL_WAVES, 5) k_interaction_nl(DevPar
     const bool img = active && period_image(P, ci, pi, false, cpb, ppb);
     if (!img) { cpb = ci; ppb = pi; }
     nl_scan(P, K, S, A, B, cbeg, cend, lst, lane, head, cnt, cpb, ppb, img, vi, rhoi, sri, artpi, shift);
-    nl_drain(K, S, A, B, lst, lane, head, cnt, ppb, two_h, i_h, vi, rhoi, sri, artpi, shift);
+    nl_drain(K, S, A, B, lst, lane, head, cnt, ppb, two_h, i_h, vi, rhoi, sri, artpi, shift, wall_end);
   }
   if (active) interaction_epilogue(P, A, B, i, S, vi);
 }
