@@ -134,7 +134,8 @@ def bench_batch(args):
     ctx.step(args.warmup, graph=args.graph)
     ctx.sync()
     names = {_abi.INTERACTION_TILE: "tile", _abi.INTERACTION_REFERENCE_ORDER: "reference",
-             _abi.INTERACTION_NEIGHBOR_LIST: "neighbor_list", _abi.INTERACTION_PAIR_LIST: "pair_list"}
+             _abi.INTERACTION_NEIGHBOR_LIST: "neighbor_list", _abi.INTERACTION_PAIR_LIST: "pair_list",
+             _abi.INTERACTION_COLUMNS: "columns"}
     if args.variant == "auto":
         variant, shape, tz, remap, _ = ctx.autotune(reps=2)
     else:
@@ -252,7 +253,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--eager", action="store_true",
                     help="enqueue every kernel per step instead of replaying the captured two-step hipGraph")
-    ap.add_argument("--variant", default="auto", choices=["auto", "tile", "reference", "neighbor_list", "pair_list"],
+    ap.add_argument("--variant", default="auto",
+                    choices=["auto", "tile", "reference", "neighbor_list", "pair_list", "columns"],
                     help="interaction kernel variant (all bit-identical); auto times each and keeps the fastest")
     ap.add_argument("--remap", type=int, default=0, help="XCD block renumbering for reference/neighbor_list")
     ap.add_argument("--tile", default="", metavar="SHAPE,TZ", help="fix the tile shape/depth instead of autotuning")
@@ -294,7 +296,8 @@ def main():
     print(f"[bench] case={args.case} np={case.np} npb={case.npb} npm={case.npm}", file=sys.stderr, flush=True)
     from retainwall_code_amd import _abi
     names = {_abi.INTERACTION_TILE: "tile", _abi.INTERACTION_REFERENCE_ORDER: "reference",
-             _abi.INTERACTION_NEIGHBOR_LIST: "neighbor_list", _abi.INTERACTION_PAIR_LIST: "pair_list"}
+             _abi.INTERACTION_NEIGHBOR_LIST: "neighbor_list", _abi.INTERACTION_PAIR_LIST: "pair_list",
+             _abi.INTERACTION_COLUMNS: "columns"}
     ids = {v: k for k, v in names.items()}
     variant = ids.get(args.variant, _abi.INTERACTION_REFERENCE_ORDER)
     ctx = sim.open_case(case, variant=variant)
@@ -308,6 +311,9 @@ def main():
         ctx.set_tile(tile_shape, tile_tz)
     elif args.variant == "auto":
         variant, tile_shape, tile_tz, remap, _ = ctx.autotune(reps=3)  # untimed: pick the fastest kernel
+    elif args.variant == "columns":  # untimed: block order and register build of the COLUMNS kernel
+        ctx.set_interaction(variant, remap)
+        variant, tile_shape, tile_tz, remap, _ = ctx.autotune(reps=3)
     else:
         ctx.set_interaction(variant, remap)
     if variant != _abi.INTERACTION_TILE:
@@ -337,12 +343,13 @@ def main():
             rec = json.load(open(pmc))
             if rec.get("kernel") == {"tile": "k_interaction_tile", "reference": "k_interaction_ref",
                                      "neighbor_list": "k_interaction_nl",
-                                     "pair_list": "k_interaction_plist"}[vname]:
+                                     "pair_list": "k_interaction_plist", "columns": "k_interaction_col"}[vname]:
                 traffic = rec.get("hbm_bytes_per_launch")  # rocprofv3 FETCH/WRITE_SIZE, copy-calibrated
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": {"tile": "k_interaction_tile", "reference": "k_interaction_ref",
-                           "neighbor_list": "k_interaction_nl", "pair_list": "k_interaction_plist"}[vname],
+                           "neighbor_list": "k_interaction_nl", "pair_list": "k_interaction_plist",
+                           "columns": "k_interaction_col"}[vname],
                 "avg_launch_ms": round(avg_ms, 4),
                 "algorithmic_bytes_per_launch": bytes_call,
                 "valu_tflops": round(flops_call / (avg_ms * 1e-3) / 1e12, 2),

@@ -47,21 +47,36 @@ enum lq_shape { LQ_CUBOID = 0, LQ_CIRCLE = 1 };      /* types.h:49 */
  * bit for bit.  MORTON interleaves the cell coordinates instead (extension). */
 enum lq_key_mode { LQ_KEY_LINEAR = 0, LQ_KEY_MORTON = 1 };
 
-/* Interaction kernel variant.  All four produce bit-identical results.
- * REFERENCE_ORDER (default) walks the 27 cells per thread exactly as
- * particle_interaction.cu:117-140; TILE stages each cell-column tile's
- * neighbourhood in LDS and scans it from there; NEIGHBOR_LIST scans global
- * memory per lane and compacts hits into LDS lists before evaluating them;
+/* Interaction kernel variant.  The first four produce bit-identical results
+ * (the reference's per-particle summation order).  REFERENCE_ORDER walks the
+ * 27 cells per thread exactly as particle_interaction.cu:117-140; TILE stages
+ * each cell-column tile's neighbourhood in LDS and scans it from there;
+ * NEIGHBOR_LIST (the default of the Python and C++ hosts) scans global memory
+ * per lane and compacts hits into LDS lists before evaluating them;
  * PAIR_LIST scans once per step into per-slot pair lists in HBM (cut-off
  * plus a 1% skin, traversal order kept) that both interaction calls of the
  * step evaluate, falling back to the cell scan on the device when the
  * predictor moved a particle further than the skin allows or a list
- * overflowed.  lq_autotune picks the fastest for the current state. */
+ * overflowed.  COLUMNS searches h-sized sub-cells: per call it indexes every
+ * particle by 2h column, sub-column and sub-cell height, and each soil
+ * particle scans, in the 5 x 5 sub-columns around it, only the sub-cell
+ * window that can hold a neighbour (about 180 candidates where the 27-cell
+ * scan tests 450).  Its neighbour sets are the 27-cell scan's, bit for bit
+ * (same FP32 predicate), but it sums the pairs in column order with
+ * hardware reciprocal / square root / exp2 / log2, so its results agree with
+ * the others to FP32 rounding, not bit for bit.
+ * Size limits: REFERENCE_ORDER, TILE, NEIGHBOR_LIST and COLUMNS handle every
+ * context (fewer than 2^26 particles, the lq_create limit); PAIR_LIST needs
+ * fewer than 2^25 particles; COLUMNS needs LQ_KEY_LINEAR, no periodic planes
+ * and at most 256 cells along z.  lq_autotune picks the fastest for the
+ * current state within the bit-identical four, or between the two block
+ * orders of COLUMNS for a context set to COLUMNS. */
 enum lq_interaction_variant {
   LQ_INTERACTION_REFERENCE_ORDER = 0,
   LQ_INTERACTION_TILE = 1,
   LQ_INTERACTION_NEIGHBOR_LIST = 2,
-  LQ_INTERACTION_PAIR_LIST = 3
+  LQ_INTERACTION_PAIR_LIST = 3,
+  LQ_INTERACTION_COLUMNS = 4
 };
 
 /* RigidMotion, types.h:66-76 (the per-step imposed motion of one body). */

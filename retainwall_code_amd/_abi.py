@@ -22,6 +22,7 @@ FIX, NONFIX = 0, 1
 CUBOID, CIRCLE = 0, 1
 KEY_LINEAR, KEY_MORTON = 0, 1
 INTERACTION_REFERENCE_ORDER, INTERACTION_TILE, INTERACTION_NEIGHBOR_LIST, INTERACTION_PAIR_LIST = 0, 1, 2, 3
+INTERACTION_COLUMNS = 4
 MLP_IN, MLP_OUT, MLP_HIDDEN, MLP_NPARAM, MLP_NNORM = 14, 7, 128, 35847, 42
 
 # lq_field

@@ -25,8 +25,8 @@ LIB = os.path.join(HERE, "libretainwall.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("LQ_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["lq_physics.hip", "lq_grid.hip", "lq_interact.hip", "lq_pairlist.hip", "lq_surrogate.hip",
-               "lq_capi.hip"]
+HIP_SOURCES = ["lq_physics.hip", "lq_grid.hip", "lq_interact.hip", "lq_pairlist.hip", "lq_columns.hip",
+               "lq_surrogate.hip", "lq_capi.hip"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-fhip-fp32-correctly-rounded-divide-sqrt",

@@ -34,6 +34,7 @@ struct lq_ctx {
   int nbits = 0;
   int npm0 = 0;
   int tile_shape = 1, tile_tz = 7;
+  int col_wide = 0;  // COLUMNS: 128-VGPR build instead of the 96-VGPR one
   int xcd_remap = 1;  // renumber interaction blocks per XCD (lq_set_interaction / lq_autotune)
   long long host_step = 0;
   // instrumentation
@@ -265,6 +266,11 @@ static void fill_static(const lq_params& p, DevParams& d) {
     d.plist_cap = kPairListCap;
     d.plist_stride = std::max(1, p.npm);
   }
+  // sub-column windows of the COLUMNS variant: the cut-off radius squared
+  // with 1e-3 relative headroom, and a margin of 1% of h for float rounding
+  // and for particles the predictor moved past their sub-cell
+  d.col_r2 = d.r2cut * 1.001f;
+  d.col_margin = 0.01f * p.h;
   d.gx = p.gravity[0]; d.gy = p.gravity[1]; d.gz = p.gravity[2];
   d.damping = p.damping;
   d.knorm = p.kernel_normalization_par; d.kgrad = p.kernel_gradient_normalization_par;
@@ -380,6 +386,13 @@ static void layout_cases(lq_ctx* c, int gdim[3]) {
   gdim[0] = gx; gdim[1] = y + row_h; gdim[2] = gz;
 }
 
+// The COLUMNS variant needs linear keys (a 2h column's cells are
+// consecutive keys), at most col_max_cnz() cells along z and no periodic
+// image pass.
+static bool cols_supported(const lq_params& p) {
+  return p.key_mode == LQ_KEY_LINEAR && !p.periodbound && p.cell_num[2] <= col_max_cnz();
+}
+
 static int create_impl(const lq_params* cases, int n_cases, int device, lq_ctx** out) {
   if (!cases || !out || n_cases < 1) return LQ_ERR_ARG;
   *out = nullptr;
@@ -391,7 +404,10 @@ static int create_impl(const lq_params* cases, int n_cases, int device, lq_ctx**
     // periodic planes are one case's absolute coordinates: single-case
     // contexts only, with the interaction variants that run the image pass
     if (p.periodbound && (n_cases > 1 || p.interaction_variant == LQ_INTERACTION_TILE ||
-                          p.interaction_variant == LQ_INTERACTION_PAIR_LIST))
+                          p.interaction_variant == LQ_INTERACTION_PAIR_LIST ||
+                          p.interaction_variant == LQ_INTERACTION_COLUMNS))
+      return LQ_ERR_ARG;
+    if (p.interaction_variant < LQ_INTERACTION_REFERENCE_ORDER || p.interaction_variant > LQ_INTERACTION_COLUMNS)
       return LQ_ERR_ARG;
     if (p.key_mode != LQ_KEY_LINEAR) return LQ_ERR_ARG;
     if (p.n_motion < 0 || p.n_motion > LQ_RIGID_MOTION_MAX) return LQ_ERR_ARG;
@@ -471,6 +487,13 @@ static int create_impl(const lq_params* cases, int n_cases, int device, lq_ctx**
   st |= dalloc(c, &B.s3, n); st |= dalloc(c, &B.fdef, n); st |= dalloc(c, &B.deferred, n);
   st |= dalloc(c, &B.str_tmp_a, n); st |= dalloc(c, &B.str_tmp_b, n);
   st |= dalloc(c, &B.rec, 4 * n);
+  if (cols_supported(par)) {
+    st |= dalloc(c, &B.aux, n);
+    const size_t ncol = size_t(par.cell_num[0]) * size_t(par.cell_num[1]);
+    st |= dalloc(c, &B.sub_start, ncol * size_t(8 * par.cell_num[2] + 1));
+    st |= dalloc(c, &B.colcnt, ncol);
+    st |= dalloc(c, &B.colinfo, ncol);
+  }
   if (par.np < (1 << 25)) {  // pair-list entries hold the slot in 25 bits
     st |= dalloc(c, &B.plist, size_t(kPairListCap) * size_t(std::max(1, par.npm)));
     st |= dalloc(c, &B.pcount, size_t(std::max(1, par.npm)));
@@ -1055,6 +1078,9 @@ static void enq_interaction_kernel(lq_ctx* c) {
     launch_interaction_plist(c->stream, c->dP, c->npm0, c->B.cur, c->B, c->B.cell_beg, c->B.cell_end, c->xcd_remap);
     launch_interaction(c->stream, c->dP, c->npm0, c->B.cur, c->B, c->B.cell_beg, c->B.cell_end,
                        LQ_INTERACTION_PAIR_LIST, c->xcd_remap, false);  // guarded cell-scan fallback
+  } else if (c->par.interaction_variant == LQ_INTERACTION_COLUMNS) {
+    launch_col_index(c->stream, c->dP, c->par.cell_num[0], c->par.cell_num[1], c->B.cur, c->B);
+    launch_interaction_col(c->stream, c->dP, c->par.np, c->npm0, c->B.cur, c->B, c->xcd_remap, c->col_wide);
   } else if (c->par.interaction_variant == LQ_INTERACTION_TILE) {
     if (c->npm0 > 0)
       launch_interaction_tile(c->stream, c->dP, c->par.cell_num[0], c->par.cell_num[1], c->par.cell_num[2], c->B.cur,
@@ -1073,7 +1099,9 @@ static void enq_plist_build(lq_ctx* c) {
 static int enq_interaction(lq_ctx* c, int stage) {
   const bool build = stage == 0 && c->par.interaction_variant == LQ_INTERACTION_PAIR_LIST;
   launch_reset_deferred(c->stream, c->dP, build ? 1 : 0);
-  launch_pair_prep(c->stream, c->dP, c->par.npb + c->npm0, c->B.cur, c->B);
+  // COLUMNS writes its records in aux order while it indexes the sub-cells
+  if (c->par.interaction_variant != LQ_INTERACTION_COLUMNS)
+    launch_pair_prep(c->stream, c->dP, c->par.npb + c->npm0, c->B.cur, c->B);
   if (build) {
     Section s(c, LQ_K_PAIR_LIST_BUILD);
     enq_plist_build(c);
@@ -1566,10 +1594,13 @@ extern "C" int lq_surrogate_log_read(lq_ctx* c, float* x, float* y, int max_rows
 // NEIGHBOR_LIST, whether blocks are renumbered per XCD.
 extern "C" int lq_set_interaction(lq_ctx* c, int variant, int xcd_remap) {
   if (!c) return LQ_ERR_ARG;
-  if (variant < LQ_INTERACTION_REFERENCE_ORDER || variant > LQ_INTERACTION_PAIR_LIST)
+  if (variant < LQ_INTERACTION_REFERENCE_ORDER || variant > LQ_INTERACTION_COLUMNS)
     return fail(c, LQ_ERR_ARG, "lq_set_interaction: variant out of range");
   if (variant == LQ_INTERACTION_PAIR_LIST && !c->B.plist)
     return fail(c, LQ_ERR_ARG, "lq_set_interaction: pair lists need fewer than 2^25 particles");
+  if (variant == LQ_INTERACTION_COLUMNS && !c->B.aux)
+    return fail(c, LQ_ERR_ARG, "lq_set_interaction: COLUMNS needs linear keys, no periodic planes and at most " +
+                                   std::to_string(col_max_cnz()) + " cells along z");
   if (c->par.periodbound && (variant == LQ_INTERACTION_TILE || variant == LQ_INTERACTION_PAIR_LIST))
     return fail(c, LQ_ERR_ARG, "lq_set_interaction: periodic boundaries need REFERENCE_ORDER or NEIGHBOR_LIST");
   c->par.interaction_variant = variant;
@@ -1633,12 +1664,18 @@ extern "C" int lq_autotune(lq_ctx* c, int reps, int* variant_out, int* shape_out
   launch_pair_prep(c->stream, c->dP, c->par.npb + c->npm0, A, c->B);
   struct Cand { int variant, shape, tz, remap; };
   std::vector<Cand> cands;
-  for (int remap = 0; remap < 2; remap++) {
+  // COLUMNS sums in another order than the others (which agree bit for bit),
+  // so it is a candidate only for a context already set to it, and then the
+  // only one
+  const bool cols = c->par.interaction_variant == LQ_INTERACTION_COLUMNS;
+  for (int remap = 0; remap < 2 && cols; remap++)
+    for (int wide = 0; wide < 2; wide++) cands.push_back({LQ_INTERACTION_COLUMNS, wide, 0, remap});
+  for (int remap = 0; remap < 2 && !cols; remap++) {
     cands.push_back({LQ_INTERACTION_REFERENCE_ORDER, c->tile_shape, c->tile_tz, remap});
     cands.push_back({LQ_INTERACTION_NEIGHBOR_LIST, c->tile_shape, c->tile_tz, remap});
     if (c->B.plist && !c->par.periodbound) cands.push_back({LQ_INTERACTION_PAIR_LIST, c->tile_shape, c->tile_tz, remap});
   }
-  for (int sh = 0; sh < (c->par.periodbound ? 0 : interaction_tile_shapes()); sh++) {
+  for (int sh = 0; sh < (c->par.periodbound || cols ? 0 : interaction_tile_shapes()); sh++) {
     const int waves = interaction_tile_waves(sh);
     const int base = std::max(1, (waves * 64 * 9) / (10 * 16));  // ~90% of the lanes at ~16 soil per cell
     for (int tz = std::max(1, base - 2); tz <= std::min(interaction_tile_maxtz(sh), base + 2); tz++)
@@ -1648,12 +1685,20 @@ extern "C" int lq_autotune(lq_ctx* c, int reps, int* variant_out, int* shape_out
   hipEventCreate(&e0);
   hipEventCreate(&e1);
   double best = 1e30;
-  Cand win = {c->par.interaction_variant, c->tile_shape, c->tile_tz, c->xcd_remap};
-  for (const Cand& k : cands) {
+  // for COLUMNS, shape is the register build (1: 128 VGPRs)
+  auto apply = [&](const Cand& k) {
     c->par.interaction_variant = k.variant;
-    c->tile_shape = k.shape;
-    c->tile_tz = k.tz;
+    if (k.variant == LQ_INTERACTION_COLUMNS) {
+      c->col_wide = k.shape;
+    } else {
+      c->tile_shape = k.shape;
+      c->tile_tz = k.tz;
+    }
     c->xcd_remap = k.remap;
+  };
+  Cand win = {c->par.interaction_variant, cols ? c->col_wide : c->tile_shape, c->tile_tz, c->xcd_remap};
+  for (const Cand& k : cands) {
+    apply(k);
     auto step_calls = [&]() {  // the two interaction calls of one step
       const bool pl = k.variant == LQ_INTERACTION_PAIR_LIST;
       launch_reset_deferred(c->stream, c->dP, pl ? 1 : 0);
@@ -1682,10 +1727,7 @@ extern "C" int lq_autotune(lq_ctx* c, int reps, int* variant_out, int* shape_out
   HIPCHECK(c, hipMemcpyAsync(A.cvar, T.cvar, n * 16, hipMemcpyDeviceToDevice, c->stream));
   HIPCHECK(c, hipMemcpyAsync(A.cf, T.cf, n * 16, hipMemcpyDeviceToDevice, c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
-  c->par.interaction_variant = win.variant;
-  c->tile_shape = win.shape;
-  c->tile_tz = win.tz;
-  c->xcd_remap = win.remap;
+  apply(win);
   c->layout_version++;
   if (variant_out) *variant_out = win.variant;
   if (shape_out) *shape_out = win.shape;
@@ -1708,7 +1750,12 @@ extern "C" int lq_neighbor_lists(lq_ctx* c, int max_nb, int32_t* counts, int32_t
   HIPCHECK(c, hipMalloc(&dc, std::max<size_t>(n, 1) * 4));
   hipError_t e = hipMalloc(&di, std::max<size_t>(n * max_nb, 1) * 4);
   if (e != hipSuccess) { hipFree(dc); return fail(c, LQ_ERR_HIP, "hipMalloc ids"); }
-  launch_neighbor_list(c->stream, c->dP, int(n), c->B.cur, c->B.cell_beg, c->B.cell_end, max_nb, dc, di);
+  if (c->par.interaction_variant == LQ_INTERACTION_COLUMNS) {  // the same windows as the interaction
+    launch_col_index(c->stream, c->dP, c->par.cell_num[0], c->par.cell_num[1], c->B.cur, c->B);
+    launch_neighbor_list_col(c->stream, c->dP, int(n), c->B.cur, c->B, max_nb, dc, di);
+  } else {
+    launch_neighbor_list(c->stream, c->dP, int(n), c->B.cur, c->B.cell_beg, c->B.cell_end, max_nb, dc, di);
+  }
   e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(counts, dc, n * 4, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(ids, di, n * max_nb * 4, hipMemcpyDeviceToHost, c->stream);

@@ -1,0 +1,458 @@
+// lq_columns.hip -- the COLUMNS interaction variant: candidates from
+// h-sized sub-cells instead of whole 2h cells.
+//
+// Every 2h cell splits into 2 x 2 x 2 sub-cells of edge h (sub-cell bit per
+// axis: relative position >= h).  Per interaction call an auxiliary index
+// lists every particle (wall and soil) grouped by 2h column (cx, cy), then
+// by sub-column (sx, sy), then by sub-cell height z' = 2 cz + sz, walls before
+// soil and slot order within a sub-cell.  A sub-column's sub-cells are thus
+// contiguous along z, and sub_start holds the dense exclusive start of every
+// (column, sub-column, z') bucket.  Each aux entry carries the particle's
+// relative position and (slot << 2 | cz & 3).
+//
+// The interaction kernel then visits, per soil particle, the 5 x 5
+// sub-columns within two sub-cells of its own; for each it takes only the
+// z' window that can hold a particle closer than 2h (from the distance of
+// the particle to the sub-column, with a 1% of h margin), one contiguous
+// range of the aux index.  The pair predicate is the reference's
+// r^2 <= r2_cut on the same FP32 operands (cell offsets from the 2h cells),
+// so the neighbour set equals that of the 27-cell scan
+// (particle_interaction.cu:117-140); the pairs are summed in column order
+// instead of the reference's cell order, and the per-pair terms use the
+// hardware reciprocal, square root, exp2 and log2 (one rounding each).  On
+// average 180 candidates are tested per particle where the 2h-cell scan
+// tests 450.
+#include <algorithm>
+#include "lq_device.hpp"
+#include "lq_launch.hpp"
+
+namespace lq {
+
+__device__ __forceinline__ int sub_bits(float p, float h) { return p >= h ? 1 : 0; }
+
+// Wall and soil counts of every 2h column (cx, cy): cells (cx, cy, 0..cnz-1),
+// one wave per column, its lanes over z.
+__global__ void __launch_bounds__(256) k_col_count(const DevParams* __restrict__ P, const int* __restrict__ cbeg,
+                                                   const int* __restrict__ cend, int2* __restrict__ colcnt) {
+  const int col = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (col >= P->cnx * P->cny) return;
+  const int k0 = col * P->cnz, ncm = P->ncell_material;
+  int nw = 0, ns = 0;
+  for (int z = lane; z < P->cnz; z += 64) {
+    const int bw = cbeg[k0 + z], bs = cbeg[ncm + k0 + z];
+    if (bw >= 0) nw += cend[k0 + z] - bw;
+    if (bs >= 0) ns += cend[ncm + k0 + z] - bs;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    nw += __shfl_xor(nw, o);
+    ns += __shfl_xor(ns, o);
+  }
+  if (lane == 0) colcnt[col] = make_int2(nw, ns);
+}
+
+// Exclusive scans of the column counts (one workgroup): the column's first
+// wall slot w0 and first soil slot s0 (columns are runs of consecutive keys,
+// so in slot order), with its counts.
+__global__ void __launch_bounds__(1024) k_col_scan(const DevParams* __restrict__ P, const int2* __restrict__ colcnt,
+                                                   int4* __restrict__ colinfo) {
+  __shared__ int2 wsum[16];
+  const int ncol = P->cnx * P->cny, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int cw = 0, cs = P->npb;
+  for (int c0 = 0; c0 < ncol; c0 += 1024) {
+    const int c = c0 + threadIdx.x;
+    const int2 v = c < ncol ? colcnt[c] : make_int2(0, 0);
+    int x = v.x, y = v.y;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int a = __shfl_up(x, o), b = __shfl_up(y, o);
+      if (lane >= o) { x += a; y += b; }
+    }
+    if (lane == 63) wsum[wv] = make_int2(x, y);
+    __syncthreads();
+    int px = 0, py = 0, tx = 0, ty = 0;
+    for (int w = 0; w < 16; w++) {
+      const int2 s = wsum[w];
+      if (w < wv) { px += s.x; py += s.y; }
+      tx += s.x; ty += s.y;
+    }
+    if (c < ncol) colinfo[c] = make_int4(cw + px + x - v.x, cs + py + y - v.y, v.x, v.y);
+    cw += tx;
+    cs += ty;
+    __syncthreads();
+  }
+}
+
+// One workgroup per 2h column: counts per bucket b = q * 2cnz + z'
+// (q = 2 sx + sy, z' = 2 cz + sz) for walls and soil, the exclusive scan into
+// sub_start, then each particle's place: its bucket's start, plus the
+// bucket's wall count for soil, plus the particles of its own cell and kind
+// before it in slot order that share its sub-cell (these are the run of
+// equal cz just before it, as slots follow the keys).  The same pass writes
+// the particle's 64-byte pair record at its aux position: position with the
+// mass in w (negative for contact walls), velocity and density,
+// sigma/rho^2, m/rho and trace(sigma/rho^2) (k_pair_prep's quantities).
+constexpr int COL_MAX_CNZ = 256;
+constexpr int COL_MAX_BUCKETS = 8 * COL_MAX_CNZ + 1;
+constexpr int COL_LDS_PTS = 8192;  // particles of a column whose codes stay in LDS
+constexpr int COL_THREADS = 512;
+__global__ void __launch_bounds__(COL_THREADS) k_col_build(const DevParams* __restrict__ P, Arrays A, Buffers B,
+                                                           const int4* __restrict__ colinfo) {
+  __shared__ int cntw[COL_MAX_BUCKETS];
+  __shared__ int start[COL_MAX_BUCKETS];
+  __shared__ unsigned char oct[COL_LDS_PTS];
+  __shared__ unsigned char czs[COL_LDS_PTS];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int cnz = P->cnz, npb = P->npb;
+  const int col = blockIdx.x;
+  const int4 ci = colinfo[col];
+  const int w0 = ci.x, s0 = ci.y, nw = ci.z, n = ci.z + ci.w;
+  const int nb = 8 * cnz;
+  const float h = P->h;
+  for (int b = tid; b <= nb; b += COL_THREADS) { cntw[b] = 0; start[b] = 0; }
+  __syncthreads();
+  auto slot_of = [&](int t) { return t < nw ? w0 + t : s0 + (t - nw); };
+  auto oct_of = [&](float4 p) { return (sub_bits(p.x, h) << 2) | (sub_bits(p.y, h) << 1) | sub_bits(p.z, h); };
+  auto bucket = [&](int o, int cz) { return (o >> 1) * (2 * cnz) + 2 * cz + (o & 1); };
+  for (int t = tid; t < n; t += COL_THREADS) {
+    const int j = slot_of(t);
+    const int o = oct_of(A.pos[j]);
+    const int cz = A.cellpos[j].z;
+    if (t < COL_LDS_PTS) { oct[t] = (unsigned char)o; czs[t] = (unsigned char)cz; }
+    atomicAdd(t < nw ? &cntw[bucket(o, cz)] : &start[bucket(o, cz)], 1);
+  }
+  __syncthreads();
+  if (tid < 64) {  // exclusive scan of cntw + cnts (start holds cnts) over nb + 1 buckets
+    int carry = 0;
+    const size_t sbase = size_t(col) * (nb + 1);
+    const int base = w0 + (s0 - npb);
+    for (int b0 = 0; b0 <= nb; b0 += 64) {
+      const int b = b0 + lane;
+      const int v = b <= nb ? cntw[b] + start[b] : 0;
+      int x = v;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+      }
+      if (b <= nb) {
+        start[b] = carry + x - v;
+        B.sub_start[sbase + b] = base + carry + x - v;
+      }
+      carry += __shfl(x, 63);
+    }
+  }
+  __syncthreads();
+  const int base = w0 + (s0 - npb);
+  const float vol_def = P->vol_def;
+  for (int t = tid; t < n; t += COL_THREADS) {
+    const bool soil = t >= nw;
+    const int j = slot_of(t);
+    const float4 p = A.pos[j];
+    int o, cz;
+    if (t < COL_LDS_PTS) { o = oct[t]; cz = czs[t]; }
+    else { o = oct_of(p); cz = A.cellpos[j].z; }
+    const int b = bucket(o, cz);
+    int rank = 0;
+    for (int u = t - 1; u >= (soil ? nw : 0); u--) {
+      int ou, cu;
+      if (u < COL_LDS_PTS) { ou = oct[u]; cu = czs[u]; }
+      else { const int ju = slot_of(u); ou = oct_of(A.pos[ju]); cu = A.cellpos[ju].z; }
+      if (cu != cz) break;
+      rank += ou == o;
+    }
+    const int k = base + start[b] + (soil ? cntw[b] : 0) + rank;
+    B.aux[k] = make_float4(p.x, p.y, p.z, __int_as_float((j << 2) | (cz & 3)));
+    const float4 v = A.vel[j];
+    const float rho = v.w;
+    const float m = soil ? fabsf(p.w) : vol_def * rho;
+    const Sym sr = sdiv(ldsym(A.str_a, A.str_b, j), rho * rho);
+    float4* rc = B.rec + 4 * size_t(k);
+    rc[0] = make_float4(p.x, p.y, p.z, p.w < 0.f ? -m : m);
+    rc[1] = v;
+    rc[2] = make_float4(sr.xx, sr.yy, sr.zz, sr.xy);
+    rc[3] = make_float4(sr.xz, sr.yz, m / rho, strace(sr));
+  }
+}
+
+// ------------------------------------------------------ the window ----
+// Per-lane geometry of the sub-column scan.
+struct ColHome {
+  int xs, ys, zs;     // sub-cell coordinates of the particle (its 2h cell is xs >> 1, ...)
+  float fx, fy, fz;   // position inside its sub-cell
+};
+
+__device__ __forceinline__ ColHome col_home(const DevParams* __restrict__ P, int4 ci, float3 pi) {
+  const float h = P->h;
+  ColHome g;
+  const int sx = sub_bits(pi.x, h), sy = sub_bits(pi.y, h), sz = sub_bits(pi.z, h);
+  g.xs = 2 * ci.x + sx; g.ys = 2 * ci.y + sy; g.zs = 2 * ci.z + sz;
+  g.fx = sx ? pi.x - h : pi.x;
+  g.fy = sy ? pi.y - h : pi.y;
+  g.fz = sz ? pi.z - h : pi.z;
+  return g;
+}
+
+// Lower bound of the distance along one axis from the particle (offset f in
+// its sub-cell) to the sub-cell d sub-cells away, less the margin.
+__device__ __forceinline__ float col_gap(int d, float f, float h, float margin) {
+  float g = 0.f;
+  if (d > 0) g = float(d) * h - f;
+  else if (d < 0) g = f - float(d + 1) * h;
+  return fmaxf(g - margin, 0.f);
+}
+
+// The aux range [k0, k1) of sub-column (dx, dy) around the particle, and the
+// cell offsets ox, oy of that sub-column; empty when no particle in it can be
+// within 2h.
+__device__ __forceinline__ void col_range(const DevParams* __restrict__ P, const int* __restrict__ sub_start,
+                                          const ColHome& g, int dx, int dy, float gx2, float gy2, bool active,
+                                          int& k0, int& k1, int& ox, int& oy) {
+  k0 = k1 = 0;
+  ox = oy = 0;
+  const int X = g.xs + dx, Y = g.ys + dy;
+  const int cnz = P->cnz;
+  if (!active || X < 0 || Y < 0 || X >= 2 * P->cnx || Y >= 2 * P->cny) return;
+  const float i_h = P->i_h;
+  const float rem = P->col_r2 - gx2 - gy2;
+  if (!(rem > 0.f)) return;
+  const float dz = __builtin_amdgcn_sqrtf(rem) + P->col_margin;
+  const int zlo = max(g.zs + max(-2, int(floorf((g.fz - dz) * i_h))), 0);
+  const int zhi = min(g.zs + min(2, int(floorf((g.fz + dz) * i_h))), 2 * cnz - 1);
+  if (zlo > zhi) return;
+  const int cxj = X >> 1, cyj = Y >> 1;
+  ox = cxj - (g.xs >> 1);
+  oy = cyj - (g.ys >> 1);
+  const size_t base = size_t(cxj * P->cny + cyj) * size_t(8 * cnz + 1) + size_t((2 * (X & 1) + (Y & 1)) * (2 * cnz));
+  k0 = sub_start[base + zlo];
+  k1 = sub_start[base + zhi + 1];
+}
+
+// z offset of an aux candidate from its stored cz & 3 and the particle's cz
+// (|offset| <= 1 inside the window): 0 -> 0, 1 -> +1, 3 -> -1.
+__device__ __forceinline__ int col_oz(int code, int cz) {
+  const int d = (code - cz) & 3;
+  return d == 3 ? -1 : d;
+}
+
+// ------------------------------------------------- pair terms (fast) ----
+// The terms of particle_interaction.cu:66-111 with one rounding per
+// reciprocal, square root and power: (W / W(dr))^(W(0)/W(dr)) as
+// exp2(e log2(W / W(dr))).
+// wj = the neighbour's mass (the Adami volume mass vol_def rho for walls),
+// negative for contact walls.
+__device__ __forceinline__ void pair_terms_fast(const PairConst& P, float i_kdr, IAcc& S, float3 d, float r, float q,
+                                                float3 vi, float rhoi, Sym sri, float artpi, int shift, float4 vj,
+                                                float wj, float4 sra, float4 srb) {
+  const bool contactj = wj < 0.f;
+  const float t = 2.0f - q;
+  const float dwdr = P.kgrad * (t * t * t);
+  const float3 g = f3mul(d, dwdr);
+  const float rhoj = vj.w;
+  const float mj = fabsf(wj);
+  float3 dv = f3sub(vi, ld3(vj));
+  if (contactj && shift == 1) dv = make_float3(0.f, 0.f, 0.f);
+  const float volj = srb.z;
+  const float3 dvv = f3mul(dv, volj);
+  S.gv[0] = fmaf(-dvv.x, g.x, S.gv[0]); S.gv[1] = fmaf(-dvv.x, g.y, S.gv[1]); S.gv[2] = fmaf(-dvv.x, g.z, S.gv[2]);
+  S.gv[3] = fmaf(-dvv.y, g.x, S.gv[3]); S.gv[4] = fmaf(-dvv.y, g.y, S.gv[4]); S.gv[5] = fmaf(-dvv.y, g.z, S.gv[5]);
+  S.gv[6] = fmaf(-dvv.z, g.x, S.gv[6]); S.gv[7] = fmaf(-dvv.z, g.y, S.gv[7]); S.gv[8] = fmaf(-dvv.z, g.z, S.gv[8]);
+  const float dvg = fmaf(dv.z, g.z, fmaf(dv.y, g.y, dv.x * g.x));
+  const float keep = contactj ? 0.f : 1.f;
+  S.accden = fmaf(keep * rhoi * dvg, volj, S.accden);
+  const float sxx = sri.xx + sra.x, syy = sri.yy + sra.y, szz = sri.zz + sra.z;
+  const float sxy = sri.xy + sra.w, sxz = sri.xz + srb.x, syz = sri.yz + srb.y;
+  const float fx = fmaf(sxz, g.z, fmaf(sxy, g.y, sxx * g.x));
+  const float fy = fmaf(syz, g.z, fmaf(syy, g.y, sxy * g.x));
+  const float fz = fmaf(szz, g.z, fmaf(syz, g.y, sxz * g.x));
+  const float mf = keep * mj;
+  float s = 0.f;
+  if (P.viscosity_type == 1) {
+    // m_j alpha cs h (dv.d) / ((r^2 + eta) 0.5 (rho_i + rho_j)), approaching pairs only
+    const float visc = fmaf(dv.z, d.z, fmaf(dv.y, d.y, dv.x * d.x));
+    const float den = fmaf(r, r, P.eta) * (rhoi + rhoj);
+    const float sv = 2.0f * P.alpha_cs * P.h * visc * __builtin_amdgcn_rcpf(den);
+    s = (visc < 0.f) ? mf * sv : 0.f;
+  }
+  S.as.x = fmaf(fx, mf, S.as.x); S.as.y = fmaf(fy, mf, S.as.y); S.as.z = fmaf(fz, mf, S.as.z);
+  S.av.x = fmaf(g.x, s, S.av.x); S.av.y = fmaf(g.y, s, S.av.y); S.av.z = fmaf(g.z, s, S.av.z);
+  if (P.art_p != 0.f) {
+    const float trj = srb.w;
+    float artp = artpi + (trj > 0.f ? trj * P.art_p : 0.f);
+    if (artp > 0.f && !contactj) {
+      const float ker = P.knorm * (t * t) * (t * t) * (q + 0.5f);
+      artp = artp * __builtin_amdgcn_exp2f(P.art_exp * __builtin_amdgcn_logf(ker * i_kdr));
+      const float sa = mj * artp;
+      S.aa.x = fmaf(-g.x, sa, S.aa.x); S.aa.y = fmaf(-g.y, sa, S.aa.y); S.aa.z = fmaf(-g.z, sa, S.aa.z);
+    }
+  }
+}
+
+// ----------------------------------------------------- interaction ----
+// One lane per aux entry, so the 64 lanes of a wave are consecutive
+// particles of one sub-column: for every window they read neighbouring
+// stretches of the same target sub-column, and their pair records (stored in
+// aux order) lie close together.  Wall entries leave their lane idle.  Hits
+// go to per-lane FIFO rings in LDS as (aux index << 6 | cell-offset code) and
+// are evaluated in ring order, as in the neighbour-list variant.
+constexpr int CL_CAP = 32;    // ring entries per lane (power of two)
+constexpr int CL_CHUNK = 32;  // candidates per hit mask
+
+__device__ __forceinline__ void cl_eval(const PairConst& K, float i_kdr, IAcc& S, const Buffers& B, int e, float3 pi,
+                                        float two_h, float i_h, float3 vi, float rhoi, Sym sri, float artpi,
+                                        int shift) {
+  const float4* rc = B.rec + 4 * size_t(unsigned(e) >> 6);
+  const float4 np_ = rc[0], nv = rc[1], na = rc[2], nb = rc[3];
+  float3 d;
+  d.x = (pi.x - np_.x) - fmaf(float(e & 3), two_h, -two_h);
+  d.y = (pi.y - np_.y) - fmaf(float((e >> 2) & 3), two_h, -two_h);
+  d.z = (pi.z - np_.z) - fmaf(float((e >> 4) & 3), two_h, -two_h);
+  const float r2 = fmaf(d.z, d.z, fmaf(d.y, d.y, d.x * d.x));
+  const float r = __builtin_amdgcn_sqrtf(r2);
+  pair_terms_fast(K, i_kdr, S, d, r, r * i_h, vi, rhoi, sri, artpi, shift, nv, np_.w, na, nb);
+}
+
+__device__ __forceinline__ void cl_drain(const PairConst& K, float i_kdr, IAcc& S, const Buffers& B,
+                                         const int (*lst)[64], int lane, int head, int pop, float3 pi, float two_h,
+                                         float i_h, float3 vi, float rhoi, Sym sri, float artpi, int shift) {
+#pragma unroll 1
+  for (int k = 0; k < pop; k++)
+    cl_eval(K, i_kdr, S, B, lst[(head + k) & (CL_CAP - 1)][lane], pi, two_h, i_h, vi, rhoi, sri, artpi, shift);
+}
+
+// MINW waves per SIMD: 5 holds the kernel to 96 VGPRs (a few spill), 4
+// gives it 128; lq_autotune times both.
+template <int MINW>
+__global__ void __launch_bounds__(64, MINW) k_interaction_col(DevParams* __restrict__ P, Arrays A, Buffers B,
+                                                              int remap) {
+  __shared__ int s_list[CL_CAP][64];
+  const int lane = threadIdx.x;
+  const int k = xcd_block(remap) * 64 + lane;
+  const int nidx = P->npb + P->npm;
+  const int kh = k < nidx ? k : 0;
+  const int i = __float_as_int(B.aux[kh].w) >> 2;
+  const bool active = k < nidx && i >= P->npb;
+  const float4* rh = B.rec + 4 * size_t(kh);
+  const float4 ph = rh[0], vri = rh[1], sria = rh[2], srib = rh[3];
+  const float3 pi = ld3(ph);
+  const int4 ci = A.cellpos[i];
+  const float3 vi = ld3(vri);
+  const float rhoi = vri.w;
+  const Sym sri = {sria.x, sria.y, sria.z, sria.w, srib.x, srib.y};
+  const float artpi = srib.w > 0.f ? srib.w * P->art_p : 0.f;
+  const int shift = int(A.cvar[i].z);
+  const float two_h = P->two_h, i_h = P->i_h, r2cut = P->r2cut, h = P->h, margin = P->col_margin;
+  const float i_kdr = __builtin_amdgcn_rcpf(P->kernel_dr);
+  const PairConst K = pair_const(P);
+  IAcc S;
+  S.accden = 0.f;
+  S.as = S.av = S.aa = make_float3(0.f, 0.f, 0.f);
+#pragma unroll
+  for (int q = 0; q < 9; q++) S.gv[q] = 0.f;
+  const ColHome g = col_home(P, ci, pi);
+  int head = 0, cnt = 0;
+#pragma unroll 1
+  for (int a = 0; a < 25; a++) {
+    const int dx = a / 5 - 2, dy = a % 5 - 2;
+    const float gx = col_gap(dx, g.fx, h, margin), gy = col_gap(dy, g.fy, h, margin);
+    int j, e, ox, oy;
+    col_range(P, B.sub_start, g, dx, dy, gx * gx, gy * gy, active, j, e, ox, oy);
+    const unsigned code_xy = unsigned((ox + 1) | ((oy + 1) << 2));
+    const float offx = float(ox) * two_h, offy = float(oy) * two_h;
+    while (j < e) {
+      const int m = min(e - j, CL_CHUNK);
+      unsigned mask = 0u;
+      unsigned long long zc = 0ull;  // oz + 1 of every candidate, 2 bits each
+#pragma unroll 4
+      for (int t = 0; t < m; t++) {
+        const float4 c = B.aux[j + t];
+        const int oz = col_oz(__float_as_int(c.w), ci.z);
+        zc |= (unsigned long long)(oz + 1) << (2 * t);
+        float3 d;
+        d.x = (pi.x - c.x) - offx;
+        d.y = (pi.y - c.y) - offy;
+        d.z = (pi.z - c.z) - float(oz) * two_h;
+        if (fmaf(d.z, d.z, fmaf(d.y, d.y, d.x * d.x)) <= r2cut) mask |= 1u << t;
+      }
+      const int over = cnt + __popc(mask) - CL_CAP;
+      if (__ballot(over > 0)) {
+        int need = 0;
+#pragma unroll
+        for (int b = CL_CHUNK; b; b >>= 1)
+          if (__ballot(over >= need + b)) need += b;
+        const int pop = min(cnt, need);
+        cl_drain(K, i_kdr, S, B, s_list, lane, head, pop, pi, two_h, i_h, vi, rhoi, sri, artpi, shift);
+        head = (head + pop) & (CL_CAP - 1);
+        cnt -= pop;
+      }
+      while (mask) {
+        const int t = __ffs(mask) - 1;
+        mask &= mask - 1u;
+        const unsigned oz = unsigned(zc >> (2 * t)) & 3u;
+        s_list[(head + cnt) & (CL_CAP - 1)][lane] = int((unsigned(j + t) << 6) | code_xy | (oz << 4));
+        cnt++;
+      }
+      j += m;
+    }
+  }
+  cl_drain(K, i_kdr, S, B, s_list, lane, head, cnt, pi, two_h, i_h, vi, rhoi, sri, artpi, shift);
+  if (active) interaction_epilogue(P, A, B, i, S, vi);
+}
+
+// Neighbour lists through the same windows (parity and debugging): for soil
+// slot npb + s the original ids of its neighbours in column order.
+__global__ void k_neighbor_list_col(const DevParams* __restrict__ P, Arrays A, Buffers B, int max_nb,
+                                    int* __restrict__ counts, int* __restrict__ ids) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= P->npm) return;
+  const int i = P->npb + idx;
+  const float3 pi = ld3(A.pos[i]);
+  const int4 ci = A.cellpos[i];
+  const float two_h = P->two_h, h = P->h, margin = P->col_margin;
+  const ColHome g = col_home(P, ci, pi);
+  int n = 0;
+  for (int a = 0; a < 25; a++) {
+    const int dx = a / 5 - 2, dy = a % 5 - 2;
+    const float gx = col_gap(dx, g.fx, h, margin), gy = col_gap(dy, g.fy, h, margin);
+    int j, e, ox, oy;
+    col_range(P, B.sub_start, g, dx, dy, gx * gx, gy * gy, true, j, e, ox, oy);
+    for (; j < e; j++) {
+      const float4 c = B.aux[j];
+      const int w = __float_as_int(c.w);
+      float3 d;
+      d.x = (pi.x - c.x) - float(ox) * two_h;
+      d.y = (pi.y - c.y) - float(oy) * two_h;
+      d.z = (pi.z - c.z) - float(col_oz(w, ci.z)) * two_h;
+      if (!(fmaf(d.z, d.z, fmaf(d.y, d.y, d.x * d.x)) <= P->r2cut)) continue;
+      if (n < max_nb) ids[size_t(idx) * max_nb + n] = A.orig_id[w >> 2];
+      n++;
+    }
+  }
+  counts[idx] = n;
+}
+
+static inline int cblocks(long long n, int b) { return int((n + b - 1) / b); }
+
+void launch_col_index(hipStream_t s, const DevParams* P, int cnx, int cny, Arrays A, const Buffers& B) {
+  const int ncol = cnx * cny;
+  hipLaunchKernelGGL(k_col_count, dim3(cblocks(ncol, 4)), dim3(256), 0, s, P, (const int*)B.cell_beg,
+                     (const int*)B.cell_end, B.colcnt);
+  hipLaunchKernelGGL(k_col_scan, dim3(1), dim3(1024), 0, s, P, (const int2*)B.colcnt, B.colinfo);
+  hipLaunchKernelGGL(k_col_build, dim3(ncol), dim3(COL_THREADS), 0, s, P, A, B, (const int4*)B.colinfo);
+}
+// np: lanes over every aux entry (the live ones are npb + npm of them)
+void launch_interaction_col(hipStream_t s, DevParams* P, int np, int npm_max, Arrays A, const Buffers& B,
+                            int xcd_remap, int wide) {
+  if (npm_max <= 0) return;
+  if (wide) hipLaunchKernelGGL(k_interaction_col<4>, dim3(cblocks(np, 64)), dim3(64), 0, s, P, A, B, xcd_remap);
+  else hipLaunchKernelGGL(k_interaction_col<5>, dim3(cblocks(np, 64)), dim3(64), 0, s, P, A, B, xcd_remap);
+}
+void launch_neighbor_list_col(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const Buffers& B, int max_nb,
+                              int* counts, int* ids) {
+  if (npm_max > 0)
+    hipLaunchKernelGGL(k_neighbor_list_col, dim3(cblocks(npm_max, 256)), dim3(256), 0, s, P, A, B, max_nb, counts,
+                       ids);
+}
+int col_max_cnz() { return COL_MAX_CNZ; }
+
+}  // namespace lq

@@ -64,7 +64,10 @@ struct DevParams {
   int plist_cap, plist_stride;
   int n_cases, pad_c0, pad_c1, pad_c2;
   float4 plane_l, plane_r;  // periodic planes a x + b y + c z + d = 0 (planeleft, planeright)
-  float surcharge_force, pad_s0, pad_s1, pad_s2;  // downward force on each surcharge-block slot (-1 block: none)
+  float surcharge_force;  // downward force on each surcharge-block slot (-1 block: none)
+  // COLUMNS interaction: squared search radius (the cut-off with a margin)
+  // and the distance margin of its sub-column windows
+  float col_r2, col_margin, pad_s2;
   const float (*model_par)[LQ_MATERIAL_SIZE_MAX];  // [n_model] material rows (device table)
   const HypoConst* hypo;                           // [n_model]
   const DevCase* cases;                            // [n_cases]
@@ -135,6 +138,14 @@ struct Buffers {
   int *skey[2], *mflag, *mkey, *mslot, *mkey_alt, *mslot_alt;
   int *cell_beg, *cell_end;
   int* wall_near;    // per cell: a wall particle lies in its 27-cell neighbourhood
+  // COLUMNS interaction: sub-cell index rebuilt per call (every particle as
+  // (relative position, slot << 2 | cz & 3), grouped by 2h column,
+  // sub-column and sub-cell height) and the start of every bucket; in this
+  // variant rec is indexed by aux position, not by slot
+  float4* aux;
+  int* sub_start;
+  int2* colcnt;   // per 2h column: wall and soil counts
+  int4* colinfo;  // per 2h column: first wall slot, first soil slot, counts
   int* hist;         // radix histogram [256][nblocks]
   unsigned long long* wf_acc;  // per case: exact wall-force sum, 128-bit fixed point (lo, hi)
 };

@@ -36,6 +36,14 @@ void launch_wall_force(hipStream_t s, const DevParams* P, int np, int n_cases, A
 void launch_neighbor_list(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const int* cb, const int* ce,
                           int max_nb, int* counts, int* ids);
 
+// COLUMNS interaction variant (lq_columns.hip)
+void launch_col_index(hipStream_t s, const DevParams* P, int cnx, int cny, Arrays A, const Buffers& B);
+// wide != 0: the 128-VGPR build (4 waves per SIMD) instead of the 96-VGPR one
+void launch_interaction_col(hipStream_t s, DevParams* P, int np, int npm_max, Arrays A, const Buffers& B,
+                            int xcd_remap, int wide);
+void launch_neighbor_list_col(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const Buffers& B, int max_nb,
+                              int* counts, int* ids);
+int col_max_cnz();
 void launch_stream_copy(hipStream_t s, const float4* a, float4* b, size_t n);
 int interaction_tile_shapes();
 int interaction_tile_maxtz(int shape);

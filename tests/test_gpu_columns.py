@@ -1,0 +1,128 @@
+"""The COLUMNS interaction variant (h-sized sub-cell windows).
+
+* neighbour sets: per soil particle the same original ids as the 27-cell
+  scan, on shuffled small cases and on the full-size c2 workload;
+* one interaction from the same state against the FP32 and FP64 oracles
+  (the summation order differs from the reference's, so against both the
+  FP64 tolerances apply), small case and c2;
+* whole steps against the FP64 oracle's steps, and graph replay equal to
+  eager launches.
+"""
+import numpy as np
+import pytest
+
+from conftest import relerr
+from retainwall_code_amd import _abi, cases, sim
+from test_gpu_parity import TOL_F64_RATE, TOL_F64_STATE, bits, oracle_from, say, shuffled
+
+pytestmark = pytest.mark.gpu
+COLS = _abi.INTERACTION_COLUMNS
+
+
+def _sets_equal(ctx_a, ctx_b, max_nb=200):
+    ca, ia = ctx_a.neighbor_lists(max_nb)
+    cb, ib = ctx_b.neighbor_lists(max_nb)
+    np.testing.assert_array_equal(ca, cb)
+    assert ca.max() <= max_nb
+    ia = np.where(np.arange(max_nb)[None, :] < ca[:, None], ia, -1)
+    ib = np.where(np.arange(max_nb)[None, :] < cb[:, None], ib, -1)
+    np.testing.assert_array_equal(np.sort(ia, axis=1), np.sort(ib, axis=1))
+    return int(ca.sum())
+
+
+@pytest.mark.parametrize("name,shuffle", [("tiny", True), ("small", True), ("small", False)])
+def test_columns_neighbor_sets(name, shuffle):
+    case = cases.config(name)
+    if shuffle:
+        case = shuffled(case, 4)
+    say(f"[columns sets] case={case.name}")
+    a = sim.open_case(case, variant=_abi.INTERACTION_NEIGHBOR_LIST)
+    b = sim.open_case(case, variant=COLS)
+    for _ in range(2):
+        a.step(3)
+        b.set_interaction(_abi.INTERACTION_NEIGHBOR_LIST)
+        b.step(3)  # same trajectory bit for bit, then compare the searches
+        b.set_interaction(COLS)
+        a.sync()
+        b.sync()
+        n = _sets_equal(a, b)
+        say(f"  {n} pairs, sets identical")
+    a.close()
+    b.close()
+
+
+def _interaction_vs_oracles(case, oracle_mod, steps):
+    ctx = sim.open_case(case, variant=COLS)
+    ctx.step(steps)
+    ctx.step_calculation()
+    ctx.neighbor_search()
+    ctx.sort_arrays_primary()
+    ctx.rigid_body_update()
+    ctx.adami()
+    ctx.find_corner()
+    ctx.sync()
+    o32 = oracle_from(ctx, oracle_mod, np.float32, case)
+    o64 = oracle_from(ctx, oracle_mod, np.float64, case)
+    o32.interaction()
+    o64.interaction()
+    ctx.interaction()
+    ctx.sync()
+    soil = slice(case.npb, case.np)
+    for k in ("accad", "acestn", "spin", "cmag"):
+        g = ctx.get(k)[soil]
+        e32, e64 = relerr(g, o32[k][soil]), relerr(g, o64[k][soil])
+        say(f"  {k:7s} relerr f32={e32:.2e} f64={e64:.2e}")
+        assert e32 <= TOL_F64_RATE and e64 <= TOL_F64_RATE, k
+    np.testing.assert_array_equal(ctx.get("cvar")[soil][:, 2], o32["cvar"][soil][:, 2])  # slip flags
+    ctx.close()
+
+
+def test_columns_interaction_small(oracle_mod):
+    say("[columns interaction] case=small")
+    _interaction_vs_oracles(cases.config("small"), oracle_mod, 4)
+
+
+@pytest.mark.timeout(300)
+def test_columns_c2_sets_and_interaction(oracle_mod):
+    c2 = cases.config("c2")
+    say(f"[columns c2] np={c2.np}")
+    a = sim.open_case(c2, variant=_abi.INTERACTION_NEIGHBOR_LIST)
+    b = sim.open_case(c2, variant=COLS)
+    n = _sets_equal(a, b, 160)
+    say(f"  {n} pairs, sets identical")
+    a.close()
+    b.close()
+    _interaction_vs_oracles(c2, oracle_mod, 2)
+
+
+def test_columns_whole_steps(oracle_mod):
+    case = cases.config("small")
+    say("[columns steps] case=small, 2 + 6 steps")
+    ctx = sim.open_case(case, variant=COLS)
+    ctx.step(2)
+    ctx.sync()
+    o64 = oracle_from(ctx, oracle_mod, np.float64, case)
+    o64.run(6)
+    ctx.step(6)
+    ctx.sync()
+    soil = slice(case.npb, case.np)
+    np.testing.assert_array_equal(ctx.get("orig_id"), o64["orig_id"])
+    for k in ("pos", "vel", "str"):
+        e = relerr(ctx.get(k)[soil], o64[k][soil])
+        say(f"  {k} relerr vs FP64 oracle {e:.2e}")
+        assert e <= TOL_F64_STATE, k
+    ctx.close()
+
+
+def test_columns_graph_equals_eager():
+    case = cases.config("small")
+    a = sim.open_case(case, variant=COLS)
+    b = sim.open_case(case, variant=COLS)
+    a.step(4)
+    b.step(4, graph=True)
+    a.sync()
+    b.sync()
+    for k in ("pos", "vel", "str", "cvar", "cmag"):
+        np.testing.assert_array_equal(bits(a.get(k)), bits(b.get(k)), err_msg=k)
+    a.close()
+    b.close()
