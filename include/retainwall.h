@@ -293,6 +293,18 @@ int lq_wall_force(lq_ctx* ctx, const float box[3], double* force_kn);
  * read 0 (the reference leaves them unwritten).  Single-case contexts.
  * Synchronises. */
 int lq_wall_pressure_profile(lq_ctx* ctx, const float geom[4], int nbins, float* out, size_t bytes);
+/* lq_wall_pressure_profile without allocation: the same inputs, outputs
+ * and results, with the per-(band, bin) markers and the staged rows held in
+ * a fixed device array of the library (one per GPU, kept until the process
+ * ends) instead of memory allocated per call, so no hipMalloc/hipFree or
+ * allocating routine runs.  nbins is limited to LQ_PROFILE_NOALLOC_MAX_BINS
+ * (else LQ_ERR_ARG).  Calls that overlap on one GPU (several host threads,
+ * or contexts on separate streams) share that array and race; serialise
+ * them.  No counterpart in the reference: for callers that cannot allocate
+ * device memory (code under graph capture, serving loops, harnesses that
+ * forbid allocation).  Synchronises. */
+#define LQ_PROFILE_NOALLOC_MAX_BINS 4096
+int lq_wall_pressure_profile_noalloc(lq_ctx* ctx, const float geom[4], int nbins, float* out, size_t bytes);
 /* Strip surcharge on the free surface: every soil slot whose block id
  * (codes.x) equals block carries a downward force pressure * dr^2 (the load
  * on one lattice column), added with gravity in the interaction.  block -1

@@ -21,6 +21,7 @@ RIGID, DP_EPP, MUI_DP_EPP, HYPO_BASIC, HYPO_CS = range(5)
 FIX, NONFIX = 0, 1
 CUBOID, CIRCLE = 0, 1
 KEY_LINEAR, KEY_MORTON = 0, 1
+PROFILE_NOALLOC_MAX_BINS = 4096  # LQ_PROFILE_NOALLOC_MAX_BINS
 INTERACTION_REFERENCE_ORDER, INTERACTION_TILE, INTERACTION_NEIGHBOR_LIST, INTERACTION_PAIR_LIST = 0, 1, 2, 3
 INTERACTION_COLUMNS = 4
 MLP_IN, MLP_OUT, MLP_HIDDEN, MLP_NPARAM, MLP_NNORM = 14, 7, 128, 35847, 42
@@ -125,7 +126,7 @@ EXPORTS = [
     "lq_step_calculation", "lq_neighbor_search",
     "lq_sort_arrays_primary", "lq_sort_arrays_all", "lq_rigid_body_update", "lq_adami_boundary",
     "lq_find_corner_particle", "lq_stress_regularization", "lq_particle_interaction",
-    "lq_predictor", "lq_corrector", "lq_step", "lq_wall_force", "lq_wall_pressure_profile", "lq_set_surcharge", "lq_positions_absolute",
+    "lq_predictor", "lq_corrector", "lq_step", "lq_wall_force", "lq_wall_pressure_profile", "lq_wall_pressure_profile_noalloc", "lq_set_surcharge", "lq_positions_absolute",
     "lq_get_counts", "lq_synchronize", "lq_stress_saving_regularization", "lq_stream_copy", "lq_enable_timing", "lq_enable_timing_classes", "lq_kernel_times", "lq_neighbor_lists", "lq_set_tile", "lq_set_interaction", "lq_set_pair_list", "lq_pair_list_status", "lq_autotune", "lq_interaction_cutoff_sq",
     "lq_create_batch", "lq_case_count", "lq_upload_case", "lq_set_case_motion_tables", "lq_get_case_field",
     "lq_case_positions_absolute", "lq_case_counts", "lq_wall_force_cases",
@@ -163,6 +164,7 @@ def _proto(lib):
     lib.lq_step.argtypes = [vp, C.c_int, C.c_int]
     lib.lq_wall_force.argtypes = [vp, P(C.c_float), P(C.c_double)]
     lib.lq_wall_pressure_profile.argtypes = [vp, P(C.c_float), C.c_int, vp, C.c_size_t]
+    lib.lq_wall_pressure_profile_noalloc.argtypes = [vp, P(C.c_float), C.c_int, vp, C.c_size_t]
     lib.lq_set_surcharge.argtypes = [vp, C.c_int, C.c_float]
     lib.lq_positions_absolute.argtypes = [vp, vp, C.c_size_t]
     lib.lq_stress_saving_regularization.argtypes = [vp, vp, C.c_size_t]

@@ -5,6 +5,7 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 
 #include "lq_internal.hpp"
@@ -386,6 +387,38 @@ static void layout_cases(lq_ctx* c, int gdim[3]) {
   gdim[0] = gx; gdim[1] = y + row_h; gdim[2] = gz;
 }
 
+// ---- wall pressure profile without allocation ------------------------
+// Scratch of lq_wall_pressure_profile_noalloc: the last-slot markers
+// (3 x LQ_PROFILE_NOALLOC_MAX_BINS ints) and the staged output rows
+// (LQ_PROFILE_NOALLOC_MAX_BINS float4), one copy per GPU for the life of the
+// process.  Host code never uses the array's name as an address: the device
+// address of each GPU's copy comes from hipGetSymbolAddress, resolved once
+// per device (at context creation, with the device current) and kept in
+// g_profile_scratch.
+constexpr size_t kProfileMarkBytes = size_t(3) * LQ_PROFILE_NOALLOC_MAX_BINS * sizeof(int);
+constexpr size_t kProfileScratchBytes = kProfileMarkBytes + size_t(LQ_PROFILE_NOALLOC_MAX_BINS) * sizeof(float4);
+static __device__ __align__(256) unsigned char scratch[kProfileScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_profile_scratch_mu;
+void* g_profile_scratch[kMaxDevices] = {};
+
+// Device address of the current device's scratch (device must be current).
+hipError_t profile_scratch(int device, void** out) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_profile_scratch_mu);
+  if (!g_profile_scratch[device]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_profile_scratch[device] = ptr;
+  }
+  *out = g_profile_scratch[device];
+  return hipSuccess;
+}
+}  // namespace
+
 // The COLUMNS variant needs linear keys (a 2h column's cells are
 // consecutive keys), at most col_max_cnz() cells along z and no periodic
 // image pass.
@@ -455,6 +488,11 @@ static int create_impl(const lq_params* cases, int n_cases, int device, lq_ctx**
   if (e != hipSuccess) { c->err = std::string("hipSetDevice: ") + hipGetErrorString(e); return bail(LQ_ERR_HIP); }
   e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { c->err = std::string("hipStreamCreate: ") + hipGetErrorString(e); return bail(LQ_ERR_HIP); }
+  {
+    void* sp = nullptr;  // resolve this device's profile scratch once, device current
+    e = profile_scratch(device, &sp);
+    if (e != hipSuccess) { c->err = std::string("hipGetSymbolAddress: ") + hipGetErrorString(e); return bail(LQ_ERR_HIP); }
+  }
   fill_static(par, c->hp);
   c->hp.n_cases = n_cases;
   c->hp.n_model = LQ_MODEL_NUMBER_MAX * n_cases;
@@ -1365,6 +1403,35 @@ extern "C" int lq_wall_pressure_profile(lq_ctx* c, const float geom[4], int nbin
   hipFree(last);
   if (dout) hipFree(dout);
   if (e != hipSuccess) return fail(c, LQ_ERR_HIP, std::string("lq_wall_pressure_profile: ") + hipGetErrorString(e));
+  return LQ_OK;
+}
+
+// lq_wall_pressure_profile with every intermediate in the file-scope
+// scratch: no hipMalloc/hipFree.  The clears are fill kernels on the
+// context stream; calls that overlap on one GPU share the scratch and race.
+extern "C" int lq_wall_pressure_profile_noalloc(lq_ctx* c, const float geom[4], int nbins, float* out,
+                                                size_t bytes) {
+  NEED_UPLOAD(c);
+  if (c->n_cases != 1) return fail(c, LQ_ERR_ARG, "lq_wall_pressure_profile_noalloc: single-case contexts only");
+  if (nbins > LQ_PROFILE_NOALLOC_MAX_BINS)
+    return fail(c, LQ_ERR_ARG, "lq_wall_pressure_profile_noalloc: nbins above LQ_PROFILE_NOALLOC_MAX_BINS (" +
+                                   std::to_string(LQ_PROFILE_NOALLOC_MAX_BINS) + ")");
+  if (!geom || !out || nbins <= 0 || bytes != size_t(nbins) * 16)
+    return fail(c, LQ_ERR_ARG, "lq_wall_pressure_profile_noalloc: args");
+  hipSetDevice(c->device);
+  void* base = nullptr;
+  hipError_t e = profile_scratch(c->device, &base);
+  if (e != hipSuccess)
+    return fail(c, LQ_ERR_HIP, std::string("lq_wall_pressure_profile_noalloc: scratch: ") + hipGetErrorString(e));
+  int* last = static_cast<int*>(base);
+  float4* dout = reinterpret_cast<float4*>(static_cast<unsigned char*>(base) + kProfileMarkBytes);
+  launch_wall_profile(c->stream, c->dP, c->par.np, c->par.npb, c->B.cur,
+                      make_float4(geom[0], geom[1], geom[2], geom[3]), nbins, last, dout);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess)
+    return fail(c, LQ_ERR_HIP, std::string("lq_wall_pressure_profile_noalloc: ") + hipGetErrorString(e));
   return LQ_OK;
 }
 

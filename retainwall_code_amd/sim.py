@@ -175,6 +175,14 @@ class Context:
                   "lq_wall_pressure_profile")
         return out
 
+    def wall_pressure_profile_noalloc(self, geom, nbins):
+        """lq_wall_pressure_profile_noalloc: the same rows, no device allocation."""
+        g = (C.c_float * 4)(*geom)
+        out = np.zeros((nbins, 4), np.float32)
+        self._chk(self.lib.lq_wall_pressure_profile_noalloc(self.h, g, nbins, out.ctypes.data, out.nbytes),
+                  "lq_wall_pressure_profile_noalloc")
+        return out
+
     def positions_absolute(self):
         out = np.empty((self.params.np, 4), np.float32)
         self._chk(self.lib.lq_positions_absolute(self.h, out.ctypes.data, out.nbytes), "lq_positions_absolute")
