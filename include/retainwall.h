@@ -358,6 +358,16 @@ int lq_set_tile(lq_ctx* ctx, int shape, int tz);
  * context accepts REFERENCE_ORDER and NEIGHBOR_LIST only.  No reference
  * counterpart. */
 int lq_set_interaction(lq_ctx* ctx, int variant, int xcd_remap);
+/* Fused steps of the COLUMNS variant (off by default): lq_step then advances
+ * every soil particle whose contact force needs no redistribution (pass B)
+ * through cuPredictorMaterial / cuCorrectorMaterial (predictor_corrector.cu
+ * :52-85, 139-199, with the HypoCS update) in the interaction kernel itself,
+ * and only the rest after pass B; the results are bit-identical to the
+ * unfused order.  It is slower on MI355X (the interaction's lanes follow the
+ * sub-cell index, so the per-particle state it then updates is gathered
+ * rather than streamed).  The piecewise calls (lq_particle_interaction,
+ * lq_predictor, lq_corrector) never fuse.  No reference counterpart. */
+int lq_set_fusion(lq_ctx* ctx, int on);
 /* PAIR_LIST capacity (entries per soil slot, 1..160; default 160) and skin
  * (list radius (1 + skin) * 2h, 0..0.1; default 0.01).  The lists are
  * reused in the corrector stage while no particle moved more than

@@ -36,6 +36,7 @@ struct lq_ctx {
   int npm0 = 0;
   int tile_shape = 1, tile_tz = 7;
   int col_wide = 0;  // COLUMNS: 128-VGPR build instead of the 96-VGPR one
+  int fuse = 0;      // COLUMNS steps: predictor / corrector fused into the interaction
   int xcd_remap = 1;  // renumber interaction blocks per XCD (lq_set_interaction / lq_autotune)
   long long host_step = 0;
   // instrumentation
@@ -530,6 +531,8 @@ static int create_impl(const lq_params* cases, int n_cases, int device, lq_ctx**
     const size_t ncol = size_t(par.cell_num[0]) * size_t(par.cell_num[1]);
     st |= dalloc(c, &B.sub_start, ncol * size_t(8 * par.cell_num[2] + 1));
     st |= dalloc(c, &B.colcnt, ncol);
+    st |= dalloc(c, &B.fl_pos, n);
+    st |= dalloc(c, &B.fl_cell, n);
     st |= dalloc(c, &B.colinfo, ncol);
   }
   if (par.np < (1 << 25)) {  // pair-list entries hold the slot in 25 bits
@@ -1196,6 +1199,35 @@ extern "C" int lq_particle_interaction(lq_ctx* c) { NEED_UPLOAD(c); hipSetDevice
 extern "C" int lq_predictor(lq_ctx* c) { NEED_UPLOAD(c); hipSetDevice(c->device); enq_predictor(c); return check_launch(c); }
 extern "C" int lq_corrector(lq_ctx* c) { NEED_UPLOAD(c); hipSetDevice(c->device); enq_corrector(c); return check_launch(c); }
 
+// A COLUMNS interaction with the predictor (stage 0) or corrector (stage 1)
+// fused in: every soil particle whose contact force needs no pass B is
+// advanced in the interaction kernel; pass B (reading first-layer positions
+// saved before their update) completes the deferred ones, which the finish
+// kernels then advance, with the walls' snapshot and the moving walls.
+// Bit-identical to enq_interaction + enq_predictor / enq_corrector.
+static bool fused(const lq_ctx* c) {
+  return c->fuse && c->par.interaction_variant == LQ_INTERACTION_COLUMNS && c->B.fl_pos;
+}
+static void enq_fused(lq_ctx* c, int stage) {
+  launch_reset_deferred(c->stream, c->dP, 0);
+  {
+    Section s(c, LQ_K_INTERACTION);
+    launch_col_index(c->stream, c->dP, c->par.cell_num[0], c->par.cell_num[1], c->B.cur, c->B);
+    launch_interaction_col(c->stream, c->dP, c->par.np, c->npm0, c->B.cur, c->B, c->xcd_remap, c->col_wide,
+                           stage + 1);
+  }
+  {
+    Section s(c, LQ_K_CONTACT);
+    launch_contact(c->stream, c->dP, c->npm0, c->B.cur, c->B, c->B.cell_beg, c->B.cell_end, true);
+  }
+  {
+    Section s(c, stage == 0 ? LQ_K_PREDICTOR : LQ_K_CORRECTOR);
+    if (stage == 0) launch_predictor_finish(c->stream, c->dP, c->B.cur, c->B);
+    else launch_corrector_finish(c->stream, c->dP, c->par.npb, c->par.n_motion, c->B.cur, c->B);
+    enq_constitutive(c, stage);
+  }
+}
+
 // One step (loquat.cu:366-436, 548-560).  The corrector-section neighbour
 // search and SortArraysAll are skipped: the predictor never re-bins a
 // particle and no key changes between the two searches, so the second
@@ -1209,12 +1241,20 @@ static int enq_step(lq_ctx* c) {
   enq_adami(c);
   enq_corner(c);
   if (shep != 0 && c->host_step % shep == 0 && c->host_step != 0) enq_shepard(c);
-  enq_interaction(c, 0);
-  enq_predictor(c);
+  if (fused(c)) {
+    enq_fused(c, 0);
+  } else {
+    enq_interaction(c, 0);
+    enq_predictor(c);
+  }
   enq_adami(c);
   enq_corner(c);
-  enq_interaction(c, 1);
-  enq_corrector(c);
+  if (fused(c)) {
+    enq_fused(c, 1);
+  } else {
+    enq_interaction(c, 1);
+    enq_corrector(c);
+  }
   launch_advance(c->stream, c->dP);
   c->host_step++;
   return 0;
@@ -1672,6 +1712,14 @@ extern "C" int lq_set_interaction(lq_ctx* c, int variant, int xcd_remap) {
     return fail(c, LQ_ERR_ARG, "lq_set_interaction: periodic boundaries need REFERENCE_ORDER or NEIGHBOR_LIST");
   c->par.interaction_variant = variant;
   c->xcd_remap = xcd_remap ? 1 : 0;
+  c->layout_version++;  // re-capture graphs
+  return LQ_OK;
+}
+
+// Fused COLUMNS steps on or off (off by default).
+extern "C" int lq_set_fusion(lq_ctx* c, int on) {
+  if (!c) return LQ_ERR_ARG;
+  c->fuse = on ? 1 : 0;
   c->layout_version++;  // re-capture graphs
   return LQ_OK;
 }

@@ -321,8 +321,14 @@ __device__ __forceinline__ void cl_drain(const PairConst& K, float i_kdr, IAcc& 
 }
 
 // MINW waves per SIMD: 5 holds the kernel to 96 VGPRs (a few spill), 4
-// gives it 128; lq_autotune times both.
-template <int MINW>
+// gives it 128; lq_autotune times both.  FUSE 1 / 2: a particle whose
+// contact force needs no pass B goes on through cuPredictorMaterial /
+// cuCorrectorMaterial (predictor_slot / corrector_slot) right after its
+// pair sum.  Nothing in this kernel reads another particle's state arrays
+// (neighbours come from aux and rec), so those updates race with nothing;
+// a first-layer particle first leaves its position and cell in fl_pos /
+// fl_cell for pass B.
+template <int MINW, int FUSE>
 __global__ void __launch_bounds__(64, MINW) k_interaction_col(DevParams* __restrict__ P, Arrays A, Buffers B,
                                                               int remap) {
   __shared__ int s_list[CL_CAP][64];
@@ -396,7 +402,18 @@ __global__ void __launch_bounds__(64, MINW) k_interaction_col(DevParams* __restr
     }
   }
   cl_drain(K, i_kdr, S, B, s_list, lane, head, cnt, pi, two_h, i_h, vi, rhoi, sri, artpi, shift);
-  if (active) interaction_epilogue(P, A, B, i, S, vi);
+  if (!active) return;
+  const bool deferred = interaction_epilogue(P, A, B, i, S, vi);
+  if constexpr (FUSE != 0) {
+    if (A.cvar[i].y <= P->band) {
+      B.fl_pos[i] = A.pos[i];
+      B.fl_cell[i] = ci;
+    }
+    if (!deferred) {
+      if constexpr (FUSE == 1) predictor_slot(P, A, B, i);
+      else corrector_slot(P, A, B, i);
+    }
+  }
 }
 
 // Neighbour lists through the same windows (parity and debugging): for soil
@@ -441,11 +458,17 @@ void launch_col_index(hipStream_t s, const DevParams* P, int cnx, int cny, Array
   hipLaunchKernelGGL(k_col_build, dim3(ncol), dim3(COL_THREADS), 0, s, P, A, B, (const int4*)B.colinfo);
 }
 // np: lanes over every aux entry (the live ones are npb + npm of them)
+template <int FUSE>
+static void launch_col(hipStream_t s, DevParams* P, int np, Arrays A, const Buffers& B, int xcd_remap, int wide) {
+  if (wide) hipLaunchKernelGGL((k_interaction_col<4, FUSE>), dim3(cblocks(np, 64)), dim3(64), 0, s, P, A, B, xcd_remap);
+  else hipLaunchKernelGGL((k_interaction_col<5, FUSE>), dim3(cblocks(np, 64)), dim3(64), 0, s, P, A, B, xcd_remap);
+}
 void launch_interaction_col(hipStream_t s, DevParams* P, int np, int npm_max, Arrays A, const Buffers& B,
-                            int xcd_remap, int wide) {
+                            int xcd_remap, int wide, int fuse) {
   if (npm_max <= 0) return;
-  if (wide) hipLaunchKernelGGL(k_interaction_col<4>, dim3(cblocks(np, 64)), dim3(64), 0, s, P, A, B, xcd_remap);
-  else hipLaunchKernelGGL(k_interaction_col<5>, dim3(cblocks(np, 64)), dim3(64), 0, s, P, A, B, xcd_remap);
+  if (fuse == 1) launch_col<1>(s, P, np, A, B, xcd_remap, wide);
+  else if (fuse == 2) launch_col<2>(s, P, np, A, B, xcd_remap, wide);
+  else launch_col<0>(s, P, np, A, B, xcd_remap, wide);
 }
 void launch_neighbor_list_col(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const Buffers& B, int max_nb,
                               int* counts, int* ids) {

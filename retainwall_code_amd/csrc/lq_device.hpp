@@ -167,7 +167,8 @@ __device__ __forceinline__ void pair_terms(const PairConst& P, IAcc& S, float3 d
 
 // Epilogue of cuParticleInteractionMaterial (particle_interaction.cu:284-295)
 // with pass A of the wall contact (LoopContact, :174-237).
-__device__ __forceinline__ void interaction_epilogue(DevParams* __restrict__ P, Arrays A, const Buffers& B, int i,
+// Returns whether the particle's contact force waits for pass B.
+__device__ __forceinline__ bool interaction_epilogue(DevParams* __restrict__ P, Arrays A, const Buffers& B, int i,
                                                      const IAcc& S, float3 vi) {
   float3 damp = f3mul(vi, P->damping);
   float3 s3 = f3add(f3add(S.as, S.av), S.aa);
@@ -209,6 +210,7 @@ __device__ __forceinline__ void interaction_epilogue(DevParams* __restrict__ P, 
   B.acestn_a[i] = make_float4(S.gv[0], S.gv[4], S.gv[8], 0.5f * (S.gv[1] + S.gv[3]));
   B.acestn_b[i] = make_float2(0.5f * (S.gv[2] + S.gv[6]), 0.5f * (S.gv[5] + S.gv[7]));
   B.spin[i] = make_float4(0.5f * (S.gv[3] - S.gv[1]), 0.5f * (S.gv[6] - S.gv[2]), 0.5f * (S.gv[7] - S.gv[5]), 0.f);
+  return deferred;
 }
 
 // ------------------------------------------------------------ HypoCS ----
@@ -452,6 +454,102 @@ __device__ __forceinline__ bool constitutive_other(const DevParams* __restrict__
   if (cd.z == LQ_MUI_DP_EPP) mui_dp(mp, dt, w, D, sig);
   else if (cd.z == LQ_HYPO_BASIC) hypo_basic(P, cd.w, dt, w, D, sig);
   return false;
+}
+
+// --------------------------------------------------------- predictor ----
+// LoquatSph::Predictor + cuPredictorMaterial, predictor_corrector.cu:52-103:
+// the *_ini snapshot and the half-step update in one pass.  Returns the
+// squared displacement of slot i (soil that moves; 0 otherwise).
+__device__ __forceinline__ float predictor_slot(const DevParams* __restrict__ P, Arrays A, const Buffers& B, int i) {
+  const int npm = P->npm, npb = P->npb;
+  float4 p = A.pos[i], v = A.vel[i];
+  B.pos_ini[i] = p;
+  B.vel_ini[i] = v;
+  if (npm == 0 || i >= npb + npm) return 0.f;
+  float4 sa = A.str_a[i], na = A.stn_a[i];
+  float2 sb = A.str_b[i], nb = A.stn_b[i];
+  B.str_ini_a[i] = sa; B.str_ini_b[i] = sb;
+  B.stn_ini_a[i] = na; B.stn_ini_b[i] = nb;
+  if (i < npb) return 0.f;
+  int4 cd = A.codes[i];
+  if (cd.y != LQ_NONFIX) return 0.f;
+  const float hdt = 0.5f * P->dt;
+  float4 a = B.accad[i];
+  v = make_float4(fmaf(a.x, hdt, v.x), fmaf(a.y, hdt, v.y), fmaf(a.z, hdt, v.z), fmaf(a.w, hdt, v.w));
+  const float4 p0 = p;
+  p = make_float4(fmaf(v.x, hdt, p.x), fmaf(v.y, hdt, p.y), fmaf(v.z, hdt, p.z), p.w);
+  const float* mp = P->model_par[cd.w];
+  float rho = fmaxf(v.w, mp[13]);
+  rho = fminf(rho, mp[14]);
+  v.w = rho;
+  Sym D = ldsym(B.acestn_a, B.acestn_b, i);
+  Sym stn = {na.x, na.y, na.z, na.w, nb.x, nb.y};
+  stsym(A.stn_a, A.stn_b, i, sadd(stn, smul(D, hdt)));
+  Sym sig = {sa.x, sa.y, sa.z, sa.w, sb.x, sb.y};
+  float Mcrt = 0.f;
+  bool yld = false;
+  if (cd.z == LQ_HYPO_CS) {
+    if (!P->sur_on) hypo_cs(P, cd.w, hdt, rho, ld3(B.spin[i]), D, sig, Mcrt);
+  } else {
+    yld = constitutive_other(P, cd, hdt, ld3(B.spin[i]), D, sig);
+  }
+  stsym(A.str_a, A.str_b, i, sig);
+  A.pos[i] = p;
+  A.vel[i] = v;
+  if (!yld) {  // a Drucker-Prager yield keeps the old value (predictor_corrector.cu:78-83)
+    float4 id = A.idm[i];
+    id.x = 100.f;
+    A.idm[i] = id;
+  }
+  const float dx = p.x - p0.x, dy = p.y - p0.y, dz = p.z - p0.z;
+  return dx * dx + dy * dy + dz * dz;
+}
+
+// ------------------------------------------------- corrector (slot) ----
+__device__ __forceinline__ int rebin(float p, float two_h) { return int(p >= two_h) - int(p < 0.0f); }
+
+// cuCorrectorMaterial, predictor_corrector.cu:139-199, for soil slot i.
+__device__ __forceinline__ void corrector_slot(const DevParams* __restrict__ P, Arrays A, const Buffers& B, int i) {
+  int4 cd = A.codes[i];
+  if (cd.y != LQ_NONFIX) return;
+  const float dt = P->dt, hdt = 0.5f * dt, two_h = P->two_h;
+  float4 vini = B.vel_ini[i], a = B.accad[i], pini = B.pos_ini[i];
+  float4 v = make_float4(fmaf(a.x, dt, vini.x), fmaf(a.y, dt, vini.y), fmaf(a.z, dt, vini.z), fmaf(a.w, dt, vini.w));
+  float3 p = make_float3(fmaf(v.x + vini.x, hdt, pini.x), fmaf(v.y + vini.y, hdt, pini.y), fmaf(v.z + vini.z, hdt, pini.z));
+  const float* mp = P->model_par[cd.w];
+  float rho = fmaxf(v.w, mp[13]);
+  rho = fminf(rho, mp[14]);
+  v.w = rho;
+  int4 ci = A.cellpos[i];
+  int dx = rebin(p.x, two_h), dy = rebin(p.y, two_h), dz = rebin(p.z, two_h);
+  p.x = p.x - float(dx) * two_h; p.y = p.y - float(dy) * two_h; p.z = p.z - float(dz) * two_h;
+  ci.x += dx; ci.y += dy; ci.z += dz;
+  if (P->periodbound) wrap_period(P, ci, p);  // predictor_corrector.cu:158
+  Sym D = ldsym(B.acestn_a, B.acestn_b, i);
+  Sym stn0 = ldsym(B.stn_ini_a, B.stn_ini_b, i);
+  stsym(A.stn_a, A.stn_b, i, sadd(stn0, smul(D, dt)));
+  Sym sig = ldsym(B.str_ini_a, B.str_ini_b, i);
+  float Mcrt = 0.f;
+  bool yld = false;
+  if (cd.z == LQ_HYPO_CS) {
+    if (!P->sur_on) hypo_cs(P, cd.w, dt, rho, ld3(B.spin[i]), D, sig, Mcrt);
+  } else {
+    yld = constitutive_other(P, cd, dt, ld3(B.spin[i]), D, sig);
+  }
+  stsym(A.str_a, A.str_b, i, sig);
+  float4 id = A.idm[i];
+  if (!yld) id.x = 100.f;  // predictor_corrector.cu:175-180
+  id.w = Mcrt;
+  if (!in_case_grid(P, A.case_id, i, ci)) { ci.w = P->ncell_delete_m; id.x = 999.f; }
+  A.idm[i] = id;
+  A.cellpos[i] = ci;
+  A.pos[i] = make_float4(p.x, p.y, p.z, pini.w);
+  A.vel[i] = v;
+  float4 po = A.pos_ori[i];
+  if (P->time <= P->dt) po = make_float4(0.f, 0.f, 0.f, 0.f);
+  po = make_float4(fmaf(v.x, dt, po.x), fmaf(v.y, dt, po.y), fmaf(v.z, dt, po.z), po.w);
+  A.pos_ori[i] = po;
+  B.accad[i] = po;
 }
 
 }  // namespace lq

@@ -144,6 +144,8 @@ struct Buffers {
   // variant rec is indexed by aux position, not by slot
   float4* aux;
   int* sub_start;
+  float4* fl_pos;  // fused steps: position and cell of first-layer particles before their update
+  int4* fl_cell;
   int2* colcnt;   // per 2h column: wall and soil counts
   int4* colinfo;  // per 2h column: first wall slot, first soil slot, counts
   int* hist;         // radix histogram [256][nblocks]

@@ -25,8 +25,12 @@ void launch_interaction_plist(hipStream_t s, DevParams* P, int npm_max, Arrays A
 void launch_pair_prep(hipStream_t s, const DevParams* P, int n, Arrays A, const Buffers& B);
 void launch_interaction(hipStream_t s, DevParams* P, int npm_max, Arrays A, const Buffers& B, const int* cb,
                         const int* ce, int variant, int xcd_remap, bool periodic);
+// fused: the interaction already moved the non-deferred particles; pass B
+// reads first-layer positions and cells from B.fl_pos / B.fl_cell
 void launch_contact(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const Buffers& B, const int* cb,
-                    const int* ce);
+                    const int* ce, bool fused = false);
+void launch_predictor_finish(hipStream_t s, DevParams* P, Arrays A, const Buffers& B);
+void launch_corrector_finish(hipStream_t s, DevParams* P, int npb, int n_motion, Arrays A, const Buffers& B);
 void launch_predictor(hipStream_t s, DevParams* P, int np, Arrays A, const Buffers& B, int track_disp);
 void launch_corrector(hipStream_t s, DevParams* P, int npb, int npm_max, int n_motion, Arrays A, const Buffers& B);
 void launch_wall_profile(hipStream_t s, const DevParams* P, int np, int npb, Arrays A, float4 geom, int nbins, int* last,
@@ -39,8 +43,11 @@ void launch_neighbor_list(hipStream_t s, const DevParams* P, int npm_max, Arrays
 // COLUMNS interaction variant (lq_columns.hip)
 void launch_col_index(hipStream_t s, const DevParams* P, int cnx, int cny, Arrays A, const Buffers& B);
 // wide != 0: the 128-VGPR build (4 waves per SIMD) instead of the 96-VGPR one
+// fuse: 0 interaction only; 1 with the predictor (cuPredictorMaterial) and
+// 2 with the corrector (cuCorrectorMaterial) of every soil particle whose
+// contact force needs no pass B
 void launch_interaction_col(hipStream_t s, DevParams* P, int np, int npm_max, Arrays A, const Buffers& B,
-                            int xcd_remap, int wide);
+                            int xcd_remap, int wide, int fuse = 0);
 void launch_neighbor_list_col(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const Buffers& B, int max_nb,
                               int* counts, int* ids);
 int col_max_cnz();

@@ -629,9 +629,14 @@ __global__ void __launch_bounds__(NL_WAVE * NL_WAVES, 5) k_interaction_nl(DevPar
 // the force of the last qualifying neighbour in traversal order scaled by
 // that minimum: the wave reduces exactly those (a min and an arg-max of the
 // traversal position) and evaluates the same expression.
+// npos / ncell: where the neighbours' positions and cells are read, A.pos
+// and A.cellpos, or the copies a fused interaction keeps of its first-layer
+// particles before it moves them.
 __global__ void __launch_bounds__(256) k_contact_redistribute(const DevParams* __restrict__ P, Arrays A, Buffers B,
                                                               const int* __restrict__ cbeg,
-                                                              const int* __restrict__ cend) {
+                                                              const int* __restrict__ cend,
+                                                              const float4* __restrict__ npos,
+                                                              const int4* __restrict__ ncell) {
   const int nd = P->n_deferred;
   const float two_h = P->two_h, band = P->band, half_dr = P->half_dr;
   const int ncm = P->ncell_material;
@@ -658,8 +663,8 @@ __global__ void __launch_bounds__(256) k_contact_redistribute(const DevParams* _
       for (int j = b + lane; j < e; j += G) {
         const float4 cvj = A.cvar[j];
         if (!(cvj.y <= band)) continue;
-        const int4 cjj = A.cellpos[j];
-        const float4 pj = A.pos[j];
+        const int4 cjj = ncell[j];
+        const float4 pj = npos[j];
         const float3 pja = make_float3(pj.x + float(cjj.x - cs.ox) * two_h, pj.y + float(cjj.y - cs.oy) * two_h,
                                        pj.z + float(cjj.z - cs.oz) * two_h);
         const float3 dc = f3sub(pa, pja);
@@ -711,55 +716,6 @@ __global__ void k_contact_commit(const DevParams* __restrict__ P, Arrays A, Buff
   }
 }
 
-// --------------------------------------------------------- predictor ----
-// LoquatSph::Predictor + cuPredictorMaterial, predictor_corrector.cu:52-103:
-// the *_ini snapshot and the half-step update in one pass.  Returns the
-// squared displacement of slot i (soil that moves; 0 otherwise).
-__device__ __forceinline__ float predictor_slot(const DevParams* __restrict__ P, Arrays A, const Buffers& B, int i) {
-  const int npm = P->npm, npb = P->npb;
-  float4 p = A.pos[i], v = A.vel[i];
-  B.pos_ini[i] = p;
-  B.vel_ini[i] = v;
-  if (npm == 0 || i >= npb + npm) return 0.f;
-  float4 sa = A.str_a[i], na = A.stn_a[i];
-  float2 sb = A.str_b[i], nb = A.stn_b[i];
-  B.str_ini_a[i] = sa; B.str_ini_b[i] = sb;
-  B.stn_ini_a[i] = na; B.stn_ini_b[i] = nb;
-  if (i < npb) return 0.f;
-  int4 cd = A.codes[i];
-  if (cd.y != LQ_NONFIX) return 0.f;
-  const float hdt = 0.5f * P->dt;
-  float4 a = B.accad[i];
-  v = make_float4(fmaf(a.x, hdt, v.x), fmaf(a.y, hdt, v.y), fmaf(a.z, hdt, v.z), fmaf(a.w, hdt, v.w));
-  const float4 p0 = p;
-  p = make_float4(fmaf(v.x, hdt, p.x), fmaf(v.y, hdt, p.y), fmaf(v.z, hdt, p.z), p.w);
-  const float* mp = P->model_par[cd.w];
-  float rho = fmaxf(v.w, mp[13]);
-  rho = fminf(rho, mp[14]);
-  v.w = rho;
-  Sym D = ldsym(B.acestn_a, B.acestn_b, i);
-  Sym stn = {na.x, na.y, na.z, na.w, nb.x, nb.y};
-  stsym(A.stn_a, A.stn_b, i, sadd(stn, smul(D, hdt)));
-  Sym sig = {sa.x, sa.y, sa.z, sa.w, sb.x, sb.y};
-  float Mcrt = 0.f;
-  bool yld = false;
-  if (cd.z == LQ_HYPO_CS) {
-    if (!P->sur_on) hypo_cs(P, cd.w, hdt, rho, ld3(B.spin[i]), D, sig, Mcrt);
-  } else {
-    yld = constitutive_other(P, cd, hdt, ld3(B.spin[i]), D, sig);
-  }
-  stsym(A.str_a, A.str_b, i, sig);
-  A.pos[i] = p;
-  A.vel[i] = v;
-  if (!yld) {  // a Drucker-Prager yield keeps the old value (predictor_corrector.cu:78-83)
-    float4 id = A.idm[i];
-    id.x = 100.f;
-    A.idm[i] = id;
-  }
-  const float dx = p.x - p0.x, dy = p.y - p0.y, dz = p.z - p0.z;
-  return dx * dx + dy * dy + dz * dz;
-}
-
 // With track set, the wave maximum of the squared displacement goes to
 // max_disp2_bits for the pair lists (non-negative floats order like their
 // bit patterns).  The running maximum only grows, so a wave whose maximum a
@@ -777,8 +733,6 @@ __global__ void __launch_bounds__(256) k_predictor(DevParams* __restrict__ P, Ar
 }
 
 // --------------------------------------------------------- corrector ----
-__device__ __forceinline__ int rebin(float p, float two_h) { return int(p >= two_h) - int(p < 0.0f); }
-
 // cuCorrectorBound, predictor_corrector.cu:109-137.
 __global__ void __launch_bounds__(256) k_corrector_bound(DevParams* __restrict__ P, Arrays A, Buffers B) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -818,51 +772,25 @@ __global__ void __launch_bounds__(256) k_corrector_bound(DevParams* __restrict__
   if (!in_case_grid(P, A.case_id, i, ci)) set_err(P, DERR_DOMAIN);
 }
 
-// cuCorrectorMaterial, predictor_corrector.cu:139-199.
 __global__ void __launch_bounds__(256) k_corrector_material(const DevParams* __restrict__ P, Arrays A, Buffers B) {
-  int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= P->npm) return;
-  const int i = P->npb + idx;
-  int4 cd = A.codes[i];
-  if (cd.y != LQ_NONFIX) return;
-  const float dt = P->dt, hdt = 0.5f * dt, two_h = P->two_h;
-  float4 vini = B.vel_ini[i], a = B.accad[i], pini = B.pos_ini[i];
-  float4 v = make_float4(fmaf(a.x, dt, vini.x), fmaf(a.y, dt, vini.y), fmaf(a.z, dt, vini.z), fmaf(a.w, dt, vini.w));
-  float3 p = make_float3(fmaf(v.x + vini.x, hdt, pini.x), fmaf(v.y + vini.y, hdt, pini.y), fmaf(v.z + vini.z, hdt, pini.z));
-  const float* mp = P->model_par[cd.w];
-  float rho = fmaxf(v.w, mp[13]);
-  rho = fminf(rho, mp[14]);
-  v.w = rho;
-  int4 ci = A.cellpos[i];
-  int dx = rebin(p.x, two_h), dy = rebin(p.y, two_h), dz = rebin(p.z, two_h);
-  p.x = p.x - float(dx) * two_h; p.y = p.y - float(dy) * two_h; p.z = p.z - float(dz) * two_h;
-  ci.x += dx; ci.y += dy; ci.z += dz;
-  if (P->periodbound) wrap_period(P, ci, p);  // predictor_corrector.cu:158
-  Sym D = ldsym(B.acestn_a, B.acestn_b, i);
-  Sym stn0 = ldsym(B.stn_ini_a, B.stn_ini_b, i);
-  stsym(A.stn_a, A.stn_b, i, sadd(stn0, smul(D, dt)));
-  Sym sig = ldsym(B.str_ini_a, B.str_ini_b, i);
-  float Mcrt = 0.f;
-  bool yld = false;
-  if (cd.z == LQ_HYPO_CS) {
-    if (!P->sur_on) hypo_cs(P, cd.w, dt, rho, ld3(B.spin[i]), D, sig, Mcrt);
-  } else {
-    yld = constitutive_other(P, cd, dt, ld3(B.spin[i]), D, sig);
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < P->npm) corrector_slot(P, A, B, P->npb + idx);
+}
+
+// The slots a fused interaction leaves to the predictor (the snapshot of
+// walls and of deleted slots, and the deferred soil, whose acceleration pass
+// B completes), then those it leaves to the corrector (the deferred soil).
+__global__ void __launch_bounds__(256) k_predictor_finish(DevParams* __restrict__ P, Arrays A, Buffers B) {
+  const int npb = P->npb, tail = P->npb + P->npm, nd = P->n_deferred;
+  const int n = npb + nd + (P->np - tail);
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+    const int i = t < npb ? t : t < npb + nd ? B.deferred[t - npb] : tail + (t - npb - nd);
+    predictor_slot(P, A, B, i);
   }
-  stsym(A.str_a, A.str_b, i, sig);
-  float4 id = A.idm[i];
-  if (!yld) id.x = 100.f;  // predictor_corrector.cu:175-180
-  id.w = Mcrt;
-  if (!in_case_grid(P, A.case_id, i, ci)) { ci.w = P->ncell_delete_m; id.x = 999.f; }
-  A.idm[i] = id;
-  A.cellpos[i] = ci;
-  A.pos[i] = make_float4(p.x, p.y, p.z, pini.w);
-  A.vel[i] = v;
-  float4 po = A.pos_ori[i];
-  if (P->time <= P->dt) po = make_float4(0.f, 0.f, 0.f, 0.f);
-  po = make_float4(fmaf(v.x, dt, po.x), fmaf(v.y, dt, po.y), fmaf(v.z, dt, po.z), po.w);
-  A.pos_ori[i] = po;
-  B.accad[i] = po;
+}
+__global__ void __launch_bounds__(256) k_corrector_finish(DevParams* __restrict__ P, Arrays A, Buffers B) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < P->n_deferred; k += gridDim.x * blockDim.x)
+    corrector_slot(P, A, B, B.deferred[k]);
 }
 
 // -------------------------------------------------------- wall force ----
@@ -1043,16 +971,25 @@ void launch_interaction(hipStream_t s, DevParams* P, int npm_max, Arrays A, cons
                        xcd_remap, 1);
 }
 void launch_contact(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const Buffers& B, const int* cb,
-                    const int* ce) {
+                    const int* ce, bool fused) {
   if (npm_max <= 0) return;
   int g = std::min(blocks(npm_max, 256), 1024);
   // 16 lanes per deferred particle, up to 16 * 4096 at a time
   hipLaunchKernelGGL(k_contact_redistribute, dim3(std::min(blocks(npm_max, 16), 4096)), dim3(256), 0, s, P, A, B, cb,
-                     ce);
+                     ce, fused ? (const float4*)B.fl_pos : (const float4*)A.pos,
+                     fused ? (const int4*)B.fl_cell : (const int4*)A.cellpos);
   hipLaunchKernelGGL(k_contact_commit, dim3(g), dim3(256), 0, s, P, A, B);
 }
 void launch_predictor(hipStream_t s, DevParams* P, int np, Arrays A, const Buffers& B, int track_disp) {
   if (np > 0) hipLaunchKernelGGL(k_predictor, dim3(blocks(np, 256)), dim3(256), 0, s, P, A, B, track_disp);
+}
+void launch_predictor_finish(hipStream_t s, DevParams* P, Arrays A, const Buffers& B) {
+  hipLaunchKernelGGL(k_predictor_finish, dim3(1024), dim3(256), 0, s, P, A, B);
+}
+void launch_corrector_finish(hipStream_t s, DevParams* P, int npb, int n_motion, Arrays A, const Buffers& B) {
+  if (n_motion > 0 && npb > 0)
+    hipLaunchKernelGGL(k_corrector_bound, dim3(blocks(npb, 256)), dim3(256), 0, s, P, A, B);
+  hipLaunchKernelGGL(k_corrector_finish, dim3(256), dim3(256), 0, s, P, A, B);
 }
 void launch_corrector(hipStream_t s, DevParams* P, int npb, int npm_max, int n_motion, Arrays A, const Buffers& B) {
   if (n_motion > 0 && npb > 0)

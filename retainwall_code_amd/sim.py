@@ -216,6 +216,10 @@ class Context:
     def set_interaction(self, variant, xcd_remap=1):
         self._chk(self.lib.lq_set_interaction(self.h, variant, xcd_remap), "lq_set_interaction")
 
+    def set_fusion(self, on=True):
+        """Fused COLUMNS steps (lq_set_fusion)."""
+        self._chk(self.lib.lq_set_fusion(self.h, 1 if on else 0), "lq_set_fusion")
+
     def set_pair_list(self, cap=160, skin=0.01):
         self._chk(self.lib.lq_set_pair_list(self.h, cap, skin), "lq_set_pair_list")
 

@@ -6,7 +6,9 @@
   (the summation order differs from the reference's, so against both the
   FP64 tolerances apply), small case and c2;
 * whole steps against the FP64 oracle's steps, and graph replay equal to
-  eager launches.
+  eager launches;
+* steps with the predictor and corrector fused into the interaction
+  (lq_set_fusion) bit-identical to the unfused order.
 """
 import numpy as np
 import pytest
@@ -124,5 +126,53 @@ def test_columns_graph_equals_eager():
     b.sync()
     for k in ("pos", "vel", "str", "cvar", "cmag"):
         np.testing.assert_array_equal(bits(a.get(k)), bits(b.get(k)), err_msg=k)
+    a.close()
+    b.close()
+
+
+FUSED_FIELDS = ["pos", "vel", "str", "stn", "idm", "cellpos", "orig_id", "cvar", "cmag", "pos_ori", "accad",
+                "acestn", "spin", "pos_ini", "vel_ini", "str_ini", "stn_ini"]
+
+
+@pytest.mark.parametrize("name,graph", [("small", False), ("small", True), ("tiny", True)])
+def test_fused_steps_bit_identical(name, graph):
+    """COLUMNS steps with the predictor / corrector fused into the interaction
+    equal the unfused order bit for bit (deferred contact particles finish
+    after pass B, which reads the first-layer positions saved before the
+    update)."""
+    case = cases.config(name)
+    say(f"[fused] case={name} graph={graph}")
+    a = sim.open_case(case, variant=COLS)
+    b = sim.open_case(case, variant=COLS)
+    a.set_fusion(True)
+    b.set_fusion(False)
+    for _ in range(3):
+        a.step(4, graph=graph)
+        b.step(4, graph=graph)
+        a.sync()
+        b.sync()
+        for k in FUSED_FIELDS:
+            np.testing.assert_array_equal(bits(a.get(k)), bits(b.get(k)), err_msg=k)
+        assert a.counts() == b.counts()
+    d = int(np.count_nonzero(a.get("cvar")[case.npb:, 1] > float(a.get_params().dr)))
+    say(f"  12 steps bit-identical ({d} soil particles beyond dr of a wall)")
+    a.close()
+    b.close()
+
+
+@pytest.mark.timeout(300)
+def test_fused_c2_bit_identical():
+    c2 = cases.config("c2")
+    a = sim.open_case(c2, variant=COLS)
+    b = sim.open_case(c2, variant=COLS)
+    a.set_fusion(True)
+    b.set_fusion(False)
+    a.step(6, graph=True)
+    b.step(6, graph=True)
+    a.sync()
+    b.sync()
+    for k in FUSED_FIELDS:
+        np.testing.assert_array_equal(bits(a.get(k)), bits(b.get(k)), err_msg=k)
+    say("  c2: 6 fused graph steps bit-identical to unfused")
     a.close()
     b.close()
