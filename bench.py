@@ -23,14 +23,23 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
 FP32_PEAK_TFLOPS = 157.3     # MI355X vector FP32
-# Compulsory bytes per interaction call: 176 B per soil particle (pos, vel,
-# rho, sigma/rho^2 and m/rho, flags, contact state in; acceleration, drho/dt,
-# D, spin, contact force/slip/K out), 64 B per boundary particle (pos, vel,
-# sigma/rho^2 and m/rho), 8 B per cell (begin/end).
-B_SOIL, B_BND, B_CELL = 176, 64, 8
+# Compulsory bytes per interaction call, each field once: 168 B per soil
+# particle (in: position 12, velocity 12, density 4, mass 4, stress 24,
+# flags 4, contact state 36; out: acceleration 12, drho/dt 4, D 24, spin 12,
+# contact force 12, slip 4, K 4), 56 B per wall particle (position,
+# velocity, density, stress, flags) and 8 B per cell of the linear grid
+# (begin/end), whatever key function the context uses.
+B_SOIL, B_BND, B_CELL = 168, 56, 8
 # Flops of the interaction kernel counted from its source (FMA = 2):
 # 15 per candidate distance test, 110 per neighbour pair evaluation.
 FLOP_CAND, FLOP_PAIR = 15, 110
+
+
+def _morton(x, y, z):
+    key = np.zeros(len(x), np.int64)
+    for b in range(10):
+        key |= ((x >> b) & 1) << (3 * b + 2) | ((y >> b) & 1) << (3 * b + 1) | ((z >> b) & 1) << (3 * b)
+    return key
 
 
 def candidate_counts(cellpos, beg, end, p, soil):
@@ -45,8 +54,51 @@ def candidate_counts(cellpos, beg, end, p, soil):
             for dx in (-1, 0, 1):
                 x, y, z = c[:, 0] + dx, c[:, 1] + dy, c[:, 2] + dz
                 ok = (x >= 0) & (y >= 0) & (z >= 0) & (x < nx) & (y < ny) & (z < nz)
-                k = np.where(ok, x * ny * nz + y * nz + z, 0)
+                if p.key_mode == 1:
+                    k = np.where(ok, _morton(np.maximum(x, 0), np.maximum(y, 0), np.maximum(z, 0)), 0)
+                else:
+                    k = np.where(ok, x * ny * nz + y * nz + z, 0)
                 tot += np.where(ok, cnt[k] + cnt[k + ncm], 0)
+    return tot
+
+
+def col_candidate_counts(pos, cellpos, p, soil, np_live):
+    """Per soil particle: candidates the COLUMNS variant tests (its sub-column
+    windows, with the kernel's margins), from the same float32 state."""
+    h, i_h = np.float32(p.h), np.float32(p.i_h)
+    nx, ny, nz = p.cell_num
+    rel = pos[:np_live, :3]
+    cp = cellpos[:np_live, :3].astype(np.int64)
+    sub = (rel >= h).astype(np.int64)
+    xs = 2 * cp + sub
+    ncol2 = 2 * nz
+    bucket = ((xs[:, 0] * (2 * ny) + xs[:, 1]) * ncol2 + xs[:, 2])
+    nb = (2 * nx) * (2 * ny) * ncol2
+    pre = np.concatenate([[0], np.cumsum(np.bincount(bucket, minlength=nb))])
+    f = (rel - np.where(sub == 1, h, np.float32(0))).astype(np.float64)[soil]
+    X = xs[soil]
+    r2cut = (2.0 * float(p.h)) ** 2 * 1.001
+    margin = 0.01 * float(p.h)
+    hh = float(p.h)
+
+    def gap(d, fa):
+        g = np.where(d > 0, d * hh - fa, np.where(d < 0, fa - (d + 1) * hh, 0.0))
+        return np.maximum(g - margin, 0.0)
+
+    tot = np.zeros(len(X), np.int64)
+    for dx in range(-2, 3):
+        for dy in range(-2, 3):
+            cx, cy = X[:, 0] + dx, X[:, 1] + dy
+            ok = (cx >= 0) & (cy >= 0) & (cx < 2 * nx) & (cy < 2 * ny)
+            rem = r2cut - gap(dx, f[:, 0]) ** 2 - gap(dy, f[:, 1]) ** 2
+            ok &= rem > 0
+            dz = np.sqrt(np.maximum(rem, 0)) + margin
+            zlo = np.maximum(X[:, 2] + np.maximum(-2, np.floor((f[:, 2] - dz) / hh)), 0).astype(np.int64)
+            zhi = np.minimum(X[:, 2] + np.minimum(2, np.floor((f[:, 2] + dz) / hh)), ncol2 - 1).astype(np.int64)
+            ok &= zlo <= zhi
+            base = (np.where(ok, cx, 0) * (2 * ny) + np.where(ok, cy, 0)) * ncol2
+            n = pre[np.where(ok, base + zhi + 1, 0)] - pre[np.where(ok, base + zlo, 0)]
+            tot += np.where(ok, n, 0)
     return tot
 
 
@@ -257,6 +309,7 @@ def main():
                     choices=["auto", "tile", "reference", "neighbor_list", "pair_list", "columns"],
                     help="interaction kernel variant (all bit-identical); auto times each and keeps the fastest")
     ap.add_argument("--remap", type=int, default=0, help="XCD block renumbering for reference/neighbor_list")
+    ap.add_argument("--key", default="linear", choices=["linear", "morton"], help="cell-key function")
     ap.add_argument("--tile", default="", metavar="SHAPE,TZ", help="fix the tile shape/depth instead of autotuning")
     ap.add_argument("--batch", type=int, default=0, metavar="N",
                     help="batched workload: N independent wall cases in one context (256 = the c4 workload)")
@@ -300,7 +353,8 @@ def main():
              _abi.INTERACTION_COLUMNS: "columns"}
     ids = {v: k for k, v in names.items()}
     variant = ids.get(args.variant, _abi.INTERACTION_REFERENCE_ORDER)
-    ctx = sim.open_case(case, variant=variant)
+    key_mode = _abi.KEY_MORTON if args.key == "morton" else _abi.KEY_LINEAR
+    ctx = sim.open_case(case, key_mode=key_mode, variant=variant)
     p = ctx.params
     ctx.step(args.warmup, graph=args.graph)
     ctx.sync()
@@ -310,7 +364,22 @@ def main():
         tile_shape, tile_tz = (int(v) for v in args.tile.split(","))
         ctx.set_tile(tile_shape, tile_tz)
     elif args.variant == "auto":
-        variant, tile_shape, tile_tz, remap, _ = ctx.autotune(reps=3)  # untimed: pick the fastest kernel
+        # untimed: the fastest of the bit-identical variants, then the COLUMNS
+        # variant (same neighbour sets, other summation order) if it is faster
+        variant, tile_shape, tile_tz, remap, ms_exact = ctx.autotune(reps=3)
+        if key_mode == _abi.KEY_LINEAR and not case.params.get("periodbound"):
+            best = (variant, tile_shape, tile_tz, remap)
+            ctx.set_interaction(_abi.INTERACTION_COLUMNS, 1)
+            v2, s2, t2, r2, ms_cols = ctx.autotune(reps=3)
+            print(f"[bench] autotune: {names[variant]} {ms_exact:.3f} ms, columns {ms_cols:.3f} ms per step's "
+                  "two calls", file=sys.stderr, flush=True)
+            if ms_cols < ms_exact:
+                variant, tile_shape, tile_tz, remap = v2, s2, t2, r2
+            else:
+                variant, tile_shape, tile_tz, remap = best
+                if variant == _abi.INTERACTION_TILE:
+                    ctx.set_tile(tile_shape, tile_tz)
+                ctx.set_interaction(variant, remap)
     elif args.variant == "columns":  # untimed: block order and register build of the COLUMNS kernel
         ctx.set_interaction(variant, remap)
         variant, tile_shape, tile_tz, remap, _ = ctx.autotune(reps=3)
@@ -324,9 +393,12 @@ def main():
     cellpos = ctx.get("cellpos")
     beg, end = ctx.get("cell_beg"), ctx.get("cell_end")
     cand = candidate_counts(cellpos, beg, end, p, soil)
+    if variant == _abi.INTERACTION_COLUMNS:
+        cand = col_candidate_counts(ctx.get("pos"), cellpos, p, soil, case.np)
     nbc, _ = ctx.neighbor_lists(1)
     flops_call = FLOP_CAND * float(cand.sum()) + FLOP_PAIR * float(nbc.sum())
-    bytes_call = B_SOIL * case.npm + B_BND * case.npb + B_CELL * p.ncell
+    ncell_linear = 2 * int(p.cell_num[0]) * int(p.cell_num[1]) * int(p.cell_num[2]) + 1
+    bytes_call = B_SOIL * case.npm + B_BND * case.npb + B_CELL * ncell_linear
 
     elapsed, kt = timed_steps(ctx, args.steps, args.graph)
     F = ctx.wall_force(case.probe_box)
@@ -343,7 +415,8 @@ def main():
             rec = json.load(open(pmc))
             if rec.get("kernel") == {"tile": "k_interaction_tile", "reference": "k_interaction_ref",
                                      "neighbor_list": "k_interaction_nl",
-                                     "pair_list": "k_interaction_plist", "columns": "k_interaction_col"}[vname]:
+                                     "pair_list": "k_interaction_plist", "columns": "k_interaction_col"}[vname] \
+                    and rec.get("key_mode", "linear") == args.key:
                 traffic = rec.get("hbm_bytes_per_launch")  # rocprofv3 FETCH/WRITE_SIZE, copy-calibrated
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -370,7 +443,7 @@ def main():
                    "dt": float(p.dt) if p.dt else case.params["dt"], "graph": bool(args.graph),
                    "interaction_variant": vname, "tile_shape": tile_shape, "tile_tz": tile_tz,
                    "xcd_remap": remap if variant != _abi.INTERACTION_TILE else None,
-                   "key_mode": "linear"},
+                   "key_mode": args.key},
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kt.items() if v[1] > 0} if kt else None,

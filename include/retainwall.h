@@ -44,7 +44,15 @@ enum lq_shape { LQ_CUBOID = 0, LQ_CIRCLE = 1 };      /* types.h:49 */
 
 /* Cell-key function.  LINEAR reproduces the reference hash
  * key = cx*ny*nz + cy*nz + cz (+ncell_material for soil), loquat_grid.cu:88-90,
- * bit for bit.  MORTON interleaves the cell coordinates instead (extension). */
+ * bit for bit.  MORTON interleaves the bits of the cell coordinates (x, y, z
+ * from the high end, z lowest) on the cube of 2^b cells per axis that holds
+ * the grid, so consecutive slots are compact blocks of space; lq_create then
+ * sets ncell_material = 2^(3b), ncell_delete_m = 2^(3b+1), ncell = that + 1
+ * (b <= 9).  MORTON needs a single-case context and the REFERENCE_ORDER,
+ * NEIGHBOR_LIST or PAIR_LIST variant; neighbour sets, the per-particle cell
+ * order of every sum and hence the physics are those of LINEAR keys (the
+ * order of particles inside a cell can differ once particles change cells,
+ * which changes sums by rounding only). */
 enum lq_key_mode { LQ_KEY_LINEAR = 0, LQ_KEY_MORTON = 1 };
 
 /* Interaction kernel variant.  The first four produce bit-identical results

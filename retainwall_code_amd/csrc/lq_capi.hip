@@ -443,7 +443,13 @@ static int create_impl(const lq_params* cases, int n_cases, int device, lq_ctx**
       return LQ_ERR_ARG;
     if (p.interaction_variant < LQ_INTERACTION_REFERENCE_ORDER || p.interaction_variant > LQ_INTERACTION_COLUMNS)
       return LQ_ERR_ARG;
-    if (p.key_mode != LQ_KEY_LINEAR) return LQ_ERR_ARG;
+    // Morton keys: single-case contexts, with the variants that look cells
+    // up one at a time (TILE copies runs of consecutive linear keys, COLUMNS
+    // indexes 2h columns of consecutive linear keys)
+    if (p.key_mode != LQ_KEY_LINEAR &&
+        (p.key_mode != LQ_KEY_MORTON || n_cases > 1 || p.interaction_variant == LQ_INTERACTION_TILE ||
+         p.interaction_variant == LQ_INTERACTION_COLUMNS))
+      return LQ_ERR_ARG;
     if (p.n_motion < 0 || p.n_motion > LQ_RIGID_MOTION_MAX) return LQ_ERR_ARG;
     np_tot += p.np;
   }
@@ -494,7 +500,18 @@ static int create_impl(const lq_params* cases, int n_cases, int device, lq_ctx**
     e = profile_scratch(device, &sp);
     if (e != hipSuccess) { c->err = std::string("hipGetSymbolAddress: ") + hipGetErrorString(e); return bail(LQ_ERR_HIP); }
   }
+  int mbits = 0;
+  if (par.key_mode == LQ_KEY_MORTON) {  // a cube of 2^mbits cells per axis holds the grid
+    const int nmax = std::max(par.cell_num[0], std::max(par.cell_num[1], par.cell_num[2]));
+    while ((1 << mbits) < nmax) mbits++;
+    if (3 * mbits > 29) return bail(fail(c, LQ_ERR_ARG, "lq_create: grid too large for Morton keys"));
+    par.ncell_material = 1 << (3 * mbits);
+    par.ncell_delete_m = 2 * par.ncell_material;
+    par.ncell = par.ncell_delete_m + 1;
+    c->par = par;
+  }
   fill_static(par, c->hp);
+  c->hp.mbits = mbits;
   c->hp.n_cases = n_cases;
   c->hp.n_model = LQ_MODEL_NUMBER_MAX * n_cases;
   c->hp.n_motion = par.n_motion;
@@ -1708,6 +1725,8 @@ extern "C" int lq_set_interaction(lq_ctx* c, int variant, int xcd_remap) {
   if (variant == LQ_INTERACTION_COLUMNS && !c->B.aux)
     return fail(c, LQ_ERR_ARG, "lq_set_interaction: COLUMNS needs linear keys, no periodic planes and at most " +
                                    std::to_string(col_max_cnz()) + " cells along z");
+  if (c->par.key_mode == LQ_KEY_MORTON && variant == LQ_INTERACTION_TILE)
+    return fail(c, LQ_ERR_ARG, "lq_set_interaction: TILE needs linear keys");
   if (c->par.periodbound && (variant == LQ_INTERACTION_TILE || variant == LQ_INTERACTION_PAIR_LIST))
     return fail(c, LQ_ERR_ARG, "lq_set_interaction: periodic boundaries need REFERENCE_ORDER or NEIGHBOR_LIST");
   c->par.interaction_variant = variant;
@@ -1790,7 +1809,8 @@ extern "C" int lq_autotune(lq_ctx* c, int reps, int* variant_out, int* shape_out
     cands.push_back({LQ_INTERACTION_NEIGHBOR_LIST, c->tile_shape, c->tile_tz, remap});
     if (c->B.plist && !c->par.periodbound) cands.push_back({LQ_INTERACTION_PAIR_LIST, c->tile_shape, c->tile_tz, remap});
   }
-  for (int sh = 0; sh < (c->par.periodbound || cols ? 0 : interaction_tile_shapes()); sh++) {
+  const bool no_tile = c->par.periodbound || cols || c->par.key_mode == LQ_KEY_MORTON;
+  for (int sh = 0; sh < (no_tile ? 0 : interaction_tile_shapes()); sh++) {
     const int waves = interaction_tile_waves(sh);
     const int base = std::max(1, (waves * 64 * 9) / (10 * 16));  // ~90% of the lanes at ~16 soil per cell
     for (int tz = std::max(1, base - 2); tz <= std::min(interaction_tile_maxtz(sh), base + 2); tz++)

@@ -14,7 +14,7 @@ constexpr int RS_THREADS = 256;
 constexpr int RS_ITEMS = 16;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
 
-__device__ __forceinline__ int cell_key(const DevParams* P, int4 c) { return lin_key(P, c.x, c.y, c.z); }
+__device__ __forceinline__ int cell_key(const DevParams* P, int4 c) { return grid_key(P, c.x, c.y, c.z); }
 
 // cuGetParticleCellIndex, loquat_grid.cu:71-95.  Also flags whether any key
 // differs from the previous sorted key at the same slot.
@@ -137,18 +137,24 @@ __global__ void __launch_bounds__(SC_THREADS) k_scan_tiles(int* __restrict__ dat
 
 static int scan_tiles(int n) { return (n + SC_TILE - 1) / SC_TILE; }
 
-// wall_near[c] = 1 when any of the 27 cells around cell c (linear key) holds
-// a wall particle, else 0.
+// wall_near[c] = 1 when any of the 27 cells around cell c (its key) holds
+// a wall particle, else 0 (also for Morton keys outside the grid).
 __global__ void __launch_bounds__(256) k_wall_near(const DevParams* __restrict__ P, const int* __restrict__ cbeg,
                                                    int* __restrict__ wall_near) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= P->ncell_material) return;
-  const int cz = c % P->cnz, cy = (c / P->cnz) % P->cny, cx = c / (P->cny * P->cnz);
+  int cx, cy, cz;
+  if (P->key_mode == LQ_KEY_MORTON) {
+    cx = compact3(c >> 2); cy = compact3(c >> 1); cz = compact3(c);
+    if (!in_grid(P, cx, cy, cz)) { wall_near[c] = 0; return; }
+  } else {
+    cz = c % P->cnz; cy = (c / P->cnz) % P->cny; cx = c / (P->cny * P->cnz);
+  }
   int any = 0;
   for (int dz = -1; dz <= 1; dz++)
     for (int dy = -1; dy <= 1; dy++)
       for (int dx = -1; dx <= 1; dx++)
-        if (in_grid(P, cx + dx, cy + dy, cz + dz) && cbeg[lin_key(P, cx + dx, cy + dy, cz + dz)] >= 0) any = 1;
+        if (in_grid(P, cx + dx, cy + dy, cz + dz) && cbeg[grid_key(P, cx + dx, cy + dy, cz + dz)] >= 0) any = 1;
   wall_near[c] = any;
 }
 

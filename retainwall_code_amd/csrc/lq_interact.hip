@@ -80,7 +80,7 @@ __device__ __forceinline__ void tile_home(DevParams* __restrict__ P, const Array
             } else {
               const int jx = ci.x + cx, jy = ci.y + cy, jz = ci.z + cz;
               if (!in_grid(P, jx, jy, jz)) continue;
-              const int c = lin_key(P, jx, jy, jz) + (bsel ? ncm : 0);
+              const int c = grid_key(P, jx, jy, jz) + (bsel ? ncm : 0);
               const int bb = cbeg[c];
               if (bb < 0) continue;
               j = bb;

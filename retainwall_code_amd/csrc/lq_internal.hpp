@@ -62,7 +62,7 @@ struct DevParams {
   // (entries per soil slot) and column stride (soil slots)
   float r2list, disp2_lim;
   int plist_cap, plist_stride;
-  int n_cases, pad_c0, pad_c1, pad_c2;
+  int n_cases, mbits, pad_c1, pad_c2;  // mbits: Morton bits per axis (LQ_KEY_MORTON)
   float4 plane_l, plane_r;  // periodic planes a x + b y + c z + d = 0 (planeleft, planeright)
   float surcharge_force;  // downward force on each surcharge-block slot (-1 block: none)
   // COLUMNS interaction: squared search radius (the cut-off with a margin)
@@ -165,8 +165,31 @@ __device__ __forceinline__ float kG(float kgrad, float q) {  // functions.h:143
   float t = 2.0f - q;
   return kgrad * t * t * t;
 }
-__device__ __forceinline__ int lin_key(const DevParams* P, int cx, int cy, int cz) {
-  return cx * P->cny * P->cnz + cy * P->cnz + cz;  // loquat_grid.cu:88
+// Bits of v (0 <= v < 1024) moved to every third bit position.
+__device__ __forceinline__ int spread3(int v) {
+  unsigned x = unsigned(v) & 0x3ffu;
+  x = (x | (x << 16)) & 0x030000FFu;
+  x = (x | (x << 8)) & 0x0300F00Fu;
+  x = (x | (x << 4)) & 0x030C30C3u;
+  x = (x | (x << 2)) & 0x09249249u;
+  return int(x);
+}
+// The inverse of spread3 on every third bit of k.
+__device__ __forceinline__ int compact3(int k) {
+  unsigned x = unsigned(k) & 0x09249249u;
+  x = (x | (x >> 2)) & 0x030C30C3u;
+  x = (x | (x >> 4)) & 0x0300F00Fu;
+  x = (x | (x >> 8)) & 0x030000FFu;
+  x = (x | (x >> 16)) & 0x3ffu;
+  return int(x);
+}
+// Key of cell (cx, cy, cz) of the grid (the soil layer adds ncell_material).
+// LQ_KEY_LINEAR: cx * ny * nz + cy * nz + cz (loquat_grid.cu:88);
+// LQ_KEY_MORTON: the cell coordinates' bits interleaved (x y z from the top,
+// z lowest), on the cube of 2^mbits cells per axis.
+__device__ __forceinline__ int grid_key(const DevParams* P, int cx, int cy, int cz) {
+  if (P->key_mode == LQ_KEY_MORTON) return (spread3(cx) << 2) | (spread3(cy) << 1) | spread3(cz);
+  return cx * P->cny * P->cnz + cy * P->cnz + cz;
 }
 __device__ __forceinline__ bool in_grid(const DevParams* P, int cx, int cy, int cz) {
   return cx >= 0 && cy >= 0 && cz >= 0 && cx < P->cnx && cy < P->cny && cz < P->cnz;

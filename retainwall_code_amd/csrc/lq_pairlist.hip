@@ -46,7 +46,7 @@ __global__ void __launch_bounds__(256) k_plist_build(DevParams* __restrict__ P, 
       for (int cx = -1; cx <= 1; cx++) {
         const int jx = ci.x + cx, jy = ci.y + cy, jz = ci.z + cz;
         if (!in_grid(P, jx, jy, jz)) continue;
-        const int cb = lin_key(P, jx, jy, jz);
+        const int cb = grid_key(P, jx, jy, jz);
         const int off = (cx + 1) | ((cy + 1) << 2) | ((cz + 1) << 4);
         for (int pass = 0; pass < 2; pass++) {
           const int c = pass ? cb + ncm : cb;

@@ -147,7 +147,7 @@ __global__ void __launch_bounds__(64) k_adami(const DevParams* __restrict__ P, A
   auto scan = [&](int4 ci, float3 pi) {
     int bw = 0, ew = 0, bs = 0, es = 0;  // wall and soil ranges of cell k, fetched one cell ahead
     if (in_grid(P, ci.x - 1, ci.y - 1, ci.z - 1)) {
-      const int c = lin_key(P, ci.x - 1, ci.y - 1, ci.z - 1);
+      const int c = grid_key(P, ci.x - 1, ci.y - 1, ci.z - 1);
       if (do_normal) { bw = cbeg[c]; ew = cend[c]; }
       if (do_soil) { bs = cbeg[c + ncm]; es = cend[c + ncm]; }
     }
@@ -159,7 +159,7 @@ __global__ void __launch_bounds__(64) k_adami(const DevParams* __restrict__ P, A
         const int kn = k + 1;
         const int nx = ci.x + kn % 3 - 1, ny = ci.y + (kn / 3) % 3 - 1, nz = ci.z + kn / 9 - 1;
         if (in_grid(P, nx, ny, nz)) {
-          const int c = lin_key(P, nx, ny, nz);
+          const int c = grid_key(P, nx, ny, nz);
           if (do_normal) { bwn = cbeg[c]; ewn = cend[c]; }
           if (do_soil) { bsn = cbeg[c + ncm]; esn = cend[c + ncm]; }
         }
@@ -262,7 +262,7 @@ __global__ void __launch_bounds__(256) k_corner(const DevParams* __restrict__ P,
   float4 cv = A.cvar[i];
   cv.x = 0.f;
   cv.y = 999.f;
-  if (!wall_near[lin_key(P, ci.x, ci.y, ci.z)]) {
+  if (!wall_near[grid_key(P, ci.x, ci.y, ci.z)]) {
     A.cvar[i] = cv;
     return;
   }
@@ -273,7 +273,7 @@ __global__ void __launch_bounds__(256) k_corner(const DevParams* __restrict__ P,
   float dij = 999.f;
   int b = 0, e = 0;  // wall range of cell k, fetched one cell ahead
   if (in_grid(P, ci.x - 1, ci.y - 1, ci.z - 1)) {
-    const int c = lin_key(P, ci.x - 1, ci.y - 1, ci.z - 1);
+    const int c = grid_key(P, ci.x - 1, ci.y - 1, ci.z - 1);
     b = cbeg[c];
     e = cend[c];
   }
@@ -285,7 +285,7 @@ __global__ void __launch_bounds__(256) k_corner(const DevParams* __restrict__ P,
       const int kn = k + 1;
       const int nx = ci.x + kn % 3 - 1, ny = ci.y + (kn / 3) % 3 - 1, nz = ci.z + kn / 9 - 1;
       if (in_grid(P, nx, ny, nz)) {
-        const int c = lin_key(P, nx, ny, nz);
+        const int c = grid_key(P, nx, ny, nz);
         bn = cbeg[c];
         en = cend[c];
       }
@@ -347,7 +347,7 @@ __global__ void __launch_bounds__(256) k_shepard(const DevParams* __restrict__ P
         for (int cx = -1; cx <= 1; cx++) {
           int jx = ci.x + cx, jy = ci.y + cy, jz = ci.z + cz;
           if (!in_grid(P, jx, jy, jz)) continue;
-          int c = lin_key(P, jx, jy, jz) + P->ncell_material;
+          int c = grid_key(P, jx, jy, jz) + P->ncell_material;
           for (int j = cbeg[c], e = cend[c]; j < e; j++) {
             float3 d;
             const float r2 = pair_r2(pi, A.pos[j], cx, cy, cz, P->two_h, d);
@@ -433,7 +433,7 @@ __global__ void __launch_bounds__(256) k_interaction_ref(DevParams* __restrict__
         for (int cx = -1; cx <= 1; cx++) {
           int jx = ci.x + cx, jy = ci.y + cy, jz = ci.z + cz;
           if (!in_grid(P, jx, jy, jz)) continue;
-          int cb = lin_key(P, jx, jy, jz);
+          int cb = grid_key(P, jx, jy, jz);
           for (int pass = 0; pass < 2; pass++) {
             const bool bound = pass == 0;
             int c = bound ? cb : cb + P->ncell_material;
@@ -532,7 +532,7 @@ __device__ __forceinline__ void nl_scan(const DevParams* __restrict__ P, const P
     const int jx = ci.x + cx, jy = ci.y + cy, jz = ci.z + cz;
     int j = 0, e = 0;
     if (active && in_grid(P, jx, jy, jz)) {
-      const int c = lin_key(P, jx, jy, jz) + (pass ? ncm : 0);
+      const int c = grid_key(P, jx, jy, jz) + (pass ? ncm : 0);
       j = cbeg[c];
       e = cend[c];
     }
@@ -658,7 +658,7 @@ __global__ void __launch_bounds__(256) k_contact_redistribute(const DevParams* _
       const int cz = r / 9 - 1, cy = (r / 3) % 3 - 1, cx = r % 3 - 1;
       const int jx = ci.x + cx, jy = ci.y + cy, jz = ci.z + cz;
       if (!in_grid(P, jx, jy, jz)) continue;
-      const int c = lin_key(P, jx, jy, jz) + ncm;
+      const int c = grid_key(P, jx, jy, jz) + ncm;
       const int b = cbeg[c], e = cend[c];
       for (int j = b + lane; j < e; j += G) {
         const float4 cvj = A.cvar[j];
@@ -885,7 +885,7 @@ __global__ void k_neighbor_list(const DevParams* __restrict__ P, Arrays A, const
       for (int cx = -1; cx <= 1; cx++) {
         int jx = ci.x + cx, jy = ci.y + cy, jz = ci.z + cz;
         if (!in_grid(P, jx, jy, jz)) continue;
-        int cb = lin_key(P, jx, jy, jz);
+        int cb = grid_key(P, jx, jy, jz);
         for (int pass = 0; pass < 2; pass++) {
           int c = pass ? cb + P->ncell_material : cb;
           for (int j = cbeg[c], e = cend[c]; j < e; j++) {

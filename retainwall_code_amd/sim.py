@@ -29,6 +29,8 @@ class Context:
                 self.lib.lq_destroy(h)
             self.h = None
             raise LqError(f"lq_create failed ({st}): {msg}")
+        if params.key_mode != _abi.KEY_LINEAR:  # lq_create sizes the cell arrays for the key space
+            self.params = self.get_params()
 
     # -------------------------------------------------------------- util --
     def _chk(self, st, what):
