@@ -326,8 +326,16 @@ template <class T> static int neighbor_search(Par<T>& P, State<T>& S, int* ident
   return 0;
 }
 
-template <class T, class E> static void gather(std::vector<E>& tmp, E* a, int width, int n, const int32_t* idx) {
-  tmp.assign(a, a + size_t(width) * n);
+// a[i] = a[idx[i]] for the first n rows.  src_rows (>= n, default n) rows are
+// readable as sources: the stress and strain gathers cover npb + npm rows,
+// and after a deletion the particle that held the last slot can move into
+// one of them.  The reference's cuSortArrayTensor2d (loquat_grid.cu:232-251)
+// then reads a row of its tensor cache that the memcpy of npb + npm rows did
+// not refresh (stale contents of an earlier sort); here the particle keeps
+// its own row.
+template <class T, class E>
+static void gather(std::vector<E>& tmp, E* a, int width, int n, const int32_t* idx, int src_rows = -1) {
+  tmp.assign(a, a + size_t(width) * size_t(src_rows < n ? n : src_rows));
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < n; i++)
     for (int k = 0; k < width; k++) a[size_t(width) * i + k] = tmp[size_t(width) * idx[i] + k];
@@ -345,8 +353,8 @@ template <class T> static void sort_arrays_primary(const Par<T>& P, State<T>& S)
   gather<T>(t, S.vel, 4, np, S.part_idx);
   gather<T>(t, S.idm, 4, np, S.part_idx);
   gather<T>(t, S.normal, 4, P.npb, S.part_idx);
-  gather<T>(t, S.str, 6, nm, S.part_idx);
-  gather<T>(t, S.stn, 6, nm, S.part_idx);
+  gather<T>(t, S.str, 6, nm, S.part_idx, np);
+  gather<T>(t, S.stn, 6, nm, S.part_idx, np);
   gather<T>(t, S.cvar, 4, np, S.part_idx);
   gather<T>(t, S.cmag, 9, np, S.part_idx);
 }
@@ -366,10 +374,10 @@ template <class T> static void sort_arrays_all(const Par<T>& P, State<T>& S) {
   gather<T>(t, S.vel_ini, 4, np, S.part_idx);
   gather<T>(t, S.normal, 4, P.npb, S.part_idx);
   if (P.npm != 0) {
-    gather<T>(t, S.str, 6, nm, S.part_idx);
-    gather<T>(t, S.stn, 6, nm, S.part_idx);
-    gather<T>(t, S.str_ini, 6, nm, S.part_idx);
-    gather<T>(t, S.stn_ini, 6, nm, S.part_idx);
+    gather<T>(t, S.str, 6, nm, S.part_idx, np);
+    gather<T>(t, S.stn, 6, nm, S.part_idx, np);
+    gather<T>(t, S.str_ini, 6, nm, S.part_idx, np);
+    gather<T>(t, S.stn_ini, 6, nm, S.part_idx, np);
   }
   gather<T>(t, S.cvar, 4, np, S.part_idx);
   gather<T>(t, S.cmag, 9, np, S.part_idx);

@@ -818,7 +818,8 @@ __global__ void __launch_bounds__(256) k_wall_force(const DevParams* __restrict_
                                                     const float* __restrict__ boxes,
                                                     unsigned long long* __restrict__ acc) {
   const float two_h = P->two_h;
-  for (int i = P->npb + blockIdx.x * blockDim.x + threadIdx.x; i < P->npb + P->npm; i += gridDim.x * blockDim.x) {
+  // slots npb..np as utilities.cu:592, deleted particles' stale forces included
+  for (int i = P->npb + blockIdx.x * blockDim.x + threadIdx.x; i < P->np; i += gridDim.x * blockDim.x) {
     const int k = A.case_id[i];
     const float4 p = A.pos[i];
     const int3 c = case_cell(P, A.case_id, i, A.cellpos[i]);

@@ -416,7 +416,29 @@ def test_particle_deletion(oracle_mod):
     npb, npm, nd = ctx.counts()
     assert nd == 1 and npm == case.npm - 1
     assert o32.p.np_delete_m == 1 and o32.p.npm == npm
-    g = ctx.snapshot(["orig_id", "cellpos"])
+    g = ctx.snapshot(["orig_id", "cellpos", "pos", "vel", "str", "stn"])
     np.testing.assert_array_equal(g["orig_id"], o32["orig_id"])
     assert g["orig_id"][-1] == top and g["cellpos"][-1, 3] == p.ncell_delete_m
+    # the particle that held the last slot moved into the gathered rows with
+    # its own stress and strain rows
+    live = slice(case.npb, case.npb + npm)
+    for k in ("pos", "vel", "str", "stn"):
+        e = relerr(g[k][live], o32[k][live])
+        say(f"  {k} relerr vs FP32 oracle {e:.2e}")
+        assert e <= (TOL_F32_RATE if k == "stn" else TOL_F32_STATE), k
+    # the wall-force probe loops over slots npb..np (utilities.cu:592): a
+    # deleted particle's stale contact force counts when it lies in the box
+    box = (np.float32(1e3), np.float32(-1e3), np.float32(1e3))
+    f0 = ctx.wall_force(box)
+    cm = ctx.get("cmag")
+    k = int(np.argmax(np.abs(o32["cmag"][:, 0])))
+    say(f"  contact force |cmag.x| max: GPU {np.abs(cm[:, 0]).max():.4e} N, oracle {abs(o32['cmag'][k, 0]):.4e} N "
+        f"(slot {k}); wall force GPU {f0:.6e} kN, oracle {o32.wall_force(box):.6e} kN")
+    cm[-1, 0] += np.float32(123.5)
+    ctx.set("cmag", cm)
+    f1 = ctx.wall_force(box)
+    o = oracle_from(ctx, oracle_mod, np.float32, case)  # the same slots and values, summed serially
+    fo = o.wall_force(box)
+    assert abs(f1 - fo) <= 1e-9 * max(abs(fo), 1.0)
+    assert abs((f1 - f0) - 0.1235) <= 1e-6  # 123.5 N = 0.1235 kN from the deleted slot
     ctx.close()

@@ -117,6 +117,28 @@ def test_hash_sort_is_stable_and_ranges_consistent(oracle_mod):
     assert (beg[np.setdiff1d(np.arange(p.ncell), pc)] == -1).all()
 
 
+def test_sort_after_deletion_keeps_own_stress_rows(oracle_mod):
+    """After a deletion the stress and strain gathers cover npb + npm rows,
+    while the particle that held the last slot moves into one of them: it
+    takes its own row (the reference reads an unrefreshed row of its tensor
+    cache there, loquat_grid.cu:232-251)."""
+    o = soil_block(oracle_mod, n=(4, 4, 4))
+    N = o.p.np
+    o["str"][:, 0] = -1000.0 - np.arange(N)
+    o["stn"][:, 1] = np.arange(N) * 1e-3
+    o["cellpos"][0, 3] = o.p.ncell_delete_m
+    o.neighbor_search()
+    assert o.p.np_delete_m == 1 and o.p.npm == N - 1
+    idx = o["part_idx"].copy()
+    assert idx[-1] == 0 and (idx[:-1] == np.arange(1, N)).all()
+    s0, e0, ids = o["str"].copy(), o["stn"].copy(), o["orig_id"].copy()
+    o.sort_arrays_primary()
+    np.testing.assert_array_equal(o["orig_id"], ids[idx])
+    np.testing.assert_array_equal(o["str"][:N - 1], s0[idx[:N - 1]])
+    np.testing.assert_array_equal(o["stn"][:N - 1], e0[idx[:N - 1]])
+    np.testing.assert_array_equal(o["str"][N - 1], s0[N - 1])  # past npb + npm: not gathered
+
+
 def test_neighbor_lists_brute_force(oracle_mod):
     """Traversal of 27 cells x {boundary, soil} finds exactly the pairs with
     q < 2 under the FP32 predicate (brute force over all pairs)."""
