@@ -543,6 +543,7 @@ static int create_impl(const lq_params* cases, int n_cases, int device, lq_ctx**
   st |= dalloc(c, &B.s3, n); st |= dalloc(c, &B.fdef, n); st |= dalloc(c, &B.deferred, n);
   st |= dalloc(c, &B.str_tmp_a, n); st |= dalloc(c, &B.str_tmp_b, n);
   st |= dalloc(c, &B.rec, 4 * n);
+  B.rec_rows = int(n);
   if (cols_supported(par)) {
     st |= dalloc(c, &B.aux, n);
     const size_t ncol = size_t(par.cell_num[0]) * size_t(par.cell_num[1]);

@@ -89,9 +89,10 @@ __global__ void __launch_bounds__(1024) k_col_scan(const DevParams* __restrict__
 // bucket's wall count for soil, plus the particles of its own cell and kind
 // before it in slot order that share its sub-cell (these are the run of
 // equal cz just before it, as slots follow the keys).  The same pass writes
-// the particle's 64-byte pair record at its aux position: position with the
-// mass in w (negative for contact walls), velocity and density,
-// sigma/rho^2, m/rho and trace(sigma/rho^2) (k_pair_prep's quantities).
+// the particle's pair record at its aux position, as four float4 planes of
+// rec_rows entries: position with the mass in w (negative for contact
+// walls), velocity and density, sigma/rho^2, m/rho and trace(sigma/rho^2)
+// (k_pair_prep's quantities).
 constexpr int COL_MAX_CNZ = 256;
 constexpr int COL_MAX_BUCKETS = 8 * COL_MAX_CNZ + 1;
 constexpr int COL_LDS_PTS = 8192;  // particles of a column whose codes stay in LDS
@@ -167,11 +168,11 @@ __global__ void __launch_bounds__(COL_THREADS) k_col_build(const DevParams* __re
     const float rho = v.w;
     const float m = soil ? fabsf(p.w) : vol_def * rho;
     const Sym sr = sdiv(ldsym(A.str_a, A.str_b, j), rho * rho);
-    float4* rc = B.rec + 4 * size_t(k);
-    rc[0] = make_float4(p.x, p.y, p.z, p.w < 0.f ? -m : m);
-    rc[1] = v;
-    rc[2] = make_float4(sr.xx, sr.yy, sr.zz, sr.xy);
-    rc[3] = make_float4(sr.xz, sr.yz, m / rho, strace(sr));
+    const size_t rows = size_t(B.rec_rows);
+    B.rec[k] = make_float4(p.x, p.y, p.z, p.w < 0.f ? -m : m);
+    B.rec[rows + k] = v;
+    B.rec[2 * rows + k] = make_float4(sr.xx, sr.yy, sr.zz, sr.xy);
+    B.rec[3 * rows + k] = make_float4(sr.xz, sr.yz, m / rho, strace(sr));
   }
 }
 
@@ -292,7 +293,8 @@ __device__ __forceinline__ void pair_terms_fast(const PairConst& P, float i_kdr,
 // One lane per aux entry, so the 64 lanes of a wave are consecutive
 // particles of one sub-column: for every window they read neighbouring
 // stretches of the same target sub-column, and their pair records (stored in
-// aux order) lie close together.  Wall entries leave their lane idle.  Hits
+// aux order, one plane per float4, so that 8 lanes share a 128-byte line)
+// lie close together.  Wall entries leave their lane idle.  Hits
 // go to per-lane FIFO rings in LDS as (aux index << 6 | cell-offset code) and
 // are evaluated in ring order, as in the neighbour-list variant.
 constexpr int CL_CAP = 32;    // ring entries per lane (power of two)
@@ -301,8 +303,8 @@ constexpr int CL_CHUNK = 32;  // candidates per hit mask
 __device__ __forceinline__ void cl_eval(const PairConst& K, float i_kdr, IAcc& S, const Buffers& B, int e, float3 pi,
                                         float two_h, float i_h, float3 vi, float rhoi, Sym sri, float artpi,
                                         int shift) {
-  const float4* rc = B.rec + 4 * size_t(unsigned(e) >> 6);
-  const float4 np_ = rc[0], nv = rc[1], na = rc[2], nb = rc[3];
+  const size_t rows = size_t(B.rec_rows), k = unsigned(e) >> 6;
+  const float4 np_ = B.rec[k], nv = B.rec[rows + k], na = B.rec[2 * rows + k], nb = B.rec[3 * rows + k];
   float3 d;
   d.x = (pi.x - np_.x) - fmaf(float(e & 3), two_h, -two_h);
   d.y = (pi.y - np_.y) - fmaf(float((e >> 2) & 3), two_h, -two_h);
@@ -338,8 +340,8 @@ __global__ void __launch_bounds__(64, MINW) k_interaction_col(DevParams* __restr
   const int kh = k < nidx ? k : 0;
   const int i = __float_as_int(B.aux[kh].w) >> 2;
   const bool active = k < nidx && i >= P->npb;
-  const float4* rh = B.rec + 4 * size_t(kh);
-  const float4 ph = rh[0], vri = rh[1], sria = rh[2], srib = rh[3];
+  const size_t rows = size_t(B.rec_rows);
+  const float4 ph = B.rec[kh], vri = B.rec[rows + kh], sria = B.rec[2 * rows + kh], srib = B.rec[3 * rows + kh];
   const float3 pi = ld3(ph);
   const int4 ci = A.cellpos[i];
   const float3 vi = ld3(vri);

@@ -123,8 +123,12 @@ struct Buffers {
   float2* str_tmp_b;
   // Per-slot neighbour record written by k_pair_prep, 64 bytes so a pair's
   // four loads share one cache line: pos, vel, then sigma/rho^2 as
-  // (xx yy zz xy) and (xz yz, m/rho, trace).
+  // (xx yy zz xy) and (xz yz, m/rho, trace).  The COLUMNS variant stores
+  // the same four float4 per aux position as four planes of rec_rows
+  // entries (plane q at rec + q * rec_rows), so that lanes reading nearby
+  // aux positions share cache lines.
   float4* rec;
+  int rec_rows;
   float* sur_x;      // [npm][16] surrogate inputs (normalised), per soil slot
   float* sur_y;      // [members][npm][8] surrogate outputs (normalised)
   unsigned short* sur_w;  // packed bf16 weights per member (MFMA operand layout)
