@@ -205,12 +205,16 @@ __device__ __forceinline__ float col_gap(int d, float f, float h, float margin) 
 
 // The aux range [k0, k1) of sub-column (dx, dy) around the particle, and the
 // cell offsets ox, oy of that sub-column; empty when no particle in it can be
-// within 2h.
+// within 2h.  zb0 and zb1 are the sub-column's aux starts of sub-cell
+// heights 2 cz and 2 cz + 2 (cz the particle's own 2h cell level): inside
+// the window, candidates before zb0 lie one 2h cell lower, those from zb1 on
+// one higher (the window spans at most two sub-cells either side).
 __device__ __forceinline__ void col_range(const DevParams* __restrict__ P, const int* __restrict__ sub_start,
                                           const ColHome& g, int dx, int dy, float gx2, float gy2, bool active,
-                                          int& k0, int& k1, int& ox, int& oy) {
+                                          int& k0, int& k1, int& ox, int& oy, int& zb0, int& zb1) {
   k0 = k1 = 0;
   ox = oy = 0;
+  zb0 = zb1 = 0;
   const int X = g.xs + dx, Y = g.ys + dy;
   const int cnz = P->cnz;
   if (!active || X < 0 || Y < 0 || X >= 2 * P->cnx || Y >= 2 * P->cny) return;
@@ -227,6 +231,9 @@ __device__ __forceinline__ void col_range(const DevParams* __restrict__ P, const
   const size_t base = size_t(cxj * P->cny + cyj) * size_t(8 * cnz + 1) + size_t((2 * (X & 1) + (Y & 1)) * (2 * cnz));
   k0 = sub_start[base + zlo];
   k1 = sub_start[base + zhi + 1];
+  const int zc = g.zs & ~1;  // first sub-cell height of the particle's own 2h cell
+  zb0 = sub_start[base + zc];
+  zb1 = sub_start[base + zc + 2];
 }
 
 // z offset of an aux candidate from its stored cz & 3 and the particle's cz
@@ -359,27 +366,34 @@ __global__ void __launch_bounds__(64, MINW) k_interaction_col(DevParams* __restr
   for (int q = 0; q < 9; q++) S.gv[q] = 0.f;
   const ColHome g = col_home(P, ci, pi);
   int head = 0, cnt = 0;
-#pragma unroll 1
-  for (int a = 0; a < 25; a++) {
+  // window a + 1's range is fetched while window a is scanned
+  auto window = [&](int a, int& j, int& e, int& ox, int& oy, int& zb0, int& zb1) {
     const int dx = a / 5 - 2, dy = a % 5 - 2;
     const float gx = col_gap(dx, g.fx, h, margin), gy = col_gap(dy, g.fy, h, margin);
-    int j, e, ox, oy;
-    col_range(P, B.sub_start, g, dx, dy, gx * gx, gy * gy, active, j, e, ox, oy);
+    col_range(P, B.sub_start, g, dx, dy, gx * gx, gy * gy, active && a < 25, j, e, ox, oy, zb0, zb1);
+  };
+  int nj, ne, nox, noy, nzb0, nzb1;
+  window(0, nj, ne, nox, noy, nzb0, nzb1);
+#pragma unroll 1
+  for (int a = 0; a < 25; a++) {
+    int j = nj, e = ne;
+    const int ox = nox, oy = noy, zb0 = nzb0, zb1 = nzb1;
+    window(a + 1, nj, ne, nox, noy, nzb0, nzb1);
     const unsigned code_xy = unsigned((ox + 1) | ((oy + 1) << 2));
     const float offx = float(ox) * two_h, offy = float(oy) * two_h;
     while (j < e) {
       const int m = min(e - j, CL_CHUNK);
       unsigned mask = 0u;
-      unsigned long long zc = 0ull;  // oz + 1 of every candidate, 2 bits each
 #pragma unroll 4
       for (int t = 0; t < m; t++) {
         const float4 c = B.aux[j + t];
-        const int oz = col_oz(__float_as_int(c.w), ci.z);
-        zc |= (unsigned long long)(oz + 1) << (2 * t);
+        // the candidate's cell lies below, at or above the particle's own
+        // as its aux position lies before zb0, in [zb0, zb1) or after
+        const float offz = j + t < zb0 ? -two_h : (j + t < zb1 ? 0.f : two_h);
         float3 d;
         d.x = (pi.x - c.x) - offx;
         d.y = (pi.y - c.y) - offy;
-        d.z = (pi.z - c.z) - float(oz) * two_h;
+        d.z = (pi.z - c.z) - offz;
         if (fmaf(d.z, d.z, fmaf(d.y, d.y, d.x * d.x)) <= r2cut) mask |= 1u << t;
       }
       const int over = cnt + __popc(mask) - CL_CAP;
@@ -396,7 +410,7 @@ __global__ void __launch_bounds__(64, MINW) k_interaction_col(DevParams* __restr
       while (mask) {
         const int t = __ffs(mask) - 1;
         mask &= mask - 1u;
-        const unsigned oz = unsigned(zc >> (2 * t)) & 3u;
+        const unsigned oz = unsigned(j + t >= zb0) + unsigned(j + t >= zb1);  // oz + 1
         s_list[(head + cnt) & (CL_CAP - 1)][lane] = int((unsigned(j + t) << 6) | code_xy | (oz << 4));
         cnt++;
       }
@@ -433,8 +447,8 @@ __global__ void k_neighbor_list_col(const DevParams* __restrict__ P, Arrays A, B
   for (int a = 0; a < 25; a++) {
     const int dx = a / 5 - 2, dy = a % 5 - 2;
     const float gx = col_gap(dx, g.fx, h, margin), gy = col_gap(dy, g.fy, h, margin);
-    int j, e, ox, oy;
-    col_range(P, B.sub_start, g, dx, dy, gx * gx, gy * gy, true, j, e, ox, oy);
+    int j, e, ox, oy, zb0, zb1;
+    col_range(P, B.sub_start, g, dx, dy, gx * gx, gy * gy, true, j, e, ox, oy, zb0, zb1);
     for (; j < e; j++) {
       const float4 c = B.aux[j];
       const int w = __float_as_int(c.w);
