@@ -33,7 +33,7 @@ constexpr int KP = 136;   // padded row length (bf16) of W2t, W3t, W4t and activ
 constexpr int W1T = H * K1P, W2T = H * KP, W4T = 16 * KP;
 constexpr int WPACK = W1T + 2 * W2T + W4T;  // bf16 per member
 constexpr int BPACK = 3 * H + 16;           // biases per member (FP32, b4 padded)
-constexpr int MLP_WAVES = 8;
+constexpr int MLP_WAVES = 16;
 constexpr int SUR_SLOTS = 64;  // statistics counter lines (8 u64 each; audit max every 16 u32)
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
@@ -51,35 +51,46 @@ __device__ __forceinline__ float gelu_grad(float x) {
   const float du = 0.7978845608f * fmaf(0.134145f, x2, 1.0f);
   return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * du;
 }
-// The same function as gelu(): 0.5 x (1 + tanh(u)) = x / (1 + exp(-2u)),
-// evaluated with the hardware exp2 and reciprocal (about 1 ulp each; the
-// result is rounded to bf16 next).
-__device__ __forceinline__ float gelu_fast(float x) {
-  const float u = 0.7978845608f * fmaf(0.044715f * x, x * x, x);
-  return x * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-2.885390082f * u));  // exp(-2u)
-}
 __device__ __forceinline__ unsigned short bf16_bits(float f) {  // v_cvt_pk_bf16_f32, round to nearest even
   return __builtin_bit_cast(unsigned short, static_cast<__bf16>(f));
 }
 
 // ------------------------------------------------------ forward (MFMA) ----
-// One 512-thread block per CU keeps a member's weights in LDS (84 KB); each
-// of its 8 waves walks pairs of 16-row tiles, so every weight fragment read
-// from LDS feeds two MFMAs.  v_mfma_f32_16x16x32_bf16: A (16 rows x 32 k)
-// and B (32 k x 16 cols) take 8 consecutive k per lane, row / column =
-// lane % 16, k = 32 kb + 8 (lane / 16) + i; D row = 4 (lane / 16) + i,
-// column = lane % 16.  A layer reads all its input fragments into registers,
-// then overwrites the same per-wave activation rows with its outputs.
-// Padded row strides make the 16-byte fragment reads conflict-free.  Every
-// wave runs the same number of iterations, so the block barriers (which
-// order a wave's own LDS reads before its overwrites) stay uniform.
-constexpr int MLP_TILES = 2;  // 16-row tiles per wave and iteration
+// One 1024-thread block per CU keeps a member's weights in LDS (84 KB); each
+// of its 16 waves walks pairs of 16-sample tiles, so every weight fragment
+// read from LDS feeds two MFMAs.  Each layer is computed transposed,
+// out^T = W x in^T, with v_mfma_f32_16x16x32_bf16: A = 16 output features x
+// 32 inputs (W rows, feature = lane % 16), B = 32 inputs x 16 samples
+// (sample = lane % 16), 8 inputs per lane (MFMA k = 32 kb + 8 (lane / 16)
+// + i).  D holds features 16 nb + 4 (lane / 16) + j of sample lane % 16, so
+// over the 8 output blocks nb a lane holds 32 features of its sample: as
+// bf16, exactly the B fragments of the next layer once the next layer's
+// weight columns are stored in that order (k = 32 kb + 8 g + i reads
+// feature 16 (2 kb + i / 4) + 4 g + i % 4; sur_pack).  Activations thus
+// stay in registers from layer to layer.  The accumulator starts at the
+// bias.
+constexpr int MLP_TILES = 2;  // 16-sample tiles per wave and iteration
+// The same function as gelu(): 0.5 x (1 + tanh(u)) = x / (1 + exp(-2u)),
+// with -2u log2(e) = x (GA + GB x^2), evaluated with the hardware exp2 and
+// reciprocal (about 1 ulp each; the result is rounded to bf16 next).
+constexpr float GA = -2.0f * 0.7978845608f * 1.4426950409f;
+constexpr float GB = GA * 0.044715f;
+__device__ __forceinline__ float gelu_folded(float x) {
+  const float e = __builtin_amdgcn_exp2f(x * fmaf(x * x, GB, GA));
+  return x * __builtin_amdgcn_rcpf(1.0f + e);
+}
+__device__ __forceinline__ uint2 gelu4_bf16(f32x4 v) {
+  return make_uint2(unsigned(bf16_bits(gelu_folded(v[0]))) | (unsigned(bf16_bits(gelu_folded(v[1]))) << 16),
+                    unsigned(bf16_bits(gelu_folded(v[2]))) | (unsigned(bf16_bits(gelu_folded(v[3]))) << 16));
+}
+__device__ __forceinline__ bf16x8 frag(uint2 lo, uint2 hi) {
+  return __builtin_bit_cast(bf16x8, make_uint4(lo.x, lo.y, hi.x, hi.y));
+}
 __global__ void __launch_bounds__(64 * MLP_WAVES) k_mlp_bf16(const float* __restrict__ X, int n,
                                                              const unsigned short* __restrict__ Wg,
                                                              const float* __restrict__ bg, float* __restrict__ Y) {
   __shared__ __attribute__((aligned(16))) unsigned short sW[WPACK];
-  __shared__ float sB[BPACK];
-  __shared__ __attribute__((aligned(16))) unsigned short sH[MLP_WAVES][MLP_TILES][16][KP];
+  __shared__ __attribute__((aligned(16))) float sB[BPACK];
   for (int k = threadIdx.x; k < WPACK / 8; k += blockDim.x)
     reinterpret_cast<uint4*>(sW)[k] = reinterpret_cast<const uint4*>(Wg)[k];
   for (int k = threadIdx.x; k < BPACK; k += blockDim.x) sB[k] = bg[k];
@@ -91,67 +102,68 @@ __global__ void __launch_bounds__(64 * MLP_WAVES) k_mlp_bf16(const float* __rest
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int r16 = lane & 15, kg = lane >> 4;
   const int ngroups = (n + 16 * MLP_TILES - 1) / (16 * MLP_TILES);
-  const int stride = gridDim.x * MLP_WAVES;
-  const int iters = (ngroups + stride - 1) / stride;
-  for (int it = 0; it < iters; it++) {
-    const int row0 = ((it * gridDim.x + blockIdx.x) * MLP_WAVES + wv) * 16 * MLP_TILES;
+  for (int grp = blockIdx.x * MLP_WAVES + wv; grp < ngroups; grp += gridDim.x * MLP_WAVES) {
+    const int row0 = grp * 16 * MLP_TILES;
     // layer 1: 14 (padded to 32) -> 128
-    bf16x8 a[MLP_TILES];
+    bf16x8 x[MLP_TILES];
     for (int t = 0; t < MLP_TILES; t++) {
       const int row = row0 + 16 * t + r16;
       for (int i = 0; i < 8; i++) {
         const int k = 8 * kg + i;
-        a[t][i] = static_cast<__bf16>((row < n && k < SUR_IN) ? X[size_t(row) * SUR_XS + k] : 0.f);
+        x[t][i] = static_cast<__bf16>((row < n && k < SUR_IN) ? X[size_t(row) * SUR_XS + k] : 0.f);
       }
     }
+    uint2 h[MLP_TILES][8];  // bf16 activations: features 16 nb + 4 kg + 0..3 of the lane's sample
+#pragma unroll
     for (int nb = 0; nb < 8; nb++) {
-      const int col = nb * 16 + r16;
-      const bf16x8 b = *reinterpret_cast<const bf16x8*>(W1 + col * K1P + 8 * kg);
-      const float bias = sB[col];
-      for (int t = 0; t < MLP_TILES; t++) {
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t], b, acc, 0, 0, 0);
-        for (int i = 0; i < 4; i++) sH[wv][t][4 * kg + i][col] = bf16_bits(gelu_fast(acc[i] + bias));
-      }
+      const bf16x8 w = *reinterpret_cast<const bf16x8*>(W1 + (nb * 16 + r16) * K1P + 8 * kg);
+      const f32x4 bias = *reinterpret_cast<const f32x4*>(sB + nb * 16 + 4 * kg);
+#pragma unroll
+      for (int t = 0; t < MLP_TILES; t++)
+        h[t][nb] = gelu4_bf16(__builtin_amdgcn_mfma_f32_16x16x32_bf16(w, x[t], bias, 0, 0, 0));
     }
-    __syncthreads();
-    // layers 2 and 3: 128 -> 128, in place
+    // layers 2 and 3: 128 -> 128
+#pragma unroll
     for (int layer = 0; layer < 2; layer++) {
       const unsigned short* Wl = layer == 0 ? W2 : W3;
       const float* bl = sB + H * (1 + layer);
-      bf16x8 af[MLP_TILES][4];
-      for (int t = 0; t < MLP_TILES; t++)
-        for (int kb = 0; kb < 4; kb++)
-          af[t][kb] = *reinterpret_cast<const bf16x8*>(&sH[wv][t][r16][kb * 32 + 8 * kg]);
-      __syncthreads();
+      uint2 o[MLP_TILES][8];
+#pragma unroll
       for (int nb = 0; nb < 8; nb++) {
-        const int col = nb * 16 + r16;
+        const f32x4 bias = *reinterpret_cast<const f32x4*>(bl + nb * 16 + 4 * kg);
         f32x4 acc[MLP_TILES];
-        for (int t = 0; t < MLP_TILES; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int t = 0; t < MLP_TILES; t++) acc[t] = bias;
+#pragma unroll
         for (int kb = 0; kb < 4; kb++) {
-          const bf16x8 b = *reinterpret_cast<const bf16x8*>(Wl + col * KP + kb * 32 + 8 * kg);
-          for (int t = 0; t < MLP_TILES; t++) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[t][kb], b, acc[t], 0, 0, 0);
+          const bf16x8 w = *reinterpret_cast<const bf16x8*>(Wl + (nb * 16 + r16) * KP + kb * 32 + 8 * kg);
+#pragma unroll
+          for (int t = 0; t < MLP_TILES; t++)
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, frag(h[t][2 * kb], h[t][2 * kb + 1]), acc[t], 0, 0, 0);
         }
-        const float bias = bl[col];
-        for (int t = 0; t < MLP_TILES; t++)
-          for (int i = 0; i < 4; i++) sH[wv][t][4 * kg + i][col] = bf16_bits(gelu_fast(acc[t][i] + bias));
+#pragma unroll
+        for (int t = 0; t < MLP_TILES; t++) o[t][nb] = gelu4_bf16(acc[t]);
+        __builtin_amdgcn_sched_barrier(0);  // keep each block's weight reads next to its MFMAs
       }
-      __syncthreads();
+#pragma unroll
+      for (int t = 0; t < MLP_TILES; t++)
+#pragma unroll
+        for (int nb = 0; nb < 8; nb++) h[t][nb] = o[t][nb];
     }
     // layer 4: 128 -> 7 (padded to 16), no activation
+    const f32x4 b4 = *reinterpret_cast<const f32x4*>(sB + 3 * H + 4 * kg);
+#pragma unroll
     for (int t = 0; t < MLP_TILES; t++) {
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      f32x4 acc = b4;
+#pragma unroll
       for (int kb = 0; kb < 4; kb++) {
-        const bf16x8 af = *reinterpret_cast<const bf16x8*>(&sH[wv][t][r16][kb * 32 + 8 * kg]);
-        const bf16x8 b = *reinterpret_cast<const bf16x8*>(W4 + r16 * KP + kb * 32 + 8 * kg);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, b, acc, 0, 0, 0);
+        const bf16x8 w = *reinterpret_cast<const bf16x8*>(W4 + r16 * KP + kb * 32 + 8 * kg);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, frag(h[t][2 * kb], h[t][2 * kb + 1]), acc, 0, 0, 0);
       }
-      for (int i = 0; i < 4; i++) {
-        const int row = row0 + 16 * t + 4 * kg + i;
-        if (row < n && r16 < SUR_OUT) Y[size_t(row) * SUR_YS + r16] = acc[i] + sB[3 * H + r16];
-      }
+      const int row = row0 + 16 * t + r16;
+      for (int i = 0; i < 4; i++)
+        if (row < n && 4 * kg + i < SUR_OUT) Y[size_t(row) * SUR_YS + 4 * kg + i] = acc[i];
     }
-    __syncthreads();
   }
 }
 
@@ -356,13 +368,17 @@ void sur_pack(const float* p, unsigned short* w, float* b) {
   std::fill(w, w + WPACK, (unsigned short)0);
   for (int n = 0; n < H; n++)
     for (int k = 0; k < SUR_IN; k++) w[n * K1P + k] = bf(W1[k * H + n]);
+  // layers 2-4 read their inputs in the order k_mlp_bf16's registers hold
+  // them: packed column k = 32 kb + 8 g + i is input feature
+  // 16 (2 kb + i / 4) + 4 g + i % 4
+  auto feat = [](int k) { return 16 * (2 * (k >> 5) + ((k & 7) >> 2)) + 4 * ((k >> 3) & 3) + (k & 3); };
   for (int n = 0; n < H; n++)
     for (int k = 0; k < H; k++) {
-      w[W1T + n * KP + k] = bf(W2[k * H + n]);
-      w[W1T + W2T + n * KP + k] = bf(W3[k * H + n]);
+      w[W1T + n * KP + k] = bf(W2[feat(k) * H + n]);
+      w[W1T + W2T + n * KP + k] = bf(W3[feat(k) * H + n]);
     }
   for (int n = 0; n < SUR_OUT; n++)
-    for (int k = 0; k < H; k++) w[W1T + 2 * W2T + n * KP + k] = bf(W4[k * SUR_OUT + n]);
+    for (int k = 0; k < H; k++) w[W1T + 2 * W2T + n * KP + k] = bf(W4[feat(k) * SUR_OUT + n]);
   std::fill(b, b + BPACK, 0.f);
   for (int n = 0; n < H; n++) { b[n] = b1[n]; b[H + n] = b2[n]; b[2 * H + n] = b3[n]; }
   for (int n = 0; n < SUR_OUT; n++) b[3 * H + n] = b4[n];
