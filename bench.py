@@ -113,6 +113,18 @@ WORKLOADS = {
 }
 
 
+def _cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(case_name, seconds_target=20.0):
     """FP32 oracle on the host cores for a bounded number of whole steps of
     the same case: its per-particle loops run on OpenMP threads (one per
@@ -129,8 +141,10 @@ def cpu_baseline(case_name, seconds_target=20.0):
     o.abs2rel()
     o.neighbor_search()
     o.sort_arrays_primary()
+    print(f"[cpu baseline] {case_name} set up; one serial step", file=sys.stderr, flush=True)
     O.set_threads(1)
     serial = o.run(1)["seconds"]
+    print(f"[cpu baseline] serial step {serial:.2f} s; steps on {threads} threads", file=sys.stderr, flush=True)
     O.set_threads(threads)
     r = o.run(1)
     nsteps, secs = 1, r["seconds"]
@@ -139,6 +153,7 @@ def cpu_baseline(case_name, seconds_target=20.0):
         r = o.run(more)
         nsteps, secs = more, r["seconds"]
     return {"value": case.npm * nsteps / secs, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "cpu_model": _cpu_model(), "host_cpus": os.cpu_count(),
             "sample": f"{nsteps} full step(s) of the same {case.npm}-soil-particle case, FP32 oracle, "
                       f"{threads} OpenMP threads",
             "seconds": secs, "serial_value": case.npm / serial}
@@ -168,6 +183,29 @@ def timed_steps(ctx, steps, graph):
     return elapsed, ctx.kernel_times()
 
 
+def pick_variant(ctx, log=None):
+    """Untimed: the fastest of the bit-identical interaction variants
+    (lq_autotune), then the COLUMNS variant (same neighbour sets, other
+    summation order) with its own block order and register build, kept if it
+    is faster.  Returns (variant, tile_shape, tile_tz, remap)."""
+    from retainwall_code_amd import _abi
+    variant, tile_shape, tile_tz, remap, ms_exact = ctx.autotune(reps=3)
+    best = (variant, tile_shape, tile_tz, remap)
+    try:
+        ctx.set_interaction(_abi.INTERACTION_COLUMNS, 1)
+    except Exception:  # periodic planes, Morton keys or too many cells along z
+        return best
+    v2, s2, t2, r2, ms_cols = ctx.autotune(reps=3)
+    if log:
+        log(f"[bench] autotune: variant {variant} {ms_exact:.3f} ms, columns {ms_cols:.3f} ms per step's two calls")
+    if ms_cols < ms_exact:
+        return v2, s2, t2, r2
+    if variant == _abi.INTERACTION_TILE:
+        ctx.set_tile(tile_shape, tile_tz)
+    ctx.set_interaction(variant, remap)
+    return best
+
+
 def bench_batch(args):
     """N independent cases (sizes log-uniform 5k-200k soil particles, heights,
     materials and wall motions drawn per case, dx = 5 mm) stepped together in
@@ -189,7 +227,7 @@ def bench_batch(args):
              _abi.INTERACTION_NEIGHBOR_LIST: "neighbor_list", _abi.INTERACTION_PAIR_LIST: "pair_list",
              _abi.INTERACTION_COLUMNS: "columns"}
     if args.variant == "auto":
-        variant, shape, tz, remap, _ = ctx.autotune(reps=2)
+        variant, shape, tz, remap = pick_variant(ctx, lambda m: print(m, file=sys.stderr, flush=True))
     else:
         variant, remap = {v: k for k, v in names.items()}[args.variant], args.remap
         ctx.set_interaction(variant, remap)
@@ -229,71 +267,168 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
-def bench_surrogate(args):
-    """Gain test of the HypoCS surrogate: two contexts of the case advance
-    identically through warm-up while one logs exact (x, y) tuples; two
-    members are trained on them; then both contexts run the same steps, one
-    exact and one with the surrogate (guard + 1 % audit), each timed, with the
-    wall force every 10 steps.  Wall-force error = max_t |F_s - F_p| /
-    max_t |F_p|."""
-    from retainwall_code_amd import cases, sim, surrogate
-    case = cases.config(args.case)
-    plain = sim.open_case(case)
-    sur = sim.open_case(case)
-    # every 48th slot per call: 2 calls x warm-up steps x npm / 48 rows, all
-    # of them kept (no overflow), so the training set is reproducible
-    every = max(1, -(-2 * args.warmup * case.npm // 2_000_000))
-    every = max(every, 48)
-    surrogate.start_log(sur, every=every, capacity=2_000_000)
-    plain.step(args.warmup)
-    sur.step(args.warmup)
-    sur.sync()
-    x, y = surrogate.read_log(sur, max_rows=2_000_000)
-    surrogate.start_log(sur, every=0, capacity=0)
-    t0 = time.perf_counter()
+def surrogate_campaign(rows, epochs, log):
+    """Training set of the HypoCS surrogate: exact (x, y) tuples logged from
+    runs of other workloads -- c1 (2000 steps), c2 drawn with another seed
+    (another jitter and void-ratio realisation, 1000 steps) and c3 (60
+    steps) -- rows / 3 from each, spread evenly over each run; then the
+    normalisation, the guard box and two members trained on all of them
+    (Adam, batch 4096, the library's FP32 training kernels)."""
+    from retainwall_code_amd import cases, surrogate
+    plan = [("c1", lambda: cases.config("c1"), 2000), ("c2 seed 7", lambda: cases.config("c2", seed=7), 1000),
+            ("c3", lambda: cases.config("c3"), 60)]
+    xs, ys, info = [], [], []
+    for name, make, steps in plan:
+        t = time.perf_counter()
+        case = make()
+        x, y = surrogate.collect(case, steps, rows // len(plan))
+        xs.append(x)
+        ys.append(y)
+        info.append({"source": name, "steps": steps, "soil_particles": case.npm, "tuples": int(len(x)),
+                     "seconds": round(time.perf_counter() - t, 1)})
+        log(f"[surrogate] logged {len(x)} tuples from {steps} steps of {name} ({info[-1]['seconds']} s)")
+        del case
+    x = np.concatenate(xs)
+    y = np.concatenate(ys)
+    del xs, ys
     norm = surrogate.normalisation(x, y)
+    t = time.perf_counter()
     members, losses = [], []
+    # learning rate 1e-3 for `epochs` epochs, then 3e-4 and 1e-4 for half as many each
+    sched = [(epochs, 1e-3), (max(1, epochs // 2), 3e-4), (max(1, epochs // 2), 1e-4)]
     for seed in (1, 2):
-        p, hist = surrogate.train(x, y, norm, epochs=args.sur_epochs, batch=4096, lr=1e-3, seed=seed)
+        p, hist = None, []
+        for k, (ne, lr) in enumerate(sched):
+            p, h = surrogate.train(x, y, norm, epochs=ne, batch=4096, lr=lr, seed=seed + 100 * k, params=p)
+            hist += [round(float(v), 6) for v in h]
         members.append(p)
-        losses.append([round(float(v), 6) for v in hist])
-    t_train = time.perf_counter() - t0
+        losses.append(hist)
+        log(f"[surrogate] member {seed}: loss per epoch {losses[-1]}")
     lo, hi = surrogate.guard_box(x)
+    return members, norm, lo, hi, {"sources": info, "tuples": int(len(x)),
+                                   "schedule": [{"epochs": ne, "lr": lr} for ne, lr in sched],
+                                   "seconds": round(time.perf_counter() - t, 1), "loss_per_epoch": losses}
+
+
+def _run_chunks(ctx, steps, every, force):
+    """Steps in graph-replayed chunks of `every`, wall force after each;
+    the first chunk (which captures the graph) is not timed."""
+    F, el, timed = [], 0.0, 0
+    for k in range(steps // every):
+        ctx.sync()
+        ta = time.perf_counter()
+        ctx.step(every, graph=True)
+        ctx.sync()
+        if k > 0:
+            el += time.perf_counter() - ta
+            timed += every
+        F.append(force(ctx))
+    return np.array(F), el, timed
+
+
+def bench_surrogate(args):
+    """The learned form (config c5): the surrogate trained on other runs
+    (surrogate_campaign), then args.sur_steps steps of the case exact and with
+    the surrogate (two members, guard box, 1 % audit), wall force every 50
+    steps; wall-force error = max_t |F_s - F_p| / max_t |F_p|.  With --batch
+    N the two forms run a batch of N cases of the same sand (uneven sizes,
+    heights and wall motions) and plain c2 is timed beside them."""
+    from retainwall_code_amd import _abi, cases, sim, surrogate
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    members, norm, lo, hi, train = surrogate_campaign(args.sur_rows, args.sur_epochs, log)
+    every = 50
+    out_extra = {}
+    if args.batch:
+        cs = cases.batch_cases(args.batch, seed=5, dr=0.005, material=cases.hypo_cs_sand(), void_range=(0.65, 0.75))
+        plain, sur = sim.open_batch(cs), sim.open_batch(cs)
+        npm = sum(c.npm for c in cs)
+        boxes = [c.probe_box for c in cs]
+        force = lambda ctx: np.array(ctx.wall_force_cases(boxes))  # noqa: E731
+        workload = f"batch of {len(cs)} wall cases of the c2 sand (5k-200k soil each, dx 5 mm), HypoCS surrogate"
+        c2 = cases.config("c2")
+        ref = sim.open_case(c2)
+        ref.step(10, graph=True)
+        ref.sync()
+        pick_variant(ref, log)
+        t = time.perf_counter()
+        ref.step(100, graph=True)
+        ref.sync()
+        out_extra["plain_c2_value"] = round(c2.npm * 100 / (time.perf_counter() - t), 1)
+        ref.close()
+    else:
+        case = cases.config(args.case)
+        plain, sur = sim.open_case(case), sim.open_case(case)
+        npm = case.npm
+        force = lambda ctx: ctx.wall_force(case.probe_box)  # noqa: E731
+        workload = f"{args.case} with the HypoCS surrogate (2 members, guard, 1% audit)"
+    for ctx in (plain, sur):  # the same interaction variant for both forms
+        ctx.step(2, graph=True)
+        ctx.sync()
+    v = pick_variant(plain, log)
+    if v[0] == _abi.INTERACTION_TILE:
+        sur.set_tile(v[1], v[2])
+    sur.set_interaction(v[0], v[3])
+    out_extra["interaction_variant"] = int(v[0])
     surrogate.install(sur, members, norm, lo, hi, ens_tol=args.ens_tol, audit_frac=0.01)
     surrogate.enable(sur)
     surrogate.stats(sur, reset=True)
     res = {}
     for name, ctx in (("exact", plain), ("surrogate", sur)):
-        forces, el = [], 0.0
-        for _ in range(max(1, args.steps // 10)):
-            ctx.sync()
-            ta = time.perf_counter()
-            ctx.step(10, graph=True)
-            ctx.sync()
-            el += time.perf_counter() - ta
-            forces.append(ctx.wall_force(case.probe_box))
-        res[name] = (np.array(forces), el, 10 * len(forces))
-    fp, fs = res["exact"][0], res["surrogate"][0]
-    werr = float(np.max(np.abs(fs - fp)) / max(np.max(np.abs(fp)), 1e-30))
+        res[name] = _run_chunks(ctx, args.sur_steps, every, force)
+        log(f"[surrogate] {name}: {res[name][2]} timed steps in {res[name][1]:.3f} s")
     st = surrogate.stats(sur)
-    calls = st["surrogate"] + st["fallback_range"] + st["fallback_ensemble"]
-    v_exact = case.npm * res["exact"][2] / res["exact"][1]
-    v_sur = case.npm * res["surrogate"][2] / res["surrogate"][1]
-    out = {
-        "metric": "particle-steps/sec", "value": round(v_sur, 1), "unit": "particle-steps/s", "n_gpus": 1,
-        "steps": res["surrogate"][2], "warmup": args.warmup, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32 (surrogate bf16 MFMA)", "data": "synthetic",
-        "config": {"workload": f"{args.case} with the HypoCS surrogate (2 members, guard, 1% audit)",
-                   "soil_particles": case.npm, "graph": True},
-        "exact_value": round(v_exact, 1), "gain": round(v_sur / v_exact, 3),
-        "wall_force_error": werr, "wall_force_kN": {"exact": [round(float(v), 6) for v in fp],
-                                                    "surrogate": [round(float(v), 6) for v in fs]},
-        "accepted_fraction": round(st["surrogate"] / max(calls, 1), 4), "stats": st,
-        "training": {"tuples": int(len(x)), "seconds": round(t_train, 2), "loss_per_epoch": losses},
-    }
-    print(json.dumps(out), flush=True)
     plain.close()
     sur.close()
+    # noise floor of the observable: the same exact steps with another
+    # interaction variant (same neighbour sets, other FP32 summation order)
+    other = sim.open_batch(cs) if args.batch else sim.open_case(case)
+    other.step(2, graph=True)
+    alt = _abi.INTERACTION_NEIGHBOR_LIST if v[0] == _abi.INTERACTION_COLUMNS else _abi.INTERACTION_COLUMNS
+    try:
+        other.set_interaction(alt, 1)
+    except Exception:
+        alt = _abi.INTERACTION_REFERENCE_ORDER
+        other.set_interaction(alt, 1)
+    fo = _run_chunks(other, args.sur_steps, every, force)[0]
+    other.close()
+    log(f"[surrogate] spread run with variant {alt} done")
+    fp, fs = res["exact"][0], res["surrogate"][0]
+
+    def rel(a, b):  # max_t |a - b| / max_t |b|, per case when there are several
+        return np.max(np.abs(a - b), axis=0) / np.maximum(np.max(np.abs(b), axis=0), 1e-30)
+
+    errs, spread = rel(fs, fp), rel(fo, fp)
+    werr = float(np.max(errs))
+    out_extra["exact_spread"] = float(np.max(spread))
+    out_extra["spread_variant"] = int(alt)
+    if fp.ndim == 2:  # per case: error of each case's history, worst case reported
+        out_extra["wall_force_error_median_case"] = float(np.median(errs))
+        out_extra["exact_spread_median_case"] = float(np.median(spread))
+        ok = errs <= np.maximum(3.0 * spread, 0.005)
+        out_extra["cases_within_3x_spread_or_0.5pct"] = int(np.count_nonzero(ok))
+    else:
+        out_extra["wall_force_kN"] = {"exact": [round(float(x), 6) for x in fp],
+                                      "surrogate": [round(float(x), 6) for x in fs],
+                                      "exact_other_order": [round(float(x), 6) for x in fo]}
+    calls = st["surrogate"] + st["fallback_range"] + st["fallback_ensemble"]
+    v_exact = npm * res["exact"][2] / res["exact"][1]
+    v_sur = npm * res["surrogate"][2] / res["surrogate"][1]
+    out = {
+        "metric": "particle-steps/sec", "value": round(v_sur, 1), "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": res["surrogate"][2], "warmup": every, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 (surrogate bf16 MFMA)", "data": "synthetic",
+        "config": {"workload": workload, "soil_particles": npm, "graph": True, "force_every": every},
+        "exact_value": round(v_exact, 1), "gain": round(v_sur / v_exact, 3),
+        "wall_force_error": werr, "accepted_fraction": round(st["surrogate"] / max(calls, 1), 4), "stats": st,
+        "training": train,
+    }
+    if "plain_c2_value" in out_extra:
+        out["gain_vs_plain_c2"] = round(v_sur / out_extra["plain_c2_value"], 3)
+    out.update(out_extra)
+    print(json.dumps(out), flush=True)
 
 
 def main():
@@ -318,7 +453,9 @@ def main():
     ap.add_argument("--surrogate", action="store_true",
                     help="gain test of the HypoCS surrogate on the case: log, train, then time both forms")
     ap.add_argument("--ens-tol", type=float, default=0.1, help="surrogate ensemble tolerance (normalised units)")
-    ap.add_argument("--sur-epochs", type=int, default=16, help="surrogate training epochs per member")
+    ap.add_argument("--sur-epochs", type=int, default=4, help="surrogate training epochs per member")
+    ap.add_argument("--sur-rows", type=int, default=50_000_000, help="surrogate training tuples (logged from c1-c3)")
+    ap.add_argument("--sur-steps", type=int, default=1000, help="surrogate comparison steps (force every 50)")
     ap.add_argument("--probe-copy", type=int, default=0, metavar="BYTES",
                     help="only run the STREAM copy probe over BYTES (HBM bandwidth / counter calibration)")
     args = ap.parse_args()
@@ -340,10 +477,10 @@ def main():
         ctx.close()
         return
 
-    if args.batch:
-        return bench_batch(args)
     if args.surrogate:
         return bench_surrogate(args)
+    if args.batch:
+        return bench_batch(args)
 
     case = cases.config(args.case)
     print(f"[bench] case={args.case} np={case.np} npb={case.npb} npm={case.npm}", file=sys.stderr, flush=True)
@@ -364,22 +501,7 @@ def main():
         tile_shape, tile_tz = (int(v) for v in args.tile.split(","))
         ctx.set_tile(tile_shape, tile_tz)
     elif args.variant == "auto":
-        # untimed: the fastest of the bit-identical variants, then the COLUMNS
-        # variant (same neighbour sets, other summation order) if it is faster
-        variant, tile_shape, tile_tz, remap, ms_exact = ctx.autotune(reps=3)
-        if key_mode == _abi.KEY_LINEAR and not case.params.get("periodbound"):
-            best = (variant, tile_shape, tile_tz, remap)
-            ctx.set_interaction(_abi.INTERACTION_COLUMNS, 1)
-            v2, s2, t2, r2, ms_cols = ctx.autotune(reps=3)
-            print(f"[bench] autotune: {names[variant]} {ms_exact:.3f} ms, columns {ms_cols:.3f} ms per step's "
-                  "two calls", file=sys.stderr, flush=True)
-            if ms_cols < ms_exact:
-                variant, tile_shape, tile_tz, remap = v2, s2, t2, r2
-            else:
-                variant, tile_shape, tile_tz, remap = best
-                if variant == _abi.INTERACTION_TILE:
-                    ctx.set_tile(tile_shape, tile_tz)
-                ctx.set_interaction(variant, remap)
+        variant, tile_shape, tile_tz, remap = pick_variant(ctx, lambda m: print(m, file=sys.stderr, flush=True))
     elif args.variant == "columns":  # untimed: block order and register build of the COLUMNS kernel
         ctx.set_interaction(variant, remap)
         variant, tile_shape, tile_tz, remap, _ = ctx.autotune(reps=3)

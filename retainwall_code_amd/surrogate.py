@@ -105,7 +105,7 @@ def start_log(ctx, every=8, capacity=1_000_000):
     ctx._chk(ctx.lib.lq_surrogate_log(ctx.h, every, capacity), "lq_surrogate_log")
 
 
-def read_log(ctx, max_rows=None):
+def read_log(ctx, max_rows=None, sort_keys=None):
     """Logged (x, y) rows, in lexicographic order of (x, y).
 
     Waves reserve log rows with an atomic counter, so the device order of the
@@ -118,5 +118,25 @@ def read_log(ctx, max_rows=None):
     ctx._chk(ctx.lib.lq_surrogate_log_read(ctx.h, x.ctypes.data, y.ctypes.data, cap, C.byref(n)),
              "lq_surrogate_log_read")
     x, y = x[:n.value], y[:n.value]
-    order = np.lexsort(np.concatenate([x, y], axis=1).T[::-1])
+    if sort_keys is None:
+        order = np.lexsort(np.concatenate([x, y], axis=1).T[::-1])
+    else:  # a few input columns: cheaper, and as reproducible unless rows tie in all of them
+        order = np.lexsort(x[:, list(sort_keys)[::-1]].T)
     return x[order].copy(), y[order].copy()
+
+
+def collect(case, steps, rows, graph=True):
+    """Exact steps of a fresh context of `case`, logging about `rows` (x, y)
+    tuples spread over the whole run: every k-th soil slot per HypoCS call
+    with k = 2 * steps * npm / rows.  Returns the rows (sorted on four input
+    columns)."""
+    from . import sim
+    ctx = sim.open_case(case)
+    every = max(1, (2 * steps * case.npm) // max(rows, 1))
+    cap = (2 * steps * case.npm) // every + case.npm // every + 1024
+    start_log(ctx, every=every, capacity=cap)
+    ctx.step(steps, graph=graph)
+    ctx.sync()
+    x, y = read_log(ctx, max_rows=cap, sort_keys=(0, 1, 2, 6))
+    ctx.close()
+    return x, y

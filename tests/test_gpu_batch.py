@@ -1,7 +1,9 @@
 """Batched contexts (lq_create_batch): several independent wall cases stepped
 by one launch per kernel must reproduce, case by case and bit for bit, what a
 context holding that case alone computes -- every per-slot field after every
-step, absolute positions, counts and the wall force.  Cases differ in size,
+step, absolute positions, counts and the wall force (also with the COLUMNS
+variant: a 2h column holds one case's particles, the cases being laid out
+one empty cell apart in the batch grid).  Cases differ in size,
 backfill height, material (phi_c, e0) and wall motion (translation / toe
 rotation), and the batch is also replayed through a hipGraph."""
 import sys
@@ -41,7 +43,8 @@ def compare_case(batch, k, single, step):
     assert batch.case_counts(k) == single.counts()
 
 
-@pytest.mark.parametrize("variant", [_abi.INTERACTION_NEIGHBOR_LIST, _abi.INTERACTION_REFERENCE_ORDER])
+@pytest.mark.parametrize("variant", [_abi.INTERACTION_NEIGHBOR_LIST, _abi.INTERACTION_REFERENCE_ORDER,
+                                     _abi.INTERACTION_COLUMNS])
 def test_batch_equals_single_cases(variant):
     cs = small_batch()
     say(f"[batch] {len(cs)} cases, np = {[c.np for c in cs]}, variant={variant}")
@@ -145,3 +148,42 @@ def test_batch_checkpoint_restart(tmp_path):
         for n in STATE:
             np.testing.assert_array_equal(bits(b.get_case(k, n)), ref[k][n], err_msg=f"case {k} {n}")
     b.close()
+
+
+def test_batch_with_surrogate_equals_single_cases():
+    """The HypoCS surrogate (two members, guard box, 1 % audit) in a batched
+    context: every case's state, and the guard/audit counts summed over the
+    cases, equal those of one-case contexts with the same network.  The
+    network is untrained and the guard box wide, so it answers nearly every
+    call; the audit draws on (original id, step), which a case keeps in the
+    batch."""
+    sys.path.insert(0, __file__.rsplit("/", 1)[0])
+    import mlp_ref as M
+    from retainwall_code_amd import surrogate
+    cs = small_batch(3, seed=11)
+    say(f"[batch surrogate] {len(cs)} cases, np = {[c.np for c in cs]}")
+    members = [M.init_params(s).astype(np.float32) for s in (1, 2)]
+    norm = np.concatenate([np.zeros(14), np.full(14, 1e3), np.zeros(7), np.full(7, 1e-3)]).astype(np.float32)
+    lo, hi = np.full(14, -1e30, np.float32), np.full(14, 1e30, np.float32)
+    batch = sim.open_batch(cs)
+    singles = [sim.open_case(c) for c in cs]
+    for ctx in [batch] + singles:
+        surrogate.install(ctx, members, norm, lo, hi, ens_tol=1e3, audit_frac=0.01)
+        surrogate.enable(ctx)
+    for step in range(3):
+        batch.step(1)
+        for s in singles:
+            s.step(1)
+        batch.sync()
+        for k, s in enumerate(singles):
+            compare_case(batch, k, s, step)
+    sb = surrogate.stats(batch)
+    ss = [surrogate.stats(s) for s in singles]
+    say(f"  batch stats {sb}")
+    for key in ("surrogate", "fallback_range", "fallback_ensemble", "audited", "tensile_branch"):
+        assert sb[key] == sum(s[key] for s in ss), key
+    assert sb["surrogate"] > 0 and sb["audited"] > 0
+    assert sb["audit_max"] == max(s["audit_max"] for s in ss)
+    batch.close()
+    for s in singles:
+        s.close()
