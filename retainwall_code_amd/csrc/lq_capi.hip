@@ -546,6 +546,7 @@ static int create_impl(const lq_params* cases, int n_cases, int device, lq_ctx**
   B.rec_rows = int(n);
   if (cols_supported(par)) {
     st |= dalloc(c, &B.aux, n);
+    st |= dalloc(c, &B.soil_aux, n);
     const size_t ncol = size_t(par.cell_num[0]) * size_t(par.cell_num[1]);
     st |= dalloc(c, &B.sub_start, ncol * size_t(8 * par.cell_num[2] + 1));
     st |= dalloc(c, &B.colcnt, ncol);
@@ -1139,7 +1140,7 @@ static void enq_interaction_kernel(lq_ctx* c) {
                        LQ_INTERACTION_PAIR_LIST, c->xcd_remap, false);  // guarded cell-scan fallback
   } else if (c->par.interaction_variant == LQ_INTERACTION_COLUMNS) {
     launch_col_index(c->stream, c->dP, c->par.cell_num[0], c->par.cell_num[1], c->B.cur, c->B);
-    launch_interaction_col(c->stream, c->dP, c->par.np, c->npm0, c->B.cur, c->B, c->xcd_remap, c->col_wide);
+    launch_interaction_col(c->stream, c->dP, c->npm0, c->B.cur, c->B, c->xcd_remap, c->col_wide);
   } else if (c->par.interaction_variant == LQ_INTERACTION_TILE) {
     if (c->npm0 > 0)
       launch_interaction_tile(c->stream, c->dP, c->par.cell_num[0], c->par.cell_num[1], c->par.cell_num[2], c->B.cur,
@@ -1231,7 +1232,7 @@ static void enq_fused(lq_ctx* c, int stage) {
   {
     Section s(c, LQ_K_INTERACTION);
     launch_col_index(c->stream, c->dP, c->par.cell_num[0], c->par.cell_num[1], c->B.cur, c->B);
-    launch_interaction_col(c->stream, c->dP, c->par.np, c->npm0, c->B.cur, c->B, c->xcd_remap, c->col_wide,
+    launch_interaction_col(c->stream, c->dP, c->npm0, c->B.cur, c->B, c->xcd_remap, c->col_wide,
                            stage + 1);
   }
   {

@@ -92,7 +92,8 @@ __global__ void __launch_bounds__(1024) k_col_scan(const DevParams* __restrict__
 // the particle's pair record at its aux position, as four float4 planes of
 // rec_rows entries: position with the mass in w (negative for contact
 // walls), velocity and density, sigma/rho^2, m/rho and trace(sigma/rho^2)
-// (k_pair_prep's quantities).
+// (k_pair_prep's quantities).  soil_aux lists the aux positions of the soil
+// entries in aux order: the interaction's lanes.
 constexpr int COL_MAX_CNZ = 256;
 constexpr int COL_MAX_BUCKETS = 8 * COL_MAX_CNZ + 1;
 constexpr int COL_LDS_PTS = 8192;  // particles of a column whose codes stay in LDS
@@ -101,6 +102,7 @@ __global__ void __launch_bounds__(COL_THREADS) k_col_build(const DevParams* __re
                                                            const int4* __restrict__ colinfo) {
   __shared__ int cntw[COL_MAX_BUCKETS];
   __shared__ int start[COL_MAX_BUCKETS];
+  __shared__ int sstart[COL_MAX_BUCKETS];  // exclusive scan of the soil counts alone
   __shared__ unsigned char oct[COL_LDS_PTS];
   __shared__ unsigned char czs[COL_LDS_PTS];
   const int tid = threadIdx.x, lane = tid & 63;
@@ -110,7 +112,7 @@ __global__ void __launch_bounds__(COL_THREADS) k_col_build(const DevParams* __re
   const int w0 = ci.x, s0 = ci.y, nw = ci.z, n = ci.z + ci.w;
   const int nb = 8 * cnz;
   const float h = P->h;
-  for (int b = tid; b <= nb; b += COL_THREADS) { cntw[b] = 0; start[b] = 0; }
+  for (int b = tid; b <= nb; b += COL_THREADS) { cntw[b] = 0; start[b] = 0; sstart[b] = 0; }
   __syncthreads();
   auto slot_of = [&](int t) { return t < nw ? w0 + t : s0 + (t - nw); };
   auto oct_of = [&](float4 p) { return (sub_bits(p.x, h) << 2) | (sub_bits(p.y, h) << 1) | sub_bits(p.z, h); };
@@ -123,24 +125,27 @@ __global__ void __launch_bounds__(COL_THREADS) k_col_build(const DevParams* __re
     atomicAdd(t < nw ? &cntw[bucket(o, cz)] : &start[bucket(o, cz)], 1);
   }
   __syncthreads();
-  if (tid < 64) {  // exclusive scan of cntw + cnts (start holds cnts) over nb + 1 buckets
-    int carry = 0;
+  if (tid < 64) {  // exclusive scans of cntw + cnts and of cnts (start holds cnts) over nb + 1 buckets
+    int carry = 0, scarry = 0;
     const size_t sbase = size_t(col) * (nb + 1);
     const int base = w0 + (s0 - npb);
     for (int b0 = 0; b0 <= nb; b0 += 64) {
       const int b = b0 + lane;
-      const int v = b <= nb ? cntw[b] + start[b] : 0;
-      int x = v;
+      const int vs = b <= nb ? start[b] : 0;
+      const int v = b <= nb ? cntw[b] + vs : 0;
+      int x = v, xs = vs;
 #pragma unroll
       for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(x, o);
-        if (lane >= o) x += y;
+        const int y = __shfl_up(x, o), ys = __shfl_up(xs, o);
+        if (lane >= o) { x += y; xs += ys; }
       }
       if (b <= nb) {
         start[b] = carry + x - v;
+        sstart[b] = scarry + xs - vs;
         B.sub_start[sbase + b] = base + carry + x - v;
       }
       carry += __shfl(x, 63);
+      scarry += __shfl(xs, 63);
     }
   }
   __syncthreads();
@@ -164,6 +169,7 @@ __global__ void __launch_bounds__(COL_THREADS) k_col_build(const DevParams* __re
     }
     const int k = base + start[b] + (soil ? cntw[b] : 0) + rank;
     B.aux[k] = make_float4(p.x, p.y, p.z, __int_as_float((j << 2) | (cz & 3)));
+    if (soil) B.soil_aux[(s0 - npb) + sstart[b] + rank] = k;
     const float4 v = A.vel[j];
     const float rho = v.w;
     const float m = soil ? fabsf(p.w) : vol_def * rho;
@@ -297,11 +303,11 @@ __device__ __forceinline__ void pair_terms_fast(const PairConst& P, float i_kdr,
 }
 
 // ----------------------------------------------------- interaction ----
-// One lane per aux entry, so the 64 lanes of a wave are consecutive
-// particles of one sub-column: for every window they read neighbouring
-// stretches of the same target sub-column, and their pair records (stored in
+// One lane per soil aux entry (soil_aux, in aux order), so the 64 lanes of
+// a wave are consecutive soil particles of one sub-column: for every window
+// they read neighbouring stretches of the same target sub-column, and their pair records (stored in
 // aux order, one plane per float4, so that 8 lanes share a 128-byte line)
-// lie close together.  Wall entries leave their lane idle.  Hits
+// lie close together.  Hits
 // go to per-lane FIFO rings in LDS as (aux index << 6 | cell-offset code) and
 // are evaluated in ring order, as in the neighbour-list variant.
 constexpr int CL_CAP = 32;    // ring entries per lane (power of two)
@@ -342,11 +348,10 @@ __global__ void __launch_bounds__(64, MINW) k_interaction_col(DevParams* __restr
                                                               int remap) {
   __shared__ int s_list[CL_CAP][64];
   const int lane = threadIdx.x;
-  const int k = xcd_block(remap) * 64 + lane;
-  const int nidx = P->npb + P->npm;
-  const int kh = k < nidx ? k : 0;
+  const int l = xcd_block(remap) * 64 + lane;
+  const bool active = l < P->npm;
+  const int kh = active ? B.soil_aux[l] : 0;
   const int i = __float_as_int(B.aux[kh].w) >> 2;
-  const bool active = k < nidx && i >= P->npb;
   const size_t rows = size_t(B.rec_rows);
   const float4 ph = B.rec[kh], vri = B.rec[rows + kh], sria = B.rec[2 * rows + kh], srib = B.rec[3 * rows + kh];
   const float3 pi = ld3(ph);
@@ -473,18 +478,19 @@ void launch_col_index(hipStream_t s, const DevParams* P, int cnx, int cny, Array
   hipLaunchKernelGGL(k_col_scan, dim3(1), dim3(1024), 0, s, P, (const int2*)B.colcnt, B.colinfo);
   hipLaunchKernelGGL(k_col_build, dim3(ncol), dim3(COL_THREADS), 0, s, P, A, B, (const int4*)B.colinfo);
 }
-// np: lanes over every aux entry (the live ones are npb + npm of them)
+// lanes over the soil aux entries (at most npm_max; the live count is P->npm)
 template <int FUSE>
-static void launch_col(hipStream_t s, DevParams* P, int np, Arrays A, const Buffers& B, int xcd_remap, int wide) {
-  if (wide) hipLaunchKernelGGL((k_interaction_col<4, FUSE>), dim3(cblocks(np, 64)), dim3(64), 0, s, P, A, B, xcd_remap);
-  else hipLaunchKernelGGL((k_interaction_col<5, FUSE>), dim3(cblocks(np, 64)), dim3(64), 0, s, P, A, B, xcd_remap);
+static void launch_col(hipStream_t s, DevParams* P, int npm_max, Arrays A, const Buffers& B, int xcd_remap, int wide) {
+  const dim3 grid(cblocks(npm_max, 64));
+  if (wide) hipLaunchKernelGGL((k_interaction_col<4, FUSE>), grid, dim3(64), 0, s, P, A, B, xcd_remap);
+  else hipLaunchKernelGGL((k_interaction_col<5, FUSE>), grid, dim3(64), 0, s, P, A, B, xcd_remap);
 }
-void launch_interaction_col(hipStream_t s, DevParams* P, int np, int npm_max, Arrays A, const Buffers& B,
+void launch_interaction_col(hipStream_t s, DevParams* P, int npm_max, Arrays A, const Buffers& B,
                             int xcd_remap, int wide, int fuse) {
   if (npm_max <= 0) return;
-  if (fuse == 1) launch_col<1>(s, P, np, A, B, xcd_remap, wide);
-  else if (fuse == 2) launch_col<2>(s, P, np, A, B, xcd_remap, wide);
-  else launch_col<0>(s, P, np, A, B, xcd_remap, wide);
+  if (fuse == 1) launch_col<1>(s, P, npm_max, A, B, xcd_remap, wide);
+  else if (fuse == 2) launch_col<2>(s, P, npm_max, A, B, xcd_remap, wide);
+  else launch_col<0>(s, P, npm_max, A, B, xcd_remap, wide);
 }
 void launch_neighbor_list_col(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const Buffers& B, int max_nb,
                               int* counts, int* ids) {

@@ -147,6 +147,7 @@ struct Buffers {
   // sub-column and sub-cell height) and the start of every bucket; in this
   // variant rec is indexed by aux position, not by slot
   float4* aux;
+  int* soil_aux;   // aux positions of the soil entries, in aux order
   int* sub_start;
   float4* fl_pos;  // fused steps: position and cell of first-layer particles before their update
   int4* fl_cell;

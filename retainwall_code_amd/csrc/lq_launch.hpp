@@ -46,7 +46,7 @@ void launch_col_index(hipStream_t s, const DevParams* P, int cnx, int cny, Array
 // fuse: 0 interaction only; 1 with the predictor (cuPredictorMaterial) and
 // 2 with the corrector (cuCorrectorMaterial) of every soil particle whose
 // contact force needs no pass B
-void launch_interaction_col(hipStream_t s, DevParams* P, int np, int npm_max, Arrays A, const Buffers& B,
+void launch_interaction_col(hipStream_t s, DevParams* P, int npm_max, Arrays A, const Buffers& B,
                             int xcd_remap, int wide, int fuse = 0);
 void launch_neighbor_list_col(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const Buffers& B, int max_nb,
                               int* counts, int* ids);
