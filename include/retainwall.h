@@ -78,7 +78,9 @@ enum lq_key_mode { LQ_KEY_LINEAR = 0, LQ_KEY_MORTON = 1 };
  * fewer than 2^25 particles; COLUMNS needs LQ_KEY_LINEAR, no periodic planes
  * and at most 256 cells along z.  lq_autotune picks the fastest for the
  * current state within the bit-identical four, or between the two block
- * orders of COLUMNS for a context set to COLUMNS. */
+ * orders and register builds of COLUMNS for a context set to COLUMNS.  In a
+ * batched context (lq_create_batch) every variant gives each case the
+ * results, bit for bit, of a one-case context running the same variant. */
 enum lq_interaction_variant {
   LQ_INTERACTION_REFERENCE_ORDER = 0,
   LQ_INTERACTION_TILE = 1,

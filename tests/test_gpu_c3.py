@@ -10,7 +10,9 @@ field in a packed neighbour entry would silently wrap.
   relative-frame predicate r^2 <= r2_cut of particle_interaction.cu:30-35);
 * one interaction call with NEIGHBOR_LIST bit-identical to REFERENCE_ORDER
   (which indexes slots directly, as particle_interaction.cu:117-140), and
-  both equal to the FP32 oracle, on all soil and on the slots >= 2^25.
+  both equal to the FP32 oracle, on all soil and on the slots >= 2^25;
+* the COLUMNS variant: the same neighbour count for every soil slot, and
+  one interaction call within the FP64 rate tolerance of the FP32 oracle.
 """
 import ctypes as C
 
@@ -19,7 +21,7 @@ import pytest
 
 from conftest import relerr
 from retainwall_code_amd import _abi, cases, sim
-from test_gpu_parity import TOL_F32_RATE, bits, oracle_from, say
+from test_gpu_parity import TOL_F32_RATE, TOL_F64_RATE, bits, oracle_from, say
 
 pytestmark = pytest.mark.gpu
 
@@ -111,6 +113,10 @@ def test_c3_neighbour_sets_above_2p25(c3):
         assert len(s) <= counts[k] <= len(s) + len(f), f"slot {i}: count {counts[k]} vs {len(s)} (+{len(f)})"
         nfuzzy += len(f)
     say(f"  1000 slots >= 2^25: counts and ids agree ({sum(len(s) for s in sets)} pairs, {nfuzzy} at the cut)")
+    ctx.set_interaction(_abi.INTERACTION_COLUMNS)
+    counts_col, _ = ctx.neighbor_lists(1)
+    np.testing.assert_array_equal(counts_col, counts)
+    say(f"  COLUMNS windows: the same neighbour count for all {npm} soil slots")
     ctx.close()
 
 
@@ -128,7 +134,8 @@ def test_c3_interaction_variants_and_oracle(c3, oracle_mod):
     saved = {k: ctx.get(k) for k in ("cvar", "cmag")}
     o32 = oracle_from(ctx, oracle_mod, np.float32, c3)
     outs = {}
-    for name, variant in (("nl", _abi.INTERACTION_NEIGHBOR_LIST), ("ref", _abi.INTERACTION_REFERENCE_ORDER)):
+    for name, variant in (("nl", _abi.INTERACTION_NEIGHBOR_LIST), ("ref", _abi.INTERACTION_REFERENCE_ORDER),
+                          ("col", _abi.INTERACTION_COLUMNS)):
         say(f"[c3 interaction] {name}")
         for k, v in saved.items():
             ctx.set(k, v)
@@ -137,7 +144,7 @@ def test_c3_interaction_variants_and_oracle(c3, oracle_mod):
         ctx.sync()
         outs[name] = {k: ctx.get(k) for k in ("accad", "acestn", "spin", "cvar", "cmag")}
     ctx.close()
-    for k in outs["nl"]:
+    for k in ("accad", "acestn", "spin", "cvar", "cmag"):
         np.testing.assert_array_equal(bits(outs["nl"][k]), bits(outs["ref"][k]), err_msg=k)
     say("  NEIGHBOR_LIST == REFERENCE_ORDER bit for bit")
     say("[c3 interaction] FP32 oracle (threads)")
@@ -151,3 +158,7 @@ def test_c3_interaction_variants_and_oracle(c3, oracle_mod):
             e = relerr(outs["nl"][k][rows], o32[k][rows])
             say(f"  {label} {k} relerr vs FP32 oracle {e:.2e}")
             assert e <= TOL_F32_RATE, (label, k)
+            e = relerr(outs["col"][k][rows], o32[k][rows])
+            say(f"  {label} {k} COLUMNS relerr vs FP32 oracle {e:.2e}")
+            assert e <= TOL_F64_RATE, (label, k)  # other summation order and fast pair terms
+        np.testing.assert_array_equal(outs["col"]["cvar"][rows][:, 2], o32["cvar"][rows][:, 2])  # slip flags
