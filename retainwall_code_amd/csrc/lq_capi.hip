@@ -1128,9 +1128,10 @@ static int enq_corner(lq_ctx* c) {
     launch_corner(c->stream, c->dP, c->npm0, c->B.cur, c->B.cell_beg, c->B.cell_end, c->B.wall_near);
   return 0;
 }
-static int enq_shepard(lq_ctx* c) {
+static int enq_shepard(lq_ctx* c, bool due_only = false) {
   Section s(c, LQ_K_SHEPARD);
-  launch_shepard(c->stream, c->dP, c->npm0, c->B.cur, c->B.cell_beg, c->B.cell_end, c->B.str_tmp_a, c->B.str_tmp_b);
+  launch_shepard(c->stream, c->dP, c->npm0, c->B.cur, c->B.cell_beg, c->B.cell_end, c->B.str_tmp_a, c->B.str_tmp_b,
+                 true, due_only);
   return 0;
 }
 static void enq_interaction_kernel(lq_ctx* c) {
@@ -1259,7 +1260,7 @@ static int enq_step(lq_ctx* c) {
   if (c->par.n_motion > 0) launch_rigid_body_update(c->stream, c->dP);
   enq_adami(c);
   enq_corner(c);
-  if (shep != 0 && c->host_step % shep == 0 && c->host_step != 0) enq_shepard(c);
+  if (shep > 0) enq_shepard(c, true);  // runs when due at the device's step count
   if (fused(c)) {
     enq_fused(c, 0);
   } else {
@@ -1335,7 +1336,7 @@ extern "C" int lq_step(lq_ctx* c, int nsteps, int use_graph) {
   NEED_UPLOAD(c);
   if (nsteps < 0) return fail(c, LQ_ERR_ARG, "lq_step: nsteps < 0");
   hipSetDevice(c->device);
-  const bool graph_ok = use_graph && c->par.stress_shepard_filter == 0;
+  const bool graph_ok = use_graph != 0;
   int done = 0;
   if (graph_ok && !c->searched && nsteps >= 3) {
     enq_step(c);  // the first search sorts in full; the captured steps then merge

@@ -15,8 +15,10 @@ void launch_rigid_body_update(hipStream_t s, DevParams* P);
 void launch_adami(hipStream_t s, const DevParams* P, int npb, Arrays A, const int* cb, const int* ce);
 void launch_corner(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const int* cb, const int* ce,
                    const int* wall_near);
+// due_only: the kernels exit unless the step driver's filter is due at the
+// device's current step (a step graph carries them on every step)
 void launch_shepard(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const int* cb, const int* ce,
-                    float4* oa, float2* ob, bool copy_back = true);
+                    float4* oa, float2* ob, bool copy_back = true, bool due_only = false);
 void launch_reset_deferred(hipStream_t s, DevParams* P, int list_build);
 void launch_plist_build(hipStream_t s, DevParams* P, int npm_max, Arrays A, const Buffers& B, const int* cb,
                         const int* ce, int xcd_remap);

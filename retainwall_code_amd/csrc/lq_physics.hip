@@ -329,12 +329,19 @@ __global__ void __launch_bounds__(256) k_corner(const DevParams* __restrict__ P,
 }
 
 // ----------------------------------------------------------- Shepard ----
-// cuStressRegularizationShepard + loop, field_smoothing.cu:26-90.
+// The step driver's filter cadence (loquat.cu:548-561: every `shepard`
+// steps, not at step 0), decided on the device so that a captured step
+// graph carries the filter launches on every replay.
+__device__ __forceinline__ bool shepard_due(const DevParams* __restrict__ P) {
+  return P->shepard > 0 && P->step != 0 && P->step % P->shepard == 0;
+}
+// cuStressRegularizationShepard + loop, field_smoothing.cu:26-90.  due_only:
+// run only at a step where the filter is due.
 __global__ void __launch_bounds__(256) k_shepard(const DevParams* __restrict__ P, Arrays A,
                                                  const int* __restrict__ cbeg, const int* __restrict__ cend,
-                                                 float4* __restrict__ oa, float2* __restrict__ ob) {
+                                                 float4* __restrict__ oa, float2* __restrict__ ob, int due_only) {
   int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= P->npm) return;
+  if (idx >= P->npm || (due_only && !shepard_due(P))) return;
   const int i = P->npb + idx;
   const float3 pi = ld3(A.pos[i]);
   const int4 ci = A.cellpos[i];
@@ -371,9 +378,9 @@ __global__ void __launch_bounds__(256) k_shepard(const DevParams* __restrict__ P
 }
 
 __global__ void k_copy_soil_str(const DevParams* __restrict__ P, Arrays A, const float4* __restrict__ oa,
-                                const float2* __restrict__ ob) {
+                                const float2* __restrict__ ob, int due_only) {
   int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= P->npm) return;
+  if (idx >= P->npm || (due_only && !shepard_due(P))) return;
   int i = P->npb + idx;
   A.str_a[i] = oa[i];
   A.str_b[i] = ob[i];
@@ -943,11 +950,11 @@ void launch_corner(hipStream_t s, const DevParams* P, int npm_max, Arrays A, con
     hipLaunchKernelGGL(k_corner, dim3(blocks(npm_max, 256)), dim3(256), 0, s, P, A, cb, ce, wall_near, 1);
 }
 void launch_shepard(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const int* cb, const int* ce,
-                    float4* oa, float2* ob, bool copy_back) {
+                    float4* oa, float2* ob, bool copy_back, bool due_only) {
   if (npm_max <= 0) return;
-  hipLaunchKernelGGL(k_shepard, dim3(blocks(npm_max, 256)), dim3(256), 0, s, P, A, cb, ce, oa, ob);
+  hipLaunchKernelGGL(k_shepard, dim3(blocks(npm_max, 256)), dim3(256), 0, s, P, A, cb, ce, oa, ob, int(due_only));
   if (copy_back) hipLaunchKernelGGL(k_copy_soil_str, dim3(blocks(npm_max, 256)), dim3(256), 0, s, P, A, (const float4*)oa,
-                     (const float2*)ob);
+                     (const float2*)ob, int(due_only));
 }
 void launch_reset_deferred(hipStream_t s, DevParams* P, int list_build) {
   hipLaunchKernelGGL(k_reset_deferred, dim3(1), dim3(1), 0, s, P, list_build);

@@ -315,6 +315,27 @@ def test_steps_with_shepard(oracle_mod):
     ctx.close()
 
 
+def test_shepard_steps_graph_equals_eager():
+    """The filter's cadence is decided on the device, so a captured two-step
+    graph filters exactly at the steps eager launches do (here every third
+    step, so the filtered step alternates between the graph's two steps)."""
+    case = cases.config("tiny", shepard=3)
+    a, b = sim.open_case(case), sim.open_case(case)
+    a.step(9)
+    b.step(9, graph=True)
+    a.sync()
+    b.sync()
+    for k in ("pos", "vel", "str", "stn", "cvar", "cmag"):
+        np.testing.assert_array_equal(bits(a.get(k)), bits(b.get(k)), err_msg=k)
+    c = sim.open_case(cases.config("tiny"))  # the same steps without the filter differ
+    c.step(9, graph=True)
+    c.sync()
+    assert not np.array_equal(bits(a.get("str")), bits(c.get("str")))
+    say("[shepard graph] 9 steps, filter every 3: graph == eager bit for bit")
+    for x in (a, b, c):
+        x.close()
+
+
 def test_multistep_and_wall_force(oracle_mod):
     case = cases.config("tiny")
     n, every = 40, 10
