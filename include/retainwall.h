@@ -285,7 +285,9 @@ int lq_corrector(lq_ctx* ctx);
 int lq_step(lq_ctx* ctx, int nsteps, int use_graph);
 
 /* Wall earth-pressure probe, utilities.cu:578-670: F = 0.001 * sum fx [kN]
- * over soil slots with x_abs <= box[0], box[1] <= z_abs <= box[2].  The
+ * over slots npb..np (soil, deleted particles' last contact force
+ * included, as the reference's loop) with x_abs <= box[0],
+ * box[1] <= z_abs <= box[2].  The
  * reference hard-codes box = {0.04+bt+3dr, bt+0.04, bt+1.04} and sums
  * serially in FP32; here the fx values are summed exactly (128-bit fixed
  * point, 2^-64 resolution, independent of order) and scaled once.
