@@ -458,6 +458,8 @@ def main():
     ap.add_argument("--sur-steps", type=int, default=1000, help="surrogate comparison steps (force every 50)")
     ap.add_argument("--probe-copy", type=int, default=0, metavar="BYTES",
                     help="only run the STREAM copy probe over BYTES (HBM bandwidth / counter calibration)")
+    ap.add_argument("--probe-mlp", type=int, default=0, metavar="ROWS",
+                    help="only run the BF16 surrogate forward on ROWS random rows, 6 times (k_mlp_bf16 timing)")
     args = ap.parse_args()
     args.graph = not args.eager
     if args.gpus != 1:
@@ -477,6 +479,22 @@ def main():
         ctx.close()
         return
 
+    if args.probe_mlp:
+        from retainwall_code_amd import surrogate
+        rng = np.random.default_rng(0)
+        p = (rng.standard_normal(surrogate.NPARAM) * 0.1).astype(np.float32)
+        norm = np.concatenate([np.zeros(surrogate.NIN), np.ones(surrogate.NIN), np.zeros(surrogate.NOUT),
+                               np.ones(surrogate.NOUT)]).astype(np.float32)
+        x = rng.standard_normal((args.probe_mlp, surrogate.NIN)).astype(np.float32)
+        t = time.perf_counter()
+        for _ in range(6):
+            y = surrogate.forward(p, norm, x, bf16=True)
+        print(json.dumps({"probe": "mlp_bf16_forward", "rows": args.probe_mlp, "calls": 6,
+                          "flops_per_row": 2 * (surrogate.NIN * surrogate.NH + 2 * surrogate.NH * surrogate.NH +
+                                                surrogate.NH * surrogate.NOUT),
+                          "host_seconds_incl_copies": round(time.perf_counter() - t, 3),
+                          "mean_abs_y": float(np.abs(y).mean())}), flush=True)
+        return
     if args.surrogate:
         return bench_surrogate(args)
     if args.batch:
