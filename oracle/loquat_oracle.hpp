@@ -5,10 +5,17 @@
 // Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline may load
 // it, and only as the checker; the product never links or calls it.
 //
-// Templated on the arithmetic type T (float mirrors the device path operation
-// for operation, double is the FP64 reference).  Fused multiply-adds are
-// written explicitly (mad) exactly where the device kernels use them; the
-// file is compiled with -ffp-contract=off so the compiler adds no others.
+// Templated on the arithmetic type T (float mirrors this project's device
+// path operation for operation, double is the FP64 reference).  Fused
+// multiply-adds are written explicitly (mad) exactly where the device kernels
+// use them; the file is compiled with -ffp-contract=off so the compiler adds
+// no others.  The reference's own FP32 build evaluates GetKernel /
+// GetGradKernel (functions.h:142-143) and the HypoCS constants
+// (constitutive_model.h:226-230) in double through double literals; here they
+// are evaluated in T, so FP32 bit parity with the reference there is parity
+// unpinned (the FP64 instantiation follows the reference's double path).
+// Likewise parity unpinned: the rounding of CUDA's norm3df / powf / expf,
+// for which std::sqrt of an explicit FMA chain and std::pow / std::exp stand.
 //
 // Deterministic semantics chosen where the reference races or leaves state
 // stale (the same choices are made by the device path):
@@ -20,7 +27,10 @@
 //    pass-A values; the min-reduced K_i is written back after pass B;
 //  * the rotation branch of RigidBodyUpdate reads a zero translation table
 //    when the body has none;
-//  * a zero rotation increment leaves the wall normal unchanged.
+//  * a zero rotation increment leaves the wall normal unchanged;
+//  * after a deletion, a particle that moves from a slot past npb + npm into
+//    the stress/strain rows the sort gathers keeps its own rows (the
+//    reference reads an unrefreshed row of its tensor cache there).
 #pragma once
 #include <cmath>
 #include <cstdint>
