@@ -317,6 +317,30 @@ int lq_wall_pressure_profile(lq_ctx* ctx, const float geom[4], int nbins, float*
  * forbid allocation).  Synchronises. */
 #define LQ_PROFILE_NOALLOC_MAX_BINS 4096
 int lq_wall_pressure_profile_noalloc(lq_ctx* ctx, const float geom[4], int nbins, float* out, size_t bytes);
+/* Stress sensors at caller-given points, GetSoilStress/cuGetStressZz
+ * (utilities.cu:470-528; the reference's InterpolateResults call at
+ * loquat.cu:463 is commented out and its point is hard-coded).  points
+ * f32[n][3] absolute; per point the Shepard-weighted stress
+ * sum_j sigma_j W V_j / sum_j W V_j over soil slots with |x - x_j| < 2h,
+ * x_j the absolute position (relative + cell 2h in FP32) and V_j = m_j/rho_j
+ * (summed serially in cell order).  reference_rows != 0: the reference's
+ * cells (the point's own y row of cells, x +-1, every z level); 0: the 27
+ * cells around the point.  out f32[n][8] = (xx yy zz xy xz yz, sum W V,
+ * number of slots); the reference computes xx alone.  Single-case
+ * contexts.  Synchronises. */
+int lq_sensor_stress(lq_ctx* ctx, const float* points, int n, int reference_rows, float* out, size_t bytes);
+/* Free-surface heights at caller-given points (x, y), GetSurface
+ * (utilities.cu:26-91) and GetSurfaceAtPoint (:371-427), which the
+ * reference's driver never calls and which compare their absolute points
+ * with the solver's cell-relative positions; here positions are absolute.
+ * points f32[n][2]; out f32[n] = the largest absolute z of a soil slot near
+ * the vertical line through the point, -1e10 where there is none.
+ * radius > 0: horizontal distance < radius, cells x +-1 and y +-1 around
+ * the point, every z (GetSurface, radius 0.5 dr); radius < 0: |x - x_j| <
+ * -radius over the point's own y row of cells, x +-1, every z
+ * (GetSurfaceAtPoint, -2h).  |radius| <= 2h.  Single-case contexts.
+ * Synchronises. */
+int lq_surface_heights(lq_ctx* ctx, const float* points, int n, float radius, float* out, size_t bytes);
 /* Strip surcharge on the free surface: every soil slot whose block id
  * (codes.x) equals block carries a downward force pressure * dr^2 (the load
  * on one lattice column), added with gravity in the interaction.  block -1

@@ -209,6 +209,15 @@ template <class T> static int step_T(Par<T>& P, MotionTab<T>* M, State<T>& S, do
                                        double* out) {                                                  \
     Par<T> P; load_par(p, P); State<T> S = view<T>(s); wall_profile(P, S, geom, nbins, out); return 0; \
   }                                                                                                    \
+  extern "C" int or_sensor_stress_##SFX(const lq_params* p, or_state* s, const float* pts, int n, int rows,     \
+                                        double* out) {                                                 \
+    Par<T> P; load_par(p, P); State<T> S = view<T>(s); sensor_stress(P, S, pts, n, rows, out); return 0; \
+  }                                                                                                    \
+  extern "C" int or_surface_heights_##SFX(const lq_params* p, or_state* s, const float* pts, int n,     \
+                                          float radius, double* out) {                                 \
+    Par<T> P; load_par(p, P); State<T> S = view<T>(s); surface_heights(P, S, pts, n, radius, out);      \
+    return 0;                                                                                          \
+  }                                                                                                    \
   extern "C" int or_run_##SFX(lq_params* p, lq_motion_table* tabs, or_state* s, int nsteps,          \
                               double* time, int64_t* step, const float* box, int force_every,          \
                               double* forces, int* ns2_identity_all, double* seconds) {                \

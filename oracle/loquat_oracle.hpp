@@ -1109,4 +1109,78 @@ template <class T> static void wall_profile(const Par<T>& P, const State<T>& S, 
   }
 }
 
+// GetSoilStress + cuGetStressZz, utilities.cu:470-528, at the given
+// absolute points: sum_j sigma_j W V_j / sum_j W V_j over soil slots with
+// q = |x - x_j| / h < 2 (x_j = relative + cell 2h), summed serially in cell
+// order; rows: the reference's cells (own y row, every z level, z outer, x
+// inner), else the 27 cells (z, y, x nested).  out[n][8] = (xx yy zz xy xz
+// yz, sum W V, slots).
+template <class T> static void sensor_stress(const Par<T>& P, const State<T>& S, const float* pts, int n, int rows,
+                                             double* out) {
+  for (int k = 0; k < n; k++) {
+    const T x = pts[3 * k], y = pts[3 * k + 1], z = pts[3 * k + 2];
+    const T scale = T(0.5) * P.i_h;
+    const int cx0 = int(std::floor(x * scale)), cy0 = int(std::floor(y * scale)), cz0 = int(std::floor(z * scale));
+    T sum[6] = {0, 0, 0, 0, 0, 0};
+    T wsum = 0;
+    int hits = 0;
+    const int z0 = rows ? 0 : cz0 - 1, z1 = rows ? P.cnz - 1 : cz0 + 1;
+    const int y0 = rows ? cy0 : cy0 - 1, y1 = rows ? cy0 : cy0 + 1;
+    for (int cz = z0; cz <= z1; cz++)
+      for (int cy = y0; cy <= y1; cy++)
+        for (int cx = cx0 - 1; cx <= cx0 + 1; cx++) {
+          if (cx < 0 || cy < 0 || cz < 0 || cx >= P.cnx || cy >= P.cny || cz >= P.cnz) continue;
+          const int c = linear_key(P, v3<int>{cx, cy, cz}) + P.ncell_material;
+          if (S.cell_beg[c] < 0) continue;
+          for (int j = S.cell_beg[c]; j < S.cell_end[c]; j++) {
+            const v3<T> a = abs_pos(P, cell_of(S.cellpos, j), ld3(S.pos, j));
+            const T dx = x - a.x, dy = y - a.y, dz = z - a.z;
+            const T q = std::sqrt(mad(dz, dz, mad(dy, dy, dx * dx))) * P.i_h;
+            if (!(q < T(2))) continue;
+            const T ker = kernelW(P.knorm, q);
+            const T vol = S.idm[4 * j + 1] / S.vel[4 * j + 3];
+            for (int m = 0; m < 6; m++) sum[m] = sum[m] + S.str[6 * j + m] * ker * vol;
+            wsum = wsum + ker * vol;
+            hits++;
+          }
+        }
+    if (wsum != T(0))
+      for (int m = 0; m < 6; m++) sum[m] = sum[m] / wsum;
+    for (int m = 0; m < 6; m++) out[8 * k + m] = double(sum[m]);
+    out[8 * k + 6] = double(wsum);
+    out[8 * k + 7] = double(hits);
+  }
+}
+
+// GetSurface (utilities.cu:26-91) / GetSurfaceAtPoint (:371-427) at the
+// given (x, y) points, on absolute positions: the largest z of a soil slot
+// with horizontal distance < radius over cells x +-1, y +-1 (radius > 0),
+// or with |dx| < -radius over the point's own y row, x +-1 (radius < 0);
+// every z level; -1e10 when none.
+template <class T> static void surface_heights(const Par<T>& P, const State<T>& S, const float* pts, int n,
+                                               float radius, double* out) {
+  for (int k = 0; k < n; k++) {
+    const T x = pts[2 * k], y = pts[2 * k + 1];
+    const T scale = T(0.5) * P.i_h;
+    const int cx0 = int(std::floor(x * scale)), cy0 = int(std::floor(y * scale));
+    const bool strip = radius < 0.f;
+    const int y0 = strip ? cy0 : cy0 - 1, y1 = strip ? cy0 : cy0 + 1;
+    T top = T(-1.0e10f);
+    for (int cz = 0; cz < P.cnz; cz++)
+      for (int cy = y0; cy <= y1; cy++)
+        for (int cx = cx0 - 1; cx <= cx0 + 1; cx++) {
+          if (cx < 0 || cy < 0 || cx >= P.cnx || cy >= P.cny) continue;
+          const int c = linear_key(P, v3<int>{cx, cy, cz}) + P.ncell_material;
+          if (S.cell_beg[c] < 0) continue;
+          for (int j = S.cell_beg[c]; j < S.cell_end[c]; j++) {
+            const v3<T> a = abs_pos(P, cell_of(S.cellpos, j), ld3(S.pos, j));
+            const T dx = x - a.x, dy = y - a.y;
+            const bool near = strip ? std::fabs(dx) < T(-radius) : std::sqrt(mad(dy, dy, dx * dx)) < T(radius);
+            if (near && a.z > top) top = a.z;
+          }
+        }
+    out[k] = double(top);
+  }
+}
+
 }  // namespace oracle

@@ -64,6 +64,10 @@ def lib():
             getattr(L, f"or_wall_force_{sfx}").restype = C.c_double
             getattr(L, f"or_wall_profile_{sfx}").argtypes = [P(_abi.Params), P(OrState), P(C.c_float), C.c_int,
                                                              C.c_void_p]
+            getattr(L, f"or_sensor_stress_{sfx}").argtypes = [P(_abi.Params), P(OrState), C.c_void_p, C.c_int,
+                                                              C.c_int, C.c_void_p]
+            getattr(L, f"or_surface_heights_{sfx}").argtypes = [P(_abi.Params), P(OrState), C.c_void_p, C.c_int,
+                                                                C.c_float, C.c_void_p]
             getattr(L, f"or_neighbor_lists_{sfx}").argtypes = [P(_abi.Params), P(OrState), C.c_int, C.c_void_p,
                                                                C.c_void_p]
             getattr(L, f"or_run_{sfx}").argtypes = [P(_abi.Params), P(_abi.MotionTable), P(OrState), C.c_int,
@@ -194,6 +198,26 @@ class Oracle:
         g = (C.c_float * 4)(*geom)
         out = np.zeros((nbins, 4), np.float64)
         getattr(lib(), f"or_wall_profile_{self.sfx}")(C.byref(self.p), C.byref(s), g, int(nbins), out.ctypes.data)
+        return out
+
+    def sensor_stress(self, points, reference_rows=False):
+        """GetSoilStress restated (utilities.cu:470-528) at absolute points:
+        [n][8] = (xx yy zz xy xz yz, sum W V, slots)."""
+        pts = np.ascontiguousarray(np.asarray(points, np.float32).reshape(-1, 3))
+        out = np.zeros((len(pts), 8), np.float64)
+        s = self._st()
+        getattr(lib(), f"or_sensor_stress_{self.sfx}")(C.byref(self.p), C.byref(s), pts.ctypes.data, len(pts),
+                                                       1 if reference_rows else 0, out.ctypes.data)
+        return out
+
+    def surface_heights(self, points, radius):
+        """GetSurface / GetSurfaceAtPoint restated (utilities.cu:26-91,
+        371-427) on absolute positions at (x, y) points."""
+        pts = np.ascontiguousarray(np.asarray(points, np.float32).reshape(-1, 2))
+        out = np.zeros(len(pts), np.float64)
+        s = self._st()
+        getattr(lib(), f"or_surface_heights_{self.sfx}")(C.byref(self.p), C.byref(s), pts.ctypes.data, len(pts),
+                                                         C.c_float(radius), out.ctypes.data)
         return out
 
     def run(self, nsteps, box=(0, 0, 0), force_every=0):

@@ -2,6 +2,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 #include <algorithm>
@@ -1493,6 +1494,56 @@ extern "C" int lq_wall_pressure_profile_noalloc(lq_ctx* c, const float geom[4], 
   if (e != hipSuccess)
     return fail(c, LQ_ERR_HIP, std::string("lq_wall_pressure_profile_noalloc: ") + hipGetErrorString(e));
   return LQ_OK;
+}
+
+// Probes at caller-given points (GetSoilStress, GetSurface,
+// GetSurfaceAtPoint: utilities.cu:26-132, 371-528, whose points are
+// hard-coded there).  Points and results travel through per-call device
+// buffers; single-case contexts.
+static int probe_points(lq_ctx* c, const char* name, const void* pts, size_t pts_bytes, void* out, size_t out_bytes,
+                        const std::function<void(const void*, void*)>& launch) {
+  hipSetDevice(c->device);
+  void* dpts = nullptr;
+  void* dout = nullptr;
+  hipError_t e = hipMalloc(&dpts, pts_bytes);
+  if (e == hipSuccess) e = hipMalloc(&dout, out_bytes);
+  if (e == hipSuccess) e = hipMemcpyAsync(dpts, pts, pts_bytes, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    launch(dpts, dout);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(out, dout, out_bytes, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (dpts) hipFree(dpts);
+  if (dout) hipFree(dout);
+  if (e != hipSuccess) return fail(c, LQ_ERR_HIP, std::string(name) + ": " + hipGetErrorString(e));
+  return LQ_OK;
+}
+
+extern "C" int lq_sensor_stress(lq_ctx* c, const float* points, int n, int reference_rows, float* out,
+                                size_t bytes) {
+  NEED_UPLOAD(c);
+  if (c->n_cases != 1) return fail(c, LQ_ERR_ARG, "lq_sensor_stress: single-case contexts only");
+  if (!points || !out || n <= 0 || bytes != size_t(n) * 32) return fail(c, LQ_ERR_ARG, "lq_sensor_stress: args");
+  std::vector<float4> p(n);
+  for (int k = 0; k < n; k++) p[k] = make_float4(points[3 * k], points[3 * k + 1], points[3 * k + 2], 0.f);
+  return probe_points(c, "lq_sensor_stress", p.data(), p.size() * sizeof(float4), out, bytes,
+                      [&](const void* dp, void* dout) {
+                        launch_sensor_stress(c->stream, c->dP, c->B.cur, c->B.cell_beg, c->B.cell_end,
+                                             (const float4*)dp, n, reference_rows ? 1 : 0, (float4*)dout);
+                      });
+}
+
+extern "C" int lq_surface_heights(lq_ctx* c, const float* points, int n, float radius, float* out, size_t bytes) {
+  NEED_UPLOAD(c);
+  if (c->n_cases != 1) return fail(c, LQ_ERR_ARG, "lq_surface_heights: single-case contexts only");
+  if (!points || !out || n <= 0 || bytes != size_t(n) * 4) return fail(c, LQ_ERR_ARG, "lq_surface_heights: args");
+  if (!(radius != 0.f && std::fabs(radius) <= c->hp.two_h))
+    return fail(c, LQ_ERR_ARG, "lq_surface_heights: 0 < |radius| <= 2h");
+  return probe_points(c, "lq_surface_heights", points, size_t(n) * 8, out, bytes, [&](const void* dp, void* dout) {
+    launch_surface_heights(c->stream, c->dP, c->B.cur, c->B.cell_beg, c->B.cell_end, (const float2*)dp, n, radius,
+                           (float*)dout);
+  });
 }
 
 extern "C" int lq_positions_absolute(lq_ctx* c, float* dst, size_t bytes) {

@@ -35,6 +35,11 @@ void launch_predictor_finish(hipStream_t s, DevParams* P, Arrays A, const Buffer
 void launch_corrector_finish(hipStream_t s, DevParams* P, int npb, int n_motion, Arrays A, const Buffers& B);
 void launch_predictor(hipStream_t s, DevParams* P, int np, Arrays A, const Buffers& B, int track_disp);
 void launch_corrector(hipStream_t s, DevParams* P, int npb, int npm_max, int n_motion, Arrays A, const Buffers& B);
+// pts: absolute points; out[2 k], out[2 k + 1] per point (see k_sensor_stress)
+void launch_sensor_stress(hipStream_t s, const DevParams* P, Arrays A, const int* cb, const int* ce, const float4* pts,
+                          int n, int rows, float4* out);
+void launch_surface_heights(hipStream_t s, const DevParams* P, Arrays A, const int* cb, const int* ce,
+                            const float2* pts, int n, float radius, float* out);
 void launch_wall_profile(hipStream_t s, const DevParams* P, int np, int npb, Arrays A, float4 geom, int nbins, int* last,
                          float4* out);
 void launch_wall_force(hipStream_t s, const DevParams* P, int np, int n_cases, Arrays A, const float* boxes,

@@ -873,6 +873,94 @@ __global__ void __launch_bounds__(256) k_profile_write(const DevParams* __restri
   out[bin] = r;
 }
 
+// ----------------------------------------------------------- sensors ----
+// GetSoilStress + cuGetStressZz (utilities.cu:470-528), at any points: the
+// Shepard-weighted stress sum_j sigma_j W V_j / sum_j W V_j over soil slots
+// with q = |x - x_j| / h < 2, x_j the absolute position (relative position +
+// cell 2h, FP32) and V_j = m_j / rho_j.  rows != 0 takes the reference's
+// cells (the point's own y row, every z level, z outer and x inner);
+// rows == 0 the 27 cells around the point (z, y, x nested).  One thread per
+// point, serial sums.  out[k] = (xx yy zz xy) and (xz yz, sum W V, hits).
+__global__ void __launch_bounds__(64) k_sensor_stress(const DevParams* __restrict__ P, Arrays A,
+                                                      const int* __restrict__ cbeg, const int* __restrict__ cend,
+                                                      const float4* __restrict__ pts, int n, int rows,
+                                                      float4* __restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const float4 x = pts[k];
+  const float scale = 0.5f * P->i_h, two_h = P->two_h;
+  const int cx0 = int(floorf(x.x * scale)), cy0 = int(floorf(x.y * scale)), cz0 = int(floorf(x.z * scale));
+  float s[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  float wsum = 0.f;
+  int hits = 0;
+  const int z0 = rows ? 0 : cz0 - 1, z1 = rows ? P->cnz - 1 : cz0 + 1;
+  const int y0 = rows ? cy0 : cy0 - 1, y1 = rows ? cy0 : cy0 + 1;
+  for (int cz = z0; cz <= z1; cz++)
+    for (int cy = y0; cy <= y1; cy++)
+      for (int cx = cx0 - 1; cx <= cx0 + 1; cx++) {
+        if (!in_grid(P, cx, cy, cz)) continue;
+        const int c = grid_key(P, cx, cy, cz) + P->ncell_material;
+        const int b = cbeg[c], e = cend[c];
+        for (int j = b; j < e; j++) {
+          const int4 cj = A.cellpos[j];
+          const float4 pj = A.pos[j];
+          const float ax = pj.x + float(cj.x) * two_h, ay = pj.y + float(cj.y) * two_h,
+                      az = pj.z + float(cj.z) * two_h;
+          const float dx = x.x - ax, dy = x.y - ay, dz = x.z - az;
+          const float q = sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx))) * P->i_h;
+          if (!(q < 2.0f)) continue;
+          const float ker = kW(P->knorm, q);
+          const float vol = A.idm[j].y / A.vel[j].w;
+          const Sym sj = ldsym(A.str_a, A.str_b, j);
+          s[0] = s[0] + sj.xx * ker * vol; s[1] = s[1] + sj.yy * ker * vol; s[2] = s[2] + sj.zz * ker * vol;
+          s[3] = s[3] + sj.xy * ker * vol; s[4] = s[4] + sj.xz * ker * vol; s[5] = s[5] + sj.yz * ker * vol;
+          wsum = wsum + ker * vol;
+          hits++;
+        }
+      }
+  if (wsum != 0.f)
+    for (int q = 0; q < 6; q++) s[q] = s[q] / wsum;
+  out[2 * k] = make_float4(s[0], s[1], s[2], s[3]);
+  out[2 * k + 1] = make_float4(s[4], s[5], wsum, float(hits));
+}
+
+// GetSurface (utilities.cu:26-91) and GetSurfaceAtPoint (:371-427), at any
+// points (x, y): the largest absolute z of the soil slots near the vertical
+// line through the point, -1e10 when there is none.  radius > 0: horizontal
+// distance sqrt(dx^2 + dy^2) < radius over the cells x +-1, y +-1, every z
+// (GetSurface's test, with radius 0.5 dr there); radius < 0: |dx| < -radius
+// over the point's own y row, x +-1, every z (GetSurfaceAtPoint's, with
+// 2h).  Positions are absolute (relative + cell 2h, FP32).
+__global__ void __launch_bounds__(64) k_surface_heights(const DevParams* __restrict__ P, Arrays A,
+                                                        const int* __restrict__ cbeg, const int* __restrict__ cend,
+                                                        const float2* __restrict__ pts, int n, float radius,
+                                                        float* __restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const float2 x = pts[k];
+  const float scale = 0.5f * P->i_h, two_h = P->two_h;
+  const int cx0 = int(floorf(x.x * scale)), cy0 = int(floorf(x.y * scale));
+  const bool strip = radius < 0.f;
+  const int y0 = strip ? cy0 : cy0 - 1, y1 = strip ? cy0 : cy0 + 1;
+  float top = -1.0e10f;
+  for (int cz = 0; cz < P->cnz; cz++)
+    for (int cy = y0; cy <= y1; cy++)
+      for (int cx = cx0 - 1; cx <= cx0 + 1; cx++) {
+        if (!in_grid(P, cx, cy, cz)) continue;
+        const int c = grid_key(P, cx, cy, cz) + P->ncell_material;
+        const int b = cbeg[c], e = cend[c];
+        for (int j = b; j < e; j++) {
+          const int4 cj = A.cellpos[j];
+          const float4 pj = A.pos[j];
+          const float dx = x.x - (pj.x + float(cj.x) * two_h);
+          const float dy = x.y - (pj.y + float(cj.y) * two_h);
+          const bool near = strip ? fabsf(dx) < -radius : sqrtf(fmaf(dy, dy, dx * dx)) < radius;
+          if (near) top = fmaxf(top, pj.z + float(cj.z) * two_h);
+        }
+      }
+  out[k] = top;
+}
+
 __global__ void k_fill_u64(unsigned long long* a, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = 0ull;
@@ -1011,6 +1099,15 @@ void launch_wall_profile(hipStream_t s, const DevParams* P, int np, int npb, Arr
   if (np > npb)
     hipLaunchKernelGGL(k_profile_mark, dim3(blocks(np - npb, 256)), dim3(256), 0, s, P, A, geom, nbins, last);
   hipLaunchKernelGGL(k_profile_write, dim3(blocks(nbins, 256)), dim3(256), 0, s, P, A, nbins, (const int*)last, out);
+}
+void launch_sensor_stress(hipStream_t s, const DevParams* P, Arrays A, const int* cb, const int* ce, const float4* pts,
+                          int n, int rows, float4* out) {
+  if (n > 0) hipLaunchKernelGGL(k_sensor_stress, dim3(blocks(n, 64)), dim3(64), 0, s, P, A, cb, ce, pts, n, rows, out);
+}
+void launch_surface_heights(hipStream_t s, const DevParams* P, Arrays A, const int* cb, const int* ce,
+                            const float2* pts, int n, float radius, float* out) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_surface_heights, dim3(blocks(n, 64)), dim3(64), 0, s, P, A, cb, ce, pts, n, radius, out);
 }
 void launch_wall_force(hipStream_t s, const DevParams* P, int np, int n_cases, Arrays A, const float* boxes,
                        unsigned long long* acc) {

@@ -185,6 +185,24 @@ class Context:
                   "lq_wall_pressure_profile_noalloc")
         return out
 
+    def sensor_stress(self, points, reference_rows=False):
+        """Shepard-weighted stress at absolute points (lq_sensor_stress):
+        f32[n][8] = (xx yy zz xy xz yz, sum W V, slots)."""
+        pts = np.ascontiguousarray(np.asarray(points, np.float32).reshape(-1, 3))
+        out = np.zeros((len(pts), 8), np.float32)
+        self._chk(self.lib.lq_sensor_stress(self.h, pts.ctypes.data, len(pts), 1 if reference_rows else 0,
+                                            out.ctypes.data, out.nbytes), "lq_sensor_stress")
+        return out
+
+    def surface_heights(self, points, radius):
+        """Largest absolute soil z near the vertical lines through (x, y)
+        (lq_surface_heights); -1e10 where none."""
+        pts = np.ascontiguousarray(np.asarray(points, np.float32).reshape(-1, 2))
+        out = np.zeros(len(pts), np.float32)
+        self._chk(self.lib.lq_surface_heights(self.h, pts.ctypes.data, len(pts), C.c_float(radius),
+                                              out.ctypes.data, out.nbytes), "lq_surface_heights")
+        return out
+
     def positions_absolute(self):
         out = np.empty((self.params.np, 4), np.float32)
         self._chk(self.lib.lq_positions_absolute(self.h, out.ctypes.data, out.nbytes), "lq_positions_absolute")
