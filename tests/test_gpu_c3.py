@@ -162,3 +162,51 @@ def test_c3_interaction_variants_and_oracle(c3, oracle_mod):
             say(f"  {label} {k} COLUMNS relerr vs FP32 oracle {e:.2e}")
             assert e <= TOL_F64_RATE, (label, k)  # other summation order and fast pair terms
         np.testing.assert_array_equal(outs["col"]["cvar"][rows][:, 2], o32["cvar"][rows][:, 2])  # slip flags
+
+
+@pytest.mark.timeout(900)
+def test_c3_whole_step(c3, oracle_mod):
+    """One whole graph step at full size (34 M slots, above 2^25), default
+    interaction (NEIGHBOR_LIST), against the FP32 and FP64 oracles' step
+    from the same state: cell frames and order exact, state fields to the
+    per-step tolerances of the small cases; against FP64 beyond 4h of every
+    contact-wall particle (within 2h of a contact wall the contact force
+    m dv / dt amplifies FP32 rounding of the velocity by 1/dt and the slip
+    decisions can flip; the corrector's interaction carries those velocities
+    one 2h shell further)."""
+    from test_gpu_parity import TOL_F32_STATE, TOL_F64_STATE
+    ctx = sim.open_case(c3)
+    ctx.step(1, graph=True)
+    ctx.sync()
+    say("[c3 step] oracles from the GPU state")
+    o32 = oracle_from(ctx, oracle_mod, np.float32, c3)
+    o64 = oracle_from(ctx, oracle_mod, np.float64, c3)
+    oracle_mod.set_surcharge(*c3.surcharge)
+    try:
+        assert o32.run(1)["ns2_identity"]
+        o64.run(1)
+    finally:
+        oracle_mod.set_surcharge(-1, 0.0)
+    ctx.step(1, graph=True)
+    ctx.sync()
+    from scipy.spatial import cKDTree
+    g = ctx.snapshot(["pos", "vel", "str", "codes", "cellpos", "orig_id"])
+    np.testing.assert_array_equal(g["cellpos"], o32["cellpos"])
+    np.testing.assert_array_equal(g["orig_id"], o32["orig_id"])
+    soil = slice(c3.npb, c3.np)
+    pa = ctx.positions_absolute()[:, :3].astype(np.float64)
+    walls = np.nonzero(g["codes"][:c3.npb, 0] >= 10)[0]
+    dist, _ = cKDTree(pa[walls]).query(pa[soil], distance_upper_bound=8.0 * float(ctx.params.h))
+    far = np.nonzero(dist > 4.0 * float(ctx.params.h))[0] + c3.npb
+    say(f"  {len(far)} of {c3.npm} soil slots beyond 4h of every contact-wall particle")
+    assert len(far) > 0.95 * c3.npm
+    for k in ("pos", "vel", "str"):
+        e32 = relerr(g[k][soil], o32[k][soil])
+        e64 = relerr(g[k][far], o64[k][far])
+        band = relerr(g[k][soil], o64[k][soil])
+        say(f"  {k} relerr vs FP32 oracle {e32:.2e}; vs FP64 oracle {e64:.2e} (all soil {band:.2e})")
+        assert e32 <= TOL_F32_STATE and e64 <= TOL_F64_STATE, k
+    e = relerr(g["vel"][far, 3], o64["vel"][far, 3])
+    say(f"  density relerr vs FP64 oracle {e:.2e}")
+    assert e <= TOL_F64_STATE
+    ctx.close()
