@@ -62,6 +62,32 @@ def test_cli_wrong_count_exits(tmp_path):
     assert r.returncode != 0 and "Wrong number of particles" in r.stdout
 
 
+@pytest.mark.parametrize("damage,message", [
+    # a type attribute outside int/scalar/model_mat/dimension/bool (loquat_io.cu:120-123)
+    (lambda b: _edit(b + ".xml", '<damping type="scalar">', '<damping type="float">'), "Wrong data type for damping"),
+    # an unparseable parameter file (loquat_io.cu:84-88)
+    (lambda b: _edit(b + ".xml", "</damping>", "</dampin>"), "Cannot load the parameter file"),
+    # no particle file, no motion file (loquat_io.cu:260-264, 307-310)
+    (lambda b: os.remove(b + ".dat"), "wall.dat !"),
+    (lambda b: os.remove(b + ".motion"), "wall.motion !"),
+])
+def test_cli_bad_inputs_exit(tmp_path, damage, message):
+    """The reference's hard exits on bad input files, with its messages."""
+    if not os.path.exists(CLI):
+        pytest.skip("CLI not built")
+    base = caseio.write_case(cases.config("tiny"), str(tmp_path), "wall")
+    damage(base)
+    r = subprocess.run([CLI, base], capture_output=True, text=True, cwd=str(tmp_path), timeout=60)
+    assert r.returncode != 0
+    assert message in r.stdout, r.stdout[-500:]
+
+
+def _edit(path, old, new):
+    txt = open(path).read()
+    assert old in txt
+    open(path, "w").write(txt.replace(old, new, 1))
+
+
 @pytest.mark.gpu
 def test_cli_runs_case_and_matches_api(tmp_path):
     """`loquat <case>` on GenCase-format files matches the API on the same case."""
