@@ -394,6 +394,19 @@ int lq_set_tile(lq_ctx* ctx, int shape, int tz);
  * context accepts REFERENCE_ORDER and NEIGHBOR_LIST only.  No reference
  * counterpart. */
 int lq_set_interaction(lq_ctx* ctx, int variant, int xcd_remap);
+/* Register build of the COLUMNS interaction kernel: 0 the 96-VGPR build
+ * (5 waves per SIMD), 1 the 128-VGPR one (4 waves per SIMD).  Both give
+ * bit-identical results; lq_autotune picks one for a context set to
+ * COLUMNS; the default is 0.  No reference counterpart. */
+int lq_set_columns_build(lq_ctx* ctx, int build);
+/* COLUMNS steps: run the corner search (cuFindCornerParticleMaterial,
+ * adami_boundary.cu:209-292) on a second stream beside the column build and
+ * the interaction, which then leaves the wall contact of particles whose cell
+ * neighbourhood holds a wall particle to a pass after the search (on by
+ * default; fused steps do not use it).  Results are bit-identical either
+ * way.  The corner class is then not timed on its own.  No reference
+ * counterpart. */
+int lq_set_overlap(lq_ctx* ctx, int on);
 /* Fused steps of the COLUMNS variant (off by default): lq_step then advances
  * every soil particle whose contact force needs no redistribution (pass B)
  * through cuPredictorMaterial / cuCorrectorMaterial (predictor_corrector.cu

@@ -127,7 +127,7 @@ EXPORTS = [
     "lq_sort_arrays_primary", "lq_sort_arrays_all", "lq_rigid_body_update", "lq_adami_boundary",
     "lq_find_corner_particle", "lq_stress_regularization", "lq_particle_interaction",
     "lq_predictor", "lq_corrector", "lq_step", "lq_wall_force", "lq_wall_pressure_profile", "lq_wall_pressure_profile_noalloc", "lq_set_surcharge", "lq_positions_absolute",
-    "lq_get_counts", "lq_synchronize", "lq_stress_saving_regularization", "lq_stream_copy", "lq_enable_timing", "lq_enable_timing_classes", "lq_kernel_times", "lq_neighbor_lists", "lq_set_tile", "lq_set_interaction", "lq_set_fusion", "lq_sensor_stress", "lq_surface_heights", "lq_set_pair_list", "lq_pair_list_status", "lq_autotune", "lq_interaction_cutoff_sq",
+    "lq_get_counts", "lq_synchronize", "lq_stress_saving_regularization", "lq_stream_copy", "lq_enable_timing", "lq_enable_timing_classes", "lq_kernel_times", "lq_neighbor_lists", "lq_set_tile", "lq_set_interaction", "lq_set_fusion", "lq_set_overlap", "lq_set_columns_build", "lq_sensor_stress", "lq_surface_heights", "lq_set_pair_list", "lq_pair_list_status", "lq_autotune", "lq_interaction_cutoff_sq",
     "lq_create_batch", "lq_case_count", "lq_upload_case", "lq_set_case_motion_tables", "lq_get_case_field",
     "lq_case_positions_absolute", "lq_case_counts", "lq_wall_force_cases",
     "lq_mlp_normalisation", "lq_mlp_train", "lq_mlp_loss_grad", "lq_mlp_forward", "lq_mlp_last_error",
@@ -166,6 +166,8 @@ def _proto(lib):
     lib.lq_wall_pressure_profile.argtypes = [vp, P(C.c_float), C.c_int, vp, C.c_size_t]
     lib.lq_wall_pressure_profile_noalloc.argtypes = [vp, P(C.c_float), C.c_int, vp, C.c_size_t]
     lib.lq_set_fusion.argtypes = [vp, C.c_int]
+    lib.lq_set_overlap.argtypes = [vp, C.c_int]
+    lib.lq_set_columns_build.argtypes = [vp, C.c_int]
     lib.lq_sensor_stress.argtypes = [vp, vp, C.c_int, C.c_int, vp, C.c_size_t]
     lib.lq_surface_heights.argtypes = [vp, vp, C.c_int, C.c_float, vp, C.c_size_t]
     lib.lq_set_surcharge.argtypes = [vp, C.c_int, C.c_float]

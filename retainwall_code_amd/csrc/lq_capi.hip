@@ -22,6 +22,9 @@ struct TimedPair {
 struct lq_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;  // COLUMNS steps: the corner search beside the interaction
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  int overlap = 1;             // use the side stream (lq_set_overlap)
   lq_params par{};
   DevParams hp{};          // host image of the static part
   DevParams* dP = nullptr;
@@ -496,6 +499,10 @@ static int create_impl(const lq_params* cases, int n_cases, int device, lq_ctx**
   if (e != hipSuccess) { c->err = std::string("hipSetDevice: ") + hipGetErrorString(e); return bail(LQ_ERR_HIP); }
   e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { c->err = std::string("hipStreamCreate: ") + hipGetErrorString(e); return bail(LQ_ERR_HIP); }
+  e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);  // default priority, as the main stream
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
+  if (e != hipSuccess) { c->err = std::string("side stream: ") + hipGetErrorString(e); return bail(LQ_ERR_HIP); }
   {
     void* sp = nullptr;  // resolve this device's profile scratch once, device current
     e = profile_scratch(device, &sp);
@@ -629,6 +636,9 @@ extern "C" void lq_destroy(lq_ctx* c) {
   for (auto e : c->pool) hipEventDestroy(e);
   for (void* p : c->allocs) hipFree(p);
   if (c->wf_host) hipHostFree(c->wf_host);
+  if (c->ev_fork) hipEventDestroy(c->ev_fork);
+  if (c->ev_join) hipEventDestroy(c->ev_join);
+  if (c->side) hipStreamDestroy(c->side);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1158,7 +1168,10 @@ static void enq_plist_build(lq_ctx* c) {
 // stage 0: first interaction after a sort (a pair-list variant builds its
 // lists); stage 1: the corrector-stage call of the same step (lists reused
 // when still exact).
-static int enq_interaction(lq_ctx* c, int stage) {
+// beside: the corner search was enqueued on the side stream (enq_corner_beside,
+// COLUMNS only); the interaction leaves the wall contact of particles near a
+// wall for k_contact_after, which waits for the search.
+static int enq_interaction(lq_ctx* c, int stage, bool beside = false) {
   const bool build = stage == 0 && c->par.interaction_variant == LQ_INTERACTION_PAIR_LIST;
   launch_reset_deferred(c->stream, c->dP, build ? 1 : 0);
   // COLUMNS writes its records in aux order while it indexes the sub-cells
@@ -1170,10 +1183,19 @@ static int enq_interaction(lq_ctx* c, int stage) {
   }
   {
     Section s(c, LQ_K_INTERACTION);
-    enq_interaction_kernel(c);
+    if (beside) {
+      launch_col_index(c->stream, c->dP, c->par.cell_num[0], c->par.cell_num[1], c->B.cur, c->B);
+      launch_interaction_col(c->stream, c->dP, c->npm0, c->B.cur, c->B, c->xcd_remap, c->col_wide, 3);
+    } else {
+      enq_interaction_kernel(c);
+    }
   }
   {
     Section s(c, LQ_K_CONTACT);
+    if (beside) {
+      hipStreamWaitEvent(c->stream, c->ev_join, 0);
+      launch_contact_after(c->stream, c->dP, c->npm0, c->B.cur, c->B);
+    }
     launch_contact(c->stream, c->dP, c->npm0, c->B.cur, c->B, c->B.cell_beg, c->B.cell_end);
   }
   return 0;
@@ -1249,6 +1271,21 @@ static void enq_fused(lq_ctx* c, int stage) {
   }
 }
 
+// COLUMNS steps: the corner search (cuFindCornerParticleMaterial) on the side
+// stream, so that it runs beside the column build and the interaction (it is
+// latency-bound on few waves); enq_interaction(..., true) joins it before
+// the wall contact.  Not timed as a class of its own.
+static bool corner_beside(const lq_ctx* c) {
+  return c->overlap && c->side && c->par.interaction_variant == LQ_INTERACTION_COLUMNS && !(c->fuse && c->B.fl_pos);
+}
+static void enq_corner_beside(lq_ctx* c) {
+  hipEventRecord(c->ev_fork, c->stream);
+  hipStreamWaitEvent(c->side, c->ev_fork, 0);
+  if (c->par.npb > 0)
+    launch_corner(c->side, c->dP, c->npm0, c->B.cur, c->B.cell_beg, c->B.cell_end, c->B.wall_near);
+  hipEventRecord(c->ev_join, c->side);
+}
+
 // One step (loquat.cu:366-436, 548-560).  The corrector-section neighbour
 // search and SortArraysAll are skipped: the predictor never re-bins a
 // particle and no key changes between the two searches, so the second
@@ -1259,21 +1296,24 @@ static int enq_step(lq_ctx* c) {
   enq_ns(c);
   enq_sort_primary(c);
   if (c->par.n_motion > 0) launch_rigid_body_update(c->stream, c->dP);
+  const bool beside = corner_beside(c);
   enq_adami(c);
-  enq_corner(c);
+  if (beside) enq_corner_beside(c);
+  else enq_corner(c);
   if (shep > 0) enq_shepard(c, true);  // runs when due at the device's step count
   if (fused(c)) {
     enq_fused(c, 0);
   } else {
-    enq_interaction(c, 0);
+    enq_interaction(c, 0, beside);
     enq_predictor(c);
   }
   enq_adami(c);
-  enq_corner(c);
+  if (beside) enq_corner_beside(c);
+  else enq_corner(c);
   if (fused(c)) {
     enq_fused(c, 1);
   } else {
-    enq_interaction(c, 1);
+    enq_interaction(c, 1, beside);
     enq_corrector(c);
   }
   launch_advance(c->stream, c->dP);
@@ -1785,6 +1825,23 @@ extern "C" int lq_set_interaction(lq_ctx* c, int variant, int xcd_remap) {
     return fail(c, LQ_ERR_ARG, "lq_set_interaction: periodic boundaries need REFERENCE_ORDER or NEIGHBOR_LIST");
   c->par.interaction_variant = variant;
   c->xcd_remap = xcd_remap ? 1 : 0;
+  c->layout_version++;  // re-capture graphs
+  return LQ_OK;
+}
+
+// COLUMNS register build: 0 the 96-VGPR kernel, 1 the 128-VGPR one.
+extern "C" int lq_set_columns_build(lq_ctx* c, int build) {
+  if (!c) return LQ_ERR_ARG;
+  if (build < 0 || build > 1) return fail(c, LQ_ERR_ARG, "lq_set_columns_build: build 0 or 1");
+  c->col_wide = build;
+  c->layout_version++;  // re-capture graphs
+  return LQ_OK;
+}
+
+// COLUMNS steps: the corner search beside the interaction on or off (on by default).
+extern "C" int lq_set_overlap(lq_ctx* c, int on) {
+  if (!c) return LQ_ERR_ARG;
+  c->overlap = on ? 1 : 0;
   c->layout_version++;  // re-capture graphs
   return LQ_OK;
 }

@@ -309,7 +309,11 @@ __device__ __forceinline__ void pair_terms_fast(const PairConst& P, float i_kdr,
 // aux order, one plane per float4, so that 8 lanes share a 128-byte line)
 // lie close together.  Hits
 // go to per-lane FIFO rings in LDS as (aux index << 6 | cell-offset code) and
-// are evaluated in ring order, as in the neighbour-list variant.
+// are evaluated in ring order, as in the neighbour-list variant; the rings
+// are emptied at the end of every window (and earlier when one would
+// overflow), so a window's records are summed while they are still in the
+// caches.  Draining earlier or later changes no sum: each lane's hits are
+// evaluated in the order they were found.
 constexpr int CL_CAP = 32;    // ring entries per lane (power of two)
 constexpr int CL_CHUNK = 32;  // candidates per hit mask
 
@@ -339,7 +343,10 @@ __device__ __forceinline__ void cl_drain(const PairConst& K, float i_kdr, IAcc& 
 // gives it 128; lq_autotune times both.  FUSE 1 / 2: a particle whose
 // contact force needs no pass B goes on through cuPredictorMaterial /
 // cuCorrectorMaterial (predictor_slot / corrector_slot) right after its
-// pair sum.  Nothing in this kernel reads another particle's state arrays
+// pair sum.  FUSE 3: the kernel reads nothing the corner search writes
+// (the slip flag it reads, cvar.z, is written back unchanged there), so it
+// can run beside that search; the wall contact of particles near a wall
+// is then finished by k_contact_after.  Nothing in this kernel reads another particle's state arrays
 // (neighbours come from aux and rec), so those updates race with nothing;
 // a first-layer particle first leaves its position and cell in fl_pos /
 // fl_cell for pass B.
@@ -421,11 +428,23 @@ __global__ void __launch_bounds__(64, MINW) k_interaction_col(DevParams* __restr
       }
       j += m;
     }
+    // drain at the window's end, while its records are still near in the caches
+    cl_drain(K, i_kdr, S, B, s_list, lane, head, cnt, pi, two_h, i_h, vi, rhoi, sri, artpi, shift);
+    head = (head + cnt) & (CL_CAP - 1);
+    cnt = 0;
   }
-  cl_drain(K, i_kdr, S, B, s_list, lane, head, cnt, pi, two_h, i_h, vi, rhoi, sri, artpi, shift);
   if (!active) return;
+  if constexpr (FUSE == 3) {
+    // contact after the corner search: a particle whose cell has a wall
+    // particle within its 27 cells keeps its sums for k_contact_after; any
+    // other has no contact wall within 2h (the search will find none)
+    const float3 s3 = interaction_rates(B, i, S);
+    if (B.wall_near[grid_key(P, ci.x, ci.y, ci.z)]) B.s3[i] = make_float4(s3.x, s3.y, s3.z, S.accden);
+    else interaction_free(P, A, B, i, s3, S.accden, vi);
+    return;
+  }
   const bool deferred = interaction_epilogue(P, A, B, i, S, vi);
-  if constexpr (FUSE != 0) {
+  if constexpr (FUSE == 1 || FUSE == 2) {
     if (A.cvar[i].y <= P->band) {
       B.fl_pos[i] = A.pos[i];
       B.fl_cell[i] = ci;
@@ -490,6 +509,7 @@ void launch_interaction_col(hipStream_t s, DevParams* P, int npm_max, Arrays A, 
   if (npm_max <= 0) return;
   if (fuse == 1) launch_col<1>(s, P, npm_max, A, B, xcd_remap, wide);
   else if (fuse == 2) launch_col<2>(s, P, npm_max, A, B, xcd_remap, wide);
+  else if (fuse == 3) launch_col<3>(s, P, npm_max, A, B, xcd_remap, wide);
   else launch_col<0>(s, P, npm_max, A, B, xcd_remap, wide);
 }
 void launch_neighbor_list_col(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const Buffers& B, int max_nb,

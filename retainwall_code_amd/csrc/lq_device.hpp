@@ -166,12 +166,30 @@ __device__ __forceinline__ void pair_terms(const PairConst& P, IAcc& S, float3 d
 }
 
 // Epilogue of cuParticleInteractionMaterial (particle_interaction.cu:284-295)
-// with pass A of the wall contact (LoopContact, :174-237).
-// Returns whether the particle's contact force waits for pass B.
-__device__ __forceinline__ bool interaction_epilogue(DevParams* __restrict__ P, Arrays A, const Buffers& B, int i,
-                                                     const IAcc& S, float3 vi) {
+// with pass A of the wall contact (LoopContact, :174-237), in two parts.
+// interaction_rates writes the strain rate and spin and returns the sum of
+// the pair forces; interaction_contact adds damping, gravity and the pass-A
+// contact force of the corner search's nearest wall and writes the
+// acceleration (and density rate accden).  It returns whether the
+// particle's contact force waits for pass B.
+__device__ __forceinline__ float3 interaction_rates(const Buffers& B, int i, const IAcc& S) {
+  B.acestn_a[i] = make_float4(S.gv[0], S.gv[4], S.gv[8], 0.5f * (S.gv[1] + S.gv[3]));
+  B.acestn_b[i] = make_float2(0.5f * (S.gv[2] + S.gv[6]), 0.5f * (S.gv[5] + S.gv[7]));
+  B.spin[i] = make_float4(0.5f * (S.gv[3] - S.gv[1]), 0.5f * (S.gv[6] - S.gv[2]), 0.5f * (S.gv[7] - S.gv[5]), 0.f);
+  return f3add(f3add(S.as, S.av), S.aa);
+}
+// The acceleration of a particle with no contact wall within 2h (what
+// interaction_contact gives when the corner search found none).
+__device__ __forceinline__ void interaction_free(DevParams* __restrict__ P, Arrays A, const Buffers& B, int i,
+                                                 float3 s3, float accden, float3 vi) {
+  const float3 damp = f3mul(vi, P->damping);
+  const float3 g = body_acc(P, A.codes, i, A.idm[i].y);
+  const float3 acc = f3add(f3sub(f3add(s3, make_float3(0.f, 0.f, 0.f)), damp), g);
+  B.accad[i] = make_float4(acc.x, acc.y, acc.z, accden);
+}
+__device__ __forceinline__ bool interaction_contact(DevParams* __restrict__ P, Arrays A, const Buffers& B, int i,
+                                                    float3 s3, float accden, float3 vi) {
   float3 damp = f3mul(vi, P->damping);
-  float3 s3 = f3add(f3add(S.as, S.av), S.aa);
   float mi = A.idm[i].y;
   const float3 g = body_acc(P, A.codes, i, mi);
   float3 acc0 = f3add(f3sub(s3, damp), g);
@@ -206,11 +224,13 @@ __device__ __forceinline__ bool interaction_epilogue(DevParams* __restrict__ P, 
     B.s3[i] = make_float4(s3.x, s3.y, s3.z, mi);
   }
   float3 acc = f3add(f3sub(f3add(s3, ast), damp), g);
-  B.accad[i] = make_float4(acc.x, acc.y, acc.z, S.accden);
-  B.acestn_a[i] = make_float4(S.gv[0], S.gv[4], S.gv[8], 0.5f * (S.gv[1] + S.gv[3]));
-  B.acestn_b[i] = make_float2(0.5f * (S.gv[2] + S.gv[6]), 0.5f * (S.gv[5] + S.gv[7]));
-  B.spin[i] = make_float4(0.5f * (S.gv[3] - S.gv[1]), 0.5f * (S.gv[6] - S.gv[2]), 0.5f * (S.gv[7] - S.gv[5]), 0.f);
+  B.accad[i] = make_float4(acc.x, acc.y, acc.z, accden);
   return deferred;
+}
+__device__ __forceinline__ bool interaction_epilogue(DevParams* __restrict__ P, Arrays A, const Buffers& B, int i,
+                                                     const IAcc& S, float3 vi) {
+  const float3 s3 = interaction_rates(B, i, S);
+  return interaction_contact(P, A, B, i, s3, S.accden, vi);
 }
 
 // ------------------------------------------------------------ HypoCS ----

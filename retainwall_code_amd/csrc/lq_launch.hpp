@@ -31,6 +31,8 @@ void launch_interaction(hipStream_t s, DevParams* P, int npm_max, Arrays A, cons
 // reads first-layer positions and cells from B.fl_pos / B.fl_cell
 void launch_contact(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const Buffers& B, const int* cb,
                     const int* ce, bool fused = false);
+// pass A of the wall contact left by a fuse-3 COLUMNS interaction
+void launch_contact_after(hipStream_t s, DevParams* P, int npm_max, Arrays A, const Buffers& B);
 void launch_predictor_finish(hipStream_t s, DevParams* P, Arrays A, const Buffers& B);
 void launch_corrector_finish(hipStream_t s, DevParams* P, int npb, int n_motion, Arrays A, const Buffers& B);
 void launch_predictor(hipStream_t s, DevParams* P, int np, Arrays A, const Buffers& B, int track_disp);
@@ -52,7 +54,8 @@ void launch_col_index(hipStream_t s, const DevParams* P, int cnx, int cny, Array
 // wide != 0: the 128-VGPR build (4 waves per SIMD) instead of the 96-VGPR one
 // fuse: 0 interaction only; 1 with the predictor (cuPredictorMaterial) and
 // 2 with the corrector (cuCorrectorMaterial) of every soil particle whose
-// contact force needs no pass B
+// contact force needs no pass B; 3 without the wall contact of particles
+// near a wall (launch_contact_after finishes it after the corner search)
 void launch_interaction_col(hipStream_t s, DevParams* P, int npm_max, Arrays A, const Buffers& B,
                             int xcd_remap, int wide, int fuse = 0);
 void launch_neighbor_list_col(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const Buffers& B, int max_nb,

@@ -710,6 +710,20 @@ __global__ void __launch_bounds__(256) k_contact_redistribute(const DevParams* _
   }
 }
 
+// Pass A of the wall contact (particle_interaction.cu:174-237) for the
+// particles a fuse-3 COLUMNS interaction left for after the corner search
+// (their cell has a wall particle within its 27 cells): B.s3 holds their
+// pair-force sum and density rate.
+__global__ void __launch_bounds__(256) k_contact_after(DevParams* __restrict__ P, Arrays A, Buffers B) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= P->npm) return;
+  const int i = P->npb + idx;
+  const int4 ci = A.cellpos[i];
+  if (!B.wall_near[grid_key(P, ci.x, ci.y, ci.z)]) return;
+  const float4 t = B.s3[i];
+  interaction_contact(P, A, B, i, make_float3(t.x, t.y, t.z), t.w, ld3(A.vel[i]));
+}
+
 // Pass C: publish pass-B forces and the min-reduced K.
 __global__ void k_contact_commit(const DevParams* __restrict__ P, Arrays A, Buffers B) {
   const int nd = P->n_deferred;
@@ -1075,6 +1089,9 @@ void launch_contact(hipStream_t s, const DevParams* P, int npm_max, Arrays A, co
                      ce, fused ? (const float4*)B.fl_pos : (const float4*)A.pos,
                      fused ? (const int4*)B.fl_cell : (const int4*)A.cellpos);
   hipLaunchKernelGGL(k_contact_commit, dim3(g), dim3(256), 0, s, P, A, B);
+}
+void launch_contact_after(hipStream_t s, DevParams* P, int npm_max, Arrays A, const Buffers& B) {
+  if (npm_max > 0) hipLaunchKernelGGL(k_contact_after, dim3(blocks(npm_max, 256)), dim3(256), 0, s, P, A, B);
 }
 void launch_predictor(hipStream_t s, DevParams* P, int np, Arrays A, const Buffers& B, int track_disp) {
   if (np > 0) hipLaunchKernelGGL(k_predictor, dim3(blocks(np, 256)), dim3(256), 0, s, P, A, B, track_disp);

@@ -236,6 +236,14 @@ class Context:
     def set_interaction(self, variant, xcd_remap=1):
         self._chk(self.lib.lq_set_interaction(self.h, variant, xcd_remap), "lq_set_interaction")
 
+    def set_columns_build(self, build):
+        """COLUMNS register build (lq_set_columns_build): 0 96 VGPRs, 1 128 VGPRs."""
+        self._chk(self.lib.lq_set_columns_build(self.h, int(build)), "lq_set_columns_build")
+
+    def set_overlap(self, on=True):
+        """COLUMNS steps: corner search beside the interaction (lq_set_overlap)."""
+        self._chk(self.lib.lq_set_overlap(self.h, 1 if on else 0), "lq_set_overlap")
+
     def set_fusion(self, on=True):
         """Fused COLUMNS steps (lq_set_fusion)."""
         self._chk(self.lib.lq_set_fusion(self.h, 1 if on else 0), "lq_set_fusion")
