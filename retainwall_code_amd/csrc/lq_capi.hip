@@ -23,7 +23,7 @@ struct lq_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipStream_t side = nullptr;  // COLUMNS steps: the corner search beside the interaction
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_adami = nullptr, ev_join = nullptr;
   int overlap = 1;             // use the side stream (lq_set_overlap)
   lq_params par{};
   DevParams hp{};          // host image of the static part
@@ -501,6 +501,7 @@ static int create_impl(const lq_params* cases, int n_cases, int device, lq_ctx**
   if (e != hipSuccess) { c->err = std::string("hipStreamCreate: ") + hipGetErrorString(e); return bail(LQ_ERR_HIP); }
   e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);  // default priority, as the main stream
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_adami, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
   if (e != hipSuccess) { c->err = std::string("side stream: ") + hipGetErrorString(e); return bail(LQ_ERR_HIP); }
   {
@@ -555,6 +556,7 @@ static int create_impl(const lq_params* cases, int n_cases, int device, lq_ctx**
   if (cols_supported(par)) {
     st |= dalloc(c, &B.aux, n);
     st |= dalloc(c, &B.soil_aux, n);
+    st |= dalloc(c, &B.wall_aux, size_t(std::max(1, par.npb)));
     const size_t ncol = size_t(par.cell_num[0]) * size_t(par.cell_num[1]);
     st |= dalloc(c, &B.sub_start, ncol * size_t(8 * par.cell_num[2] + 1));
     st |= dalloc(c, &B.colcnt, ncol);
@@ -637,6 +639,7 @@ extern "C" void lq_destroy(lq_ctx* c) {
   for (void* p : c->allocs) hipFree(p);
   if (c->wf_host) hipHostFree(c->wf_host);
   if (c->ev_fork) hipEventDestroy(c->ev_fork);
+  if (c->ev_adami) hipEventDestroy(c->ev_adami);
   if (c->ev_join) hipEventDestroy(c->ev_join);
   if (c->side) hipStreamDestroy(c->side);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -1183,8 +1186,10 @@ static int enq_interaction(lq_ctx* c, int stage, bool beside = false) {
   }
   {
     Section s(c, LQ_K_INTERACTION);
-    if (beside) {
-      launch_col_index(c->stream, c->dP, c->par.cell_num[0], c->par.cell_num[1], c->B.cur, c->B);
+    if (beside) {  // the index and soil records beside Adami, the wall records after it
+      launch_col_index(c->stream, c->dP, c->par.cell_num[0], c->par.cell_num[1], c->B.cur, c->B, 0);
+      hipStreamWaitEvent(c->stream, c->ev_adami, 0);
+      launch_col_wall_records(c->stream, c->dP, c->par.npb, c->B.cur, c->B);
       launch_interaction_col(c->stream, c->dP, c->npm0, c->B.cur, c->B, c->xcd_remap, c->col_wide, 3);
     } else {
       enq_interaction_kernel(c);
@@ -1271,16 +1276,20 @@ static void enq_fused(lq_ctx* c, int stage) {
   }
 }
 
-// COLUMNS steps: the corner search (cuFindCornerParticleMaterial) on the side
-// stream, so that it runs beside the column build and the interaction (it is
-// latency-bound on few waves); enq_interaction(..., true) joins it before
-// the wall contact.  Not timed as a class of its own.
+// COLUMNS steps: Adami (cuAdamiBoundaryMaterial) and the corner search
+// (cuFindCornerParticleMaterial) on the side stream, so that they run beside
+// the column build and the interaction (both are latency-bound on few
+// waves); enq_interaction(..., true) waits for Adami before it writes the
+// walls' pair records and for the search before the wall contact.  Neither
+// is timed as a class of its own then.
 static bool corner_beside(const lq_ctx* c) {
   return c->overlap && c->side && c->par.interaction_variant == LQ_INTERACTION_COLUMNS && !(c->fuse && c->B.fl_pos);
 }
-static void enq_corner_beside(lq_ctx* c) {
+static void enq_walls_beside(lq_ctx* c) {
   hipEventRecord(c->ev_fork, c->stream);
   hipStreamWaitEvent(c->side, c->ev_fork, 0);
+  if (c->npm0 > 0) launch_adami(c->side, c->dP, c->par.npb, c->B.cur, c->B.cell_beg, c->B.cell_end);
+  hipEventRecord(c->ev_adami, c->side);
   if (c->par.npb > 0)
     launch_corner(c->side, c->dP, c->npm0, c->B.cur, c->B.cell_beg, c->B.cell_end, c->B.wall_near);
   hipEventRecord(c->ev_join, c->side);
@@ -1297,19 +1306,28 @@ static int enq_step(lq_ctx* c) {
   enq_sort_primary(c);
   if (c->par.n_motion > 0) launch_rigid_body_update(c->stream, c->dP);
   const bool beside = corner_beside(c);
-  enq_adami(c);
-  if (beside) enq_corner_beside(c);
-  else enq_corner(c);
-  if (shep > 0) enq_shepard(c, true);  // runs when due at the device's step count
+  if (beside) {
+    enq_walls_beside(c);
+  } else {
+    enq_adami(c);
+    enq_corner(c);
+  }
+  if (shep > 0) {  // runs when due at the device's step count, after Adami has read the stresses it filters
+    if (beside) hipStreamWaitEvent(c->stream, c->ev_adami, 0);
+    enq_shepard(c, true);
+  }
   if (fused(c)) {
     enq_fused(c, 0);
   } else {
     enq_interaction(c, 0, beside);
     enq_predictor(c);
   }
-  enq_adami(c);
-  if (beside) enq_corner_beside(c);
-  else enq_corner(c);
+  if (beside) {
+    enq_walls_beside(c);
+  } else {
+    enq_adami(c);
+    enq_corner(c);
+  }
   if (fused(c)) {
     enq_fused(c, 1);
   } else {

@@ -94,12 +94,30 @@ __global__ void __launch_bounds__(1024) k_col_scan(const DevParams* __restrict__
 // walls), velocity and density, sigma/rho^2, m/rho and trace(sigma/rho^2)
 // (k_pair_prep's quantities).  soil_aux lists the aux positions of the soil
 // entries in aux order: the interaction's lanes.
+// The pair record of slot j at aux position k (four planes of rec_rows).
+__device__ __forceinline__ void col_write_record(const DevParams* __restrict__ P, Arrays A, const Buffers& B, int j,
+                                                 int k, bool soil) {
+  const float4 p = A.pos[j];
+  const float4 v = A.vel[j];
+  const float rho = v.w;
+  const float m = soil ? fabsf(p.w) : P->vol_def * rho;
+  const Sym sr = sdiv(ldsym(A.str_a, A.str_b, j), rho * rho);
+  const size_t rows = size_t(B.rec_rows);
+  B.rec[k] = make_float4(p.x, p.y, p.z, p.w < 0.f ? -m : m);
+  B.rec[rows + k] = v;
+  B.rec[2 * rows + k] = make_float4(sr.xx, sr.yy, sr.zz, sr.xy);
+  B.rec[3 * rows + k] = make_float4(sr.xz, sr.yz, m / rho, strace(sr));
+}
+
 constexpr int COL_MAX_CNZ = 256;
 constexpr int COL_MAX_BUCKETS = 8 * COL_MAX_CNZ + 1;
 constexpr int COL_LDS_PTS = 8192;  // particles of a column whose codes stay in LDS
 constexpr int COL_THREADS = 512;
+// wall_rec == 0: wall records are left to k_col_wall_records (which runs
+// after Adami has extrapolated the walls' velocity and stress); their aux
+// positions go to wall_aux.
 __global__ void __launch_bounds__(COL_THREADS) k_col_build(const DevParams* __restrict__ P, Arrays A, Buffers B,
-                                                           const int4* __restrict__ colinfo) {
+                                                           const int4* __restrict__ colinfo, int wall_rec) {
   __shared__ int cntw[COL_MAX_BUCKETS];
   __shared__ int start[COL_MAX_BUCKETS];
   __shared__ int sstart[COL_MAX_BUCKETS];  // exclusive scan of the soil counts alone
@@ -150,7 +168,6 @@ __global__ void __launch_bounds__(COL_THREADS) k_col_build(const DevParams* __re
   }
   __syncthreads();
   const int base = w0 + (s0 - npb);
-  const float vol_def = P->vol_def;
   for (int t = tid; t < n; t += COL_THREADS) {
     const bool soil = t >= nw;
     const int j = slot_of(t);
@@ -170,16 +187,16 @@ __global__ void __launch_bounds__(COL_THREADS) k_col_build(const DevParams* __re
     const int k = base + start[b] + (soil ? cntw[b] : 0) + rank;
     B.aux[k] = make_float4(p.x, p.y, p.z, __int_as_float((j << 2) | (cz & 3)));
     if (soil) B.soil_aux[(s0 - npb) + sstart[b] + rank] = k;
-    const float4 v = A.vel[j];
-    const float rho = v.w;
-    const float m = soil ? fabsf(p.w) : vol_def * rho;
-    const Sym sr = sdiv(ldsym(A.str_a, A.str_b, j), rho * rho);
-    const size_t rows = size_t(B.rec_rows);
-    B.rec[k] = make_float4(p.x, p.y, p.z, p.w < 0.f ? -m : m);
-    B.rec[rows + k] = v;
-    B.rec[2 * rows + k] = make_float4(sr.xx, sr.yy, sr.zz, sr.xy);
-    B.rec[3 * rows + k] = make_float4(sr.xz, sr.yz, m / rho, strace(sr));
+    if (soil || wall_rec) col_write_record(P, A, B, j, k, soil);
+    else B.wall_aux[j] = k;
   }
+}
+
+// The wall slots' pair records, at the aux positions k_col_build left.
+__global__ void __launch_bounds__(256) k_col_wall_records(const DevParams* __restrict__ P, Arrays A, Buffers B) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= P->npb) return;
+  col_write_record(P, A, B, j, B.wall_aux[j], false);
 }
 
 // ------------------------------------------------------ the window ----
@@ -490,12 +507,15 @@ __global__ void k_neighbor_list_col(const DevParams* __restrict__ P, Arrays A, B
 
 static inline int cblocks(long long n, int b) { return int((n + b - 1) / b); }
 
-void launch_col_index(hipStream_t s, const DevParams* P, int cnx, int cny, Arrays A, const Buffers& B) {
+void launch_col_index(hipStream_t s, const DevParams* P, int cnx, int cny, Arrays A, const Buffers& B, int wall_rec) {
   const int ncol = cnx * cny;
   hipLaunchKernelGGL(k_col_count, dim3(cblocks(ncol, 4)), dim3(256), 0, s, P, (const int*)B.cell_beg,
                      (const int*)B.cell_end, B.colcnt);
   hipLaunchKernelGGL(k_col_scan, dim3(1), dim3(1024), 0, s, P, (const int2*)B.colcnt, B.colinfo);
-  hipLaunchKernelGGL(k_col_build, dim3(ncol), dim3(COL_THREADS), 0, s, P, A, B, (const int4*)B.colinfo);
+  hipLaunchKernelGGL(k_col_build, dim3(ncol), dim3(COL_THREADS), 0, s, P, A, B, (const int4*)B.colinfo, wall_rec);
+}
+void launch_col_wall_records(hipStream_t s, const DevParams* P, int npb, Arrays A, const Buffers& B) {
+  if (npb > 0) hipLaunchKernelGGL(k_col_wall_records, dim3(cblocks(npb, 256)), dim3(256), 0, s, P, A, B);
 }
 // lanes over the soil aux entries (at most npm_max; the live count is P->npm)
 template <int FUSE>

@@ -148,6 +148,7 @@ struct Buffers {
   // variant rec is indexed by aux position, not by slot
   float4* aux;
   int* soil_aux;   // aux positions of the soil entries, in aux order
+  int* wall_aux;   // aux position of each wall slot (when its record is written after Adami)
   int* sub_start;
   float4* fl_pos;  // fused steps: position and cell of first-layer particles before their update
   int4* fl_cell;

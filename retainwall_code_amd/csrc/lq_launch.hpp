@@ -50,7 +50,10 @@ void launch_neighbor_list(hipStream_t s, const DevParams* P, int npm_max, Arrays
                           int max_nb, int* counts, int* ids);
 
 // COLUMNS interaction variant (lq_columns.hip)
-void launch_col_index(hipStream_t s, const DevParams* P, int cnx, int cny, Arrays A, const Buffers& B);
+// wall_rec == 0: the wall slots' records are left to launch_col_wall_records
+void launch_col_index(hipStream_t s, const DevParams* P, int cnx, int cny, Arrays A, const Buffers& B,
+                      int wall_rec = 1);
+void launch_col_wall_records(hipStream_t s, const DevParams* P, int npb, Arrays A, const Buffers& B);
 // wide != 0: the 128-VGPR build (4 waves per SIMD) instead of the 96-VGPR one
 // fuse: 0 interaction only; 1 with the predictor (cuPredictorMaterial) and
 // 2 with the corrector (cuCorrectorMaterial) of every soil particle whose

@@ -221,13 +221,14 @@ def test_columns_builds_bit_identical(name):
 
 
 
-@pytest.mark.parametrize("name", ["small", "c2"])
+@pytest.mark.parametrize("name", ["small", "c2", "shepard"])
 def test_columns_corner_beside_bit_identical(name):
-    """COLUMNS steps with the corner search on a second stream beside the
-    column build and the interaction (the wall contact of near-wall
-    particles finished after the search) equal the serial order bit for
-    bit, in graph replay and eager steps."""
-    case = cases.config(name)
+    """COLUMNS steps with Adami and the corner search on a second stream
+    beside the column build and the interaction (wall records written after
+    Adami, the wall contact of near-wall particles finished after the
+    search) equal the serial order bit for bit, in graph replay and eager
+    steps; also with the Shepard filter due every second step."""
+    case = cases.config("small", shepard=2) if name == "shepard" else cases.config(name)
     say(f"[corner beside] case={name}")
     a = sim.open_case(case, variant=COLS)
     b = sim.open_case(case, variant=COLS)
