@@ -442,7 +442,7 @@ def main():
                     help="enqueue every kernel per step instead of replaying the captured two-step hipGraph")
     ap.add_argument("--variant", default="auto",
                     choices=["auto", "tile", "reference", "neighbor_list", "pair_list", "columns"],
-                    help="interaction kernel variant (all bit-identical); auto times each and keeps the fastest")
+                    help="interaction kernel variant (columns sums each particle's pairs in another order; the others are bit-identical); auto times each and keeps the fastest")
     ap.add_argument("--remap", type=int, default=0, help="XCD block renumbering for reference/neighbor_list")
     ap.add_argument("--key", default="linear", choices=["linear", "morton"], help="cell-key function")
     ap.add_argument("--tile", default="", metavar="SHAPE,TZ", help="fix the tile shape/depth instead of autotuning")

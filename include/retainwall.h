@@ -362,7 +362,10 @@ int lq_synchronize(lq_ctx* ctx);
 
 /* ---- instrumentation (no reference counterpart) --------------------------
  * When enabled, lq_step and the piecewise calls record HIP events around
- * each kernel class on the context stream; lq_kernel_times returns the
+ * each kernel class on the stream it runs on (Adami and the corner search of
+ * COLUMNS steps run on a second stream beside the column index and the
+ * interaction; their times are spans while they share the GPU with those
+ * kernels, not their time alone); lq_kernel_times returns the
  * accumulated milliseconds and launch counts per class.  Graph replays are
  * timed too: the captured graph holds the events as event-record nodes,
  * pointed at fresh events before every replay.  Once 8192 event pairs are
@@ -370,7 +373,12 @@ int lq_synchronize(lq_ctx* ctx);
 enum lq_kernel_class {
   LQ_K_NEIGHBOR_SEARCH = 0, LQ_K_SORT_ARRAYS = 1, LQ_K_ADAMI = 2, LQ_K_CORNER = 3,
   LQ_K_INTERACTION = 4, LQ_K_CONTACT = 5, LQ_K_PREDICTOR = 6, LQ_K_CORRECTOR = 7,
-  LQ_K_SHEPARD = 8, LQ_K_PAIR_LIST_BUILD = 9, LQ_K_CLASS_COUNT = 10
+  LQ_K_SHEPARD = 8, LQ_K_PAIR_LIST_BUILD = 9,
+  /* COLUMNS: the sub-cell index and the pair records it writes (two timed
+   * sections per call when Adami runs beside it: soil records, then the
+   * wall records once Adami is done); LQ_K_INTERACTION then times the
+   * interaction kernel alone */
+  LQ_K_COLUMN_INDEX = 10, LQ_K_CLASS_COUNT = 11
 };
 int lq_enable_timing(lq_ctx* ctx, int enable);
 /* Times only the classes whose bit (1 << lq_kernel_class) is set in mask
@@ -399,12 +407,14 @@ int lq_set_interaction(lq_ctx* ctx, int variant, int xcd_remap);
  * bit-identical results; lq_autotune picks one for a context set to
  * COLUMNS; the default is 0.  No reference counterpart. */
 int lq_set_columns_build(lq_ctx* ctx, int build);
-/* COLUMNS steps: run the corner search (cuFindCornerParticleMaterial,
+/* COLUMNS steps: run Adami (cuAdamiBoundaryMaterial, adami_boundary.cu:
+ * 26-158) and the corner search (cuFindCornerParticleMaterial,
  * adami_boundary.cu:209-292) on a second stream beside the column build and
- * the interaction, which then leaves the wall contact of particles whose cell
+ * the interaction.  The walls' pair records are written once Adami is done;
+ * the interaction leaves the wall contact of particles whose cell
  * neighbourhood holds a wall particle to a pass after the search (on by
  * default; fused steps do not use it).  Results are bit-identical either
- * way.  The corner class is then not timed on its own.  No reference
+ * way.  Their timed sections then sit on the second stream.  No reference
  * counterpart. */
 int lq_set_overlap(lq_ctx* ctx, int on);
 /* Fused steps of the COLUMNS variant (off by default): lq_step then advances

@@ -60,7 +60,7 @@ FIELDS = {
 
 # lq_kernel_class
 KERNEL_CLASSES = ["neighbor_search", "sort_arrays", "adami", "corner", "interaction",
-                  "contact", "predictor", "corrector", "shepard", "pair_list_build"]
+                  "contact", "predictor", "corrector", "shepard", "pair_list_build", "column_index"]
 
 
 class RigidMotion(C.Structure):

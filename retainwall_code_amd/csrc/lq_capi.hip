@@ -133,25 +133,28 @@ static hipEvent_t ev_get(lq_ctx* c) {
 }
 // During stream capture an event record is kept in the graph as a node only
 // with the external flag (a plain record just orders the capture).
-static void ev_record(lq_ctx* c, hipEvent_t e) {
-  if (c->capturing) hipEventRecordWithFlags(e, c->stream, hipEventRecordExternal);
-  else hipEventRecord(e, c->stream);
+static void ev_record(lq_ctx* c, hipEvent_t e, hipStream_t s) {
+  if (c->capturing) hipEventRecordWithFlags(e, s, hipEventRecordExternal);
+  else hipEventRecord(e, s);
 }
+// A timed section on the library's stream, or on the side stream for work
+// that the main stream joins before it reaches its next timed section.
 struct Section {
   lq_ctx* c;
   int cls;
+  hipStream_t s;
   hipEvent_t a = nullptr;
-  Section(lq_ctx* c_, int cls_) : c(c_), cls(cls_) {
+  Section(lq_ctx* c_, int cls_, hipStream_t s_ = nullptr) : c(c_), cls(cls_), s(s_ ? s_ : c_->stream) {
     if (on()) {
       a = ev_get(c);
-      ev_record(c, a);
+      ev_record(c, a, s);
     }
   }
   bool on() const { return c->timing && ((c->timing_mask >> cls) & 1u); }
   ~Section() {
     if (on()) {
       hipEvent_t b = ev_get(c);
-      ev_record(c, b);
+      ev_record(c, b, s);
       c->pending.push_back({cls, a, b});
     }
   }
@@ -1154,7 +1157,6 @@ static void enq_interaction_kernel(lq_ctx* c) {
     launch_interaction(c->stream, c->dP, c->npm0, c->B.cur, c->B, c->B.cell_beg, c->B.cell_end,
                        LQ_INTERACTION_PAIR_LIST, c->xcd_remap, false);  // guarded cell-scan fallback
   } else if (c->par.interaction_variant == LQ_INTERACTION_COLUMNS) {
-    launch_col_index(c->stream, c->dP, c->par.cell_num[0], c->par.cell_num[1], c->B.cur, c->B);
     launch_interaction_col(c->stream, c->dP, c->npm0, c->B.cur, c->B, c->xcd_remap, c->col_wide);
   } else if (c->par.interaction_variant == LQ_INTERACTION_TILE) {
     if (c->npm0 > 0)
@@ -1171,9 +1173,10 @@ static void enq_plist_build(lq_ctx* c) {
 // stage 0: first interaction after a sort (a pair-list variant builds its
 // lists); stage 1: the corrector-stage call of the same step (lists reused
 // when still exact).
-// beside: the corner search was enqueued on the side stream (enq_corner_beside,
-// COLUMNS only); the interaction leaves the wall contact of particles near a
-// wall for k_contact_after, which waits for the search.
+// beside: Adami and the corner search were enqueued on the side stream
+// (enq_walls_beside, COLUMNS only); the walls' pair records wait for Adami,
+// and the interaction leaves the wall contact of particles near a wall for
+// k_contact_after, which waits for the search.
 static int enq_interaction(lq_ctx* c, int stage, bool beside = false) {
   const bool build = stage == 0 && c->par.interaction_variant == LQ_INTERACTION_PAIR_LIST;
   launch_reset_deferred(c->stream, c->dP, build ? 1 : 0);
@@ -1184,16 +1187,24 @@ static int enq_interaction(lq_ctx* c, int stage, bool beside = false) {
     Section s(c, LQ_K_PAIR_LIST_BUILD);
     enq_plist_build(c);
   }
+  if (beside) {  // the index and soil records beside Adami, the wall records after it
+    {
+      Section s(c, LQ_K_COLUMN_INDEX);
+      launch_col_index(c->stream, c->dP, c->par.cell_num[0], c->par.cell_num[1], c->B.cur, c->B, 0);
+    }
+    hipStreamWaitEvent(c->stream, c->ev_adami, 0);
+    {
+      Section s(c, LQ_K_COLUMN_INDEX);
+      launch_col_wall_records(c->stream, c->dP, c->par.npb, c->B.cur, c->B);
+    }
+  } else if (c->par.interaction_variant == LQ_INTERACTION_COLUMNS) {
+    Section s(c, LQ_K_COLUMN_INDEX);
+    launch_col_index(c->stream, c->dP, c->par.cell_num[0], c->par.cell_num[1], c->B.cur, c->B);
+  }
   {
     Section s(c, LQ_K_INTERACTION);
-    if (beside) {  // the index and soil records beside Adami, the wall records after it
-      launch_col_index(c->stream, c->dP, c->par.cell_num[0], c->par.cell_num[1], c->B.cur, c->B, 0);
-      hipStreamWaitEvent(c->stream, c->ev_adami, 0);
-      launch_col_wall_records(c->stream, c->dP, c->par.npb, c->B.cur, c->B);
-      launch_interaction_col(c->stream, c->dP, c->npm0, c->B.cur, c->B, c->xcd_remap, c->col_wide, 3);
-    } else {
-      enq_interaction_kernel(c);
-    }
+    if (beside) launch_interaction_col(c->stream, c->dP, c->npm0, c->B.cur, c->B, c->xcd_remap, c->col_wide, 3);
+    else enq_interaction_kernel(c);
   }
   {
     Section s(c, LQ_K_CONTACT);
@@ -1259,8 +1270,11 @@ static bool fused(const lq_ctx* c) {
 static void enq_fused(lq_ctx* c, int stage) {
   launch_reset_deferred(c->stream, c->dP, 0);
   {
-    Section s(c, LQ_K_INTERACTION);
+    Section s(c, LQ_K_COLUMN_INDEX);
     launch_col_index(c->stream, c->dP, c->par.cell_num[0], c->par.cell_num[1], c->B.cur, c->B);
+  }
+  {
+    Section s(c, LQ_K_INTERACTION);
     launch_interaction_col(c->stream, c->dP, c->npm0, c->B.cur, c->B, c->xcd_remap, c->col_wide,
                            stage + 1);
   }
@@ -1280,18 +1294,25 @@ static void enq_fused(lq_ctx* c, int stage) {
 // (cuFindCornerParticleMaterial) on the side stream, so that they run beside
 // the column build and the interaction (both are latency-bound on few
 // waves); enq_interaction(..., true) waits for Adami before it writes the
-// walls' pair records and for the search before the wall contact.  Neither
-// is timed as a class of its own then.
+// walls' pair records and for the search before the wall contact.  Their
+// timed sections sit on the side stream; the contact section's wait on
+// ev_join orders them before the main stream's next section ends.
 static bool corner_beside(const lq_ctx* c) {
   return c->overlap && c->side && c->par.interaction_variant == LQ_INTERACTION_COLUMNS && !(c->fuse && c->B.fl_pos);
 }
 static void enq_walls_beside(lq_ctx* c) {
   hipEventRecord(c->ev_fork, c->stream);
   hipStreamWaitEvent(c->side, c->ev_fork, 0);
-  if (c->npm0 > 0) launch_adami(c->side, c->dP, c->par.npb, c->B.cur, c->B.cell_beg, c->B.cell_end);
+  {
+    Section t(c, LQ_K_ADAMI, c->side);
+    if (c->npm0 > 0) launch_adami(c->side, c->dP, c->par.npb, c->B.cur, c->B.cell_beg, c->B.cell_end);
+  }
   hipEventRecord(c->ev_adami, c->side);
-  if (c->par.npb > 0)
-    launch_corner(c->side, c->dP, c->npm0, c->B.cur, c->B.cell_beg, c->B.cell_end, c->B.wall_near);
+  {
+    Section t(c, LQ_K_CORNER, c->side);
+    if (c->par.npb > 0)
+      launch_corner(c->side, c->dP, c->npm0, c->B.cur, c->B.cell_beg, c->B.cell_end, c->B.wall_near);
+  }
   hipEventRecord(c->ev_join, c->side);
 }
 
@@ -1963,13 +1984,18 @@ extern "C" int lq_autotune(lq_ctx* c, int reps, int* variant_out, int* shape_out
   Cand win = {c->par.interaction_variant, cols ? c->col_wide : c->tile_shape, c->tile_tz, c->xcd_remap};
   for (const Cand& k : cands) {
     apply(k);
+    auto one_call = [&]() {  // COLUMNS rebuilds its sub-cell index and records per call
+      if (k.variant == LQ_INTERACTION_COLUMNS)
+        launch_col_index(c->stream, c->dP, c->par.cell_num[0], c->par.cell_num[1], c->B.cur, c->B);
+      enq_interaction_kernel(c);
+    };
     auto step_calls = [&]() {  // the two interaction calls of one step
       const bool pl = k.variant == LQ_INTERACTION_PAIR_LIST;
       launch_reset_deferred(c->stream, c->dP, pl ? 1 : 0);
       if (pl) enq_plist_build(c);
-      enq_interaction_kernel(c);
+      one_call();
       launch_reset_deferred(c->stream, c->dP, 0);
-      enq_interaction_kernel(c);
+      one_call();
     };
     step_calls();  // warm-up
     hipEventRecord(e0, c->stream);

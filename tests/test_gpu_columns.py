@@ -243,3 +243,43 @@ def test_columns_corner_beside_bit_identical(name):
     say("  3 graph + 3 eager steps bit-identical")
     a.close()
     b.close()
+
+
+def test_columns_beside_timed_sections():
+    """With Adami and the corner search on the side stream, their timed
+    sections are recorded there, and the column index (two sections per
+    call) is timed apart from the interaction kernel: eager steps and graph
+    replays count the same launches per class, each class takes time, and
+    the timed steps' state equals the untimed serial order's."""
+    case = cases.config("small")
+    say("[beside timing] case=small steps=2+4")
+    a = sim.open_case(case, variant=COLS)
+    b = sim.open_case(case, variant=COLS)
+    ref = sim.open_case(case, variant=COLS)
+    ref.set_overlap(False)
+    for c in (a, b):
+        c.enable_timing(True)
+    a.step(2, graph=False)
+    b.step(2, graph=True)
+    ref.step(2, graph=True)
+    a.sync()
+    b.sync()
+    a.enable_timing(True)  # reset the totals
+    b.enable_timing(True)
+    a.step(4, graph=False)
+    b.step(4, graph=True)
+    ref.step(4, graph=True)
+    ka, kb = a.kernel_times(), b.kernel_times()
+    say(f"  eager adami {ka['adami']} corner {ka['corner']}; graph adami {kb['adami']} corner {kb['corner']}")
+    for k in ["neighbor_search", "sort_arrays", "adami", "corner", "column_index", "interaction", "contact",
+              "predictor", "corrector"]:
+        assert kb[k][1] == ka[k][1], k
+        assert ka[k][0] > 0.0 and kb[k][0] > 0.0, k
+    assert ka["adami"][1] == 8 and ka["corner"][1] == 8  # twice per step
+    assert ka["interaction"][1] == 8 and ka["column_index"][1] == 16  # soil records, then wall records
+    ref.sync()
+    for k in ("pos", "vel", "str", "cvar", "cmag"):
+        np.testing.assert_array_equal(bits(b.get(k)), bits(ref.get(k)), err_msg=k)
+        np.testing.assert_array_equal(bits(a.get(k)), bits(ref.get(k)), err_msg=k)
+    for c in (a, b, ref):
+        c.close()
