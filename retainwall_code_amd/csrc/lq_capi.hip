@@ -1106,12 +1106,13 @@ extern "C" int lq_case_count(const lq_ctx* c) { return c ? c->n_cases : 0; }
 static int enq_step_calc(lq_ctx* c) { launch_step_calc(c->stream, c->dP); return 0; }
 // After the first search the sort merges the few slots whose cell changed
 // into the previous order (the sorted keys alternate between two buffers).
-static int enq_ns(lq_ctx* c) {
+// flag_cleared: the step's first launch has cleared the sort flag.
+static int enq_ns(lq_ctx* c, bool flag_cleared = false) {
   Section s(c, LQ_K_NEIGHBOR_SEARCH);
   Buffers& B = c->B;
   const int* prev = c->searched ? B.part_cel : nullptr;
   int* out = B.part_cel == B.skey[0] ? B.skey[1] : B.skey[0];
-  launch_neighbor_search(c->stream, c->dP, c->par.np, c->nbits, c->par.ncell, B.cur, B, prev, out);
+  launch_neighbor_search(c->stream, c->dP, c->par.np, c->nbits, c->par.ncell, B.cur, B, prev, out, !flag_cleared);
   c->searched = true;
   return 0;
 }
@@ -1320,12 +1321,14 @@ static void enq_walls_beside(lq_ctx* c) {
 // search and SortArraysAll are skipped: the predictor never re-bins a
 // particle and no key changes between the two searches, so the second
 // stable sort is the identity permutation (asserted by the oracle tests).
+// StepCalculation, the search's flag clear and RigidBodyUpdate go in one
+// launch at the start (the bodies' motion tables are read by nothing before
+// Adami, which follows the sort).
 static int enq_step(lq_ctx* c) {
   const int shep = c->par.stress_shepard_filter;
-  enq_step_calc(c);
-  enq_ns(c);
+  launch_step_begin(c->stream, c->dP, c->par.n_motion);
+  enq_ns(c, true);
   enq_sort_primary(c);
-  if (c->par.n_motion > 0) launch_rigid_body_update(c->stream, c->dP);
   const bool beside = corner_beside(c);
   if (beside) {
     enq_walls_beside(c);

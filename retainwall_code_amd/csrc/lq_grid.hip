@@ -17,27 +17,57 @@ constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
 __device__ __forceinline__ int cell_key(const DevParams* P, int4 c) { return grid_key(P, c.x, c.y, c.z); }
 
 // cuGetParticleCellIndex, loquat_grid.cu:71-95.  Also flags whether any key
-// differs from the previous sorted key at the same slot.
-__global__ void k_hash(DevParams* __restrict__ P, Arrays A, int* __restrict__ keys, int* __restrict__ vals,
-                       const int* __restrict__ prev_sorted, int* __restrict__ mflag) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P->np) return;
-  int4 c = A.cellpos[i];
-  int key;
-  if (c.w == P->ncell_delete_m) {
-    key = P->ncell_delete_m;
-  } else {
-    key = cell_key(P, c);
-    if (A.idm[i].x >= 100.0f) key += P->ncell_material;  // MATERIAL_NUMBER, types.h:37
-    c.w = key;
-    A.cellpos[i] = c;
+// differs from the previous sorted key at the same slot; with mflag, the
+// flags of slots 0..np (slot np: 0) and their per-1024 block sums (the first
+// pass of the flag scan).  The same launch empties the cell ranges
+// (the memset of loquat_grid.cu:159-160) over ncell keys:
+// nothing reads them again before k_beg_end.  1024 threads per block.
+constexpr int HASH_THREADS = 1024;
+__global__ void __launch_bounds__(HASH_THREADS) k_hash(DevParams* __restrict__ P, Arrays A, int* __restrict__ keys,
+                                                       int* __restrict__ vals, const int* __restrict__ prev_sorted,
+                                                       int* __restrict__ mflag, int* __restrict__ blocksum,
+                                                       int ncell, int* __restrict__ cbeg, int* __restrict__ cend) {
+  __shared__ int sh_wave[HASH_THREADS / 64];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int np = P->np;
+  if (i < ncell) {
+    cbeg[i] = -1;
+    cend[i] = -1;
   }
-  keys[i] = key;
-  vals[i] = i;
-  if (prev_sorted != nullptr) {
-    const bool moved = key != prev_sorted[i];
-    if (moved) P->sort_changed = 1;
-    if (mflag) mflag[i] = moved ? 1 : 0;
+  int flag = 0;
+  if (i < np) {
+    int4 c = A.cellpos[i];
+    int key;
+    if (c.w == P->ncell_delete_m) {
+      key = P->ncell_delete_m;
+    } else {
+      key = cell_key(P, c);
+      if (A.idm[i].x >= 100.0f) key += P->ncell_material;  // MATERIAL_NUMBER, types.h:37
+      c.w = key;
+      A.cellpos[i] = c;
+    }
+    keys[i] = key;
+    vals[i] = i;
+    if (prev_sorted != nullptr) {
+      const bool moved = key != prev_sorted[i];
+      if (moved) P->sort_changed = 1;
+      flag = moved ? 1 : 0;
+      if (mflag) mflag[i] = flag;
+    }
+  } else if (i == np && mflag) {
+    mflag[np] = 0;
+  }
+  if (blocksum && blockIdx.x * HASH_THREADS <= np) {  // blocks holding flags 0..np
+    int v = flag;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if ((threadIdx.x & 63) == 0) sh_wave[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int t = 0;
+      for (int w = 0; w < HASH_THREADS / 64; w++) t += sh_wave[w];
+      blocksum[blockIdx.x] = t;
+    }
   }
 }
 
@@ -137,11 +167,33 @@ __global__ void __launch_bounds__(SC_THREADS) k_scan_tiles(int* __restrict__ dat
 
 static int scan_tiles(int n) { return (n + SC_TILE - 1) / SC_TILE; }
 
+// The flag scan's last pass over tiles of HASH_THREADS flags (one per
+// thread): the exclusive scan in place (sums: the scanned block sums of
+// k_hash) and the movers compacted in slot order into mkey / mslot.
+__global__ void __launch_bounds__(HASH_THREADS) k_scan_movers(int* __restrict__ mflag, int n,
+                                                              const int* __restrict__ sums,
+                                                              const int* __restrict__ keys, int np,
+                                                              int* __restrict__ mkey, int* __restrict__ mslot) {
+  __shared__ int sh_wave[HASH_THREADS / 64];
+  const int k = blockIdx.x * HASH_THREADS + threadIdx.x;
+  const int v = k < n ? mflag[k] : 0;
+  const int excl = block_incl_scan(v, sh_wave) - v + sums[blockIdx.x];
+  if (k < n) mflag[k] = excl;
+  if (v && k < np) {
+    mkey[excl] = keys[k];
+    mslot[excl] = k;
+  }
+}
+
 // wall_near[c] = 1 when any of the 27 cells around cell c (its key) holds
 // a wall particle, else 0 (also for Morton keys outside the grid).
-__global__ void __launch_bounds__(256) k_wall_near(const DevParams* __restrict__ P, const int* __restrict__ cbeg,
-                                                   int* __restrict__ wall_near) {
+__device__ void delete_count(DevParams* P, const int* __restrict__ cbeg, const int* __restrict__ cend);
+// Thread 0 also does the deleted-particle bookkeeping (k_delete_count),
+// which reads the same finished cell ranges.
+__global__ void __launch_bounds__(256) k_wall_near(DevParams* __restrict__ P, const int* __restrict__ cbeg,
+                                                   const int* __restrict__ cend, int* __restrict__ wall_near) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c == 0) delete_count(P, cbeg, cend);
   if (c >= P->ncell_material) return;
   int cx, cy, cz;
   if (P->key_mode == LQ_KEY_MORTON) {
@@ -207,19 +259,6 @@ __global__ void __launch_bounds__(RS_THREADS) k_radix_scatter(const int* __restr
 // order; the others ("movers") are compacted in slot order, sorted stably by
 // key, and the two sorted sequences are merged by rank.  The result equals
 // the full radix sort for any inputs; it is fast when movers are few.
-
-// Movers in slot order: mflag holds the exclusive scan of the flags
-// (np + 1 entries, the count last).
-__global__ void k_mover_compact(const DevParams* __restrict__ P, const int* __restrict__ keys,
-                                const int* __restrict__ mscan, int* __restrict__ mkey, int* __restrict__ mslot) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P->np) return;
-  const int r = mscan[i];
-  if (mscan[i + 1] != r) {
-    mkey[r] = keys[i];
-    mslot[r] = i;
-  }
-}
 
 // Stable LSD radix sort of the M movers by key in one workgroup, over 8-bit
 // digits, tiles of 1024 ranked by wave ballots as in k_radix_scatter.  M is
@@ -345,7 +384,7 @@ __global__ void k_beg_end(const DevParams* __restrict__ P, const int* __restrict
 }
 
 // Deleted-particle bookkeeping, loquat_grid.cu:165-185, kept on the device.
-__global__ void k_delete_count(DevParams* P, const int* __restrict__ cbeg, const int* __restrict__ cend) {
+__device__ void delete_count(DevParams* P, const int* __restrict__ cbeg, const int* __restrict__ cend) {
   const int nd = P->ncell_delete_m;
   int beg = cbeg[nd];
   if (beg != -1) {
@@ -354,6 +393,9 @@ __global__ void k_delete_count(DevParams* P, const int* __restrict__ cbeg, const
     P->np_delete_m += now;
   }
   if (P->npb + P->npm + P->np_delete_m != P->np) set_err(P, DERR_DELETE);
+}
+__global__ void k_delete_count(DevParams* P, const int* __restrict__ cbeg, const int* __restrict__ cend) {
+  delete_count(P, cbeg, cend);
 }
 
 // SortArraysPrimary / SortArraysAll (loquat.cu:254-287) as one gather of
@@ -475,18 +517,18 @@ int radix_tiles(int n) { return blocks(n, RS_TILE); }
 // the sort is the merge search; without it a full radix sort.  The sorted
 // keys go to out_key (not prev), the permutation to B.vals.
 void launch_neighbor_search(hipStream_t s, DevParams* P, int np, int nbits, int ncell, Arrays A, Buffers& B,
-                            const int* prev, int* out_key) {
-  hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, s, P);
-  hipLaunchKernelGGL(k_hash, dim3(blocks(np, 256)), dim3(256), 0, s, P, A, B.keys, B.vals, prev,
-                     prev ? B.mflag : nullptr);
+                            const int* prev, int* out_key, bool clear_flag) {
+  if (clear_flag) hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, s, P);
+  // the merge path scans np + 1 flags in tiles of HASH_THREADS: the hash
+  // writes the tile sums, one workgroup scans them, k_scan_movers finishes
+  const int n = np + 1, nt = (n + HASH_THREADS - 1) / HASH_THREADS;
+  int* sums = B.hist;
+  hipLaunchKernelGGL(k_hash, dim3(blocks(std::max(n, ncell), HASH_THREADS)), dim3(HASH_THREADS), 0, s, P, A, B.keys,
+                     B.vals, prev, prev ? B.mflag : nullptr, prev ? sums : nullptr, ncell, B.cell_beg, B.cell_end);
   if (prev) {
-    const int n = np + 1, nt = scan_tiles(n);
-    int* sums = B.hist;
-    hipLaunchKernelGGL(k_scan_tile_sums, dim3(nt), dim3(SC_THREADS), 0, s, (const int*)B.mflag, n, sums);
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SC_THREADS), 0, s, sums, nt);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(SC_THREADS), 0, s, B.mflag, n, (const int*)sums);
-    hipLaunchKernelGGL(k_mover_compact, dim3(blocks(np, 256)), dim3(256), 0, s, (const DevParams*)P,
-                       (const int*)B.keys, (const int*)B.mflag, B.mkey, B.mslot);
+    hipLaunchKernelGGL(k_scan_movers, dim3(nt), dim3(HASH_THREADS), 0, s, B.mflag, n, (const int*)sums,
+                       (const int*)B.keys, np, B.mkey, B.mslot);
     hipLaunchKernelGGL(k_mover_sort, dim3(1), dim3(MS_THREADS), 0, s, (const int*)(B.mflag + np), B.mkey, B.mslot,
                        B.mkey_alt, B.mslot_alt, nbits);
     const bool odd = ((nbits + 7) / 8) & 1;
@@ -516,14 +558,13 @@ void launch_neighbor_search(hipStream_t s, DevParams* P, int np, int nbits, int 
     B.part_idx = vin;
   }
   B.part_cel = out_key;
-  launch_fill_int(s, B.cell_beg, ncell, -1);
-  launch_fill_int(s, B.cell_end, ncell, -1);
   hipLaunchKernelGGL(k_beg_end, dim3(blocks(np, 256)), dim3(256), 0, s, P, (const int*)out_key, B.cell_beg, B.cell_end);
-  if (ncell > 1) {
-    const int ncm = (ncell - 1) / 2;
-    hipLaunchKernelGGL(k_wall_near, dim3(blocks(ncm, 256)), dim3(256), 0, s, P, (const int*)B.cell_beg, B.wall_near);
-  }
-  hipLaunchKernelGGL(k_delete_count, dim3(1), dim3(1), 0, s, P, (const int*)B.cell_beg, (const int*)B.cell_end);
+  const int ncm = (ncell - 1) / 2;
+  if (ncm > 0)
+    hipLaunchKernelGGL(k_wall_near, dim3(blocks(ncm, 256)), dim3(256), 0, s, P, (const int*)B.cell_beg,
+                       (const int*)B.cell_end, B.wall_near);
+  else
+    hipLaunchKernelGGL(k_delete_count, dim3(1), dim3(1), 0, s, P, (const int*)B.cell_beg, (const int*)B.cell_end);
 }
 
 void launch_gather(hipStream_t s, const DevParams* P, int np, Arrays src, Arrays dst, const int* idx, const Buffers& B) {

@@ -12,6 +12,10 @@ void launch_rel2abs(hipStream_t s, const DevParams* P, int np, Arrays A, float4*
 void launch_step_calc(hipStream_t s, DevParams* P);
 void launch_advance(hipStream_t s, DevParams* P);
 void launch_rigid_body_update(hipStream_t s, DevParams* P);
+// StepCalculation, the search's flag clear and RigidBodyUpdate in one
+// launch at the start of a step (the bodies' motion is read by nothing
+// before Adami).
+void launch_step_begin(hipStream_t s, DevParams* P, int n_motion);
 void launch_adami(hipStream_t s, const DevParams* P, int npb, Arrays A, const int* cb, const int* ce);
 void launch_corner(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const int* cb, const int* ce,
                    const int* wall_near);
@@ -92,8 +96,10 @@ void launch_adam(hipStream_t s, float* p, const float* g, float* m, float* v, in
 void launch_gather_rows(hipStream_t s, const float* src, int ld, const int* idx, int rows, int cols, float* dst);
 // prev: sorted keys of the previous search (merge search), or nullptr (full
 // radix sort).  Sorted keys go to out_key; sets B.part_cel and B.part_idx.
+// clear_flag false: the step's first kernel (launch_step_begin) has already
+// cleared sort_changed.
 void launch_neighbor_search(hipStream_t s, DevParams* P, int np, int nbits, int ncell, Arrays A, Buffers& B,
-                            const int* prev, int* out_key);
+                            const int* prev, int* out_key, bool clear_flag = true);
 void launch_gather(hipStream_t s, const DevParams* P, int np, Arrays src, Arrays dst, const int* idx, const Buffers& B);
 // The same permutation in place (alternate set as scratch), touching only moved slots.
 void launch_gather_moved(hipStream_t s, const DevParams* P, int np, Arrays cur, Arrays alt, const int* idx);

@@ -78,11 +78,9 @@ __global__ void k_reset_deferred(DevParams* P, int list_build) {
 }
 
 
-// RigidBodyUpdate, adami_boundary.cu:175-203 (one thread).
-__global__ void k_rigid_body_update(DevParams* P) {
-  const float t = P->time, dt = P->dt;
-  // one thread per body: each body touches only its own table and motion
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P->n_motion; i += gridDim.x * blockDim.x) {
+// RigidBodyUpdate, adami_boundary.cu:175-203, for body i at time t.
+__device__ void rigid_body_one(DevParams* P, int i, float t, float dt) {
+  {
     DevTable& m = P->tab[i];
     DevMotion& mv = P->mv[i];
     if (m.is_rotation) {
@@ -109,6 +107,26 @@ __global__ void k_rigid_body_update(DevParams* P) {
       }
     }
   }
+}
+// one thread per body: each body touches only its own table and motion
+__global__ void k_rigid_body_update(DevParams* P) {
+  const float t = P->time, dt = P->dt;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P->n_motion; i += gridDim.x * blockDim.x)
+    rigid_body_one(P, i, t, dt);
+}
+// k_step_calc, the search's sort_changed clear and k_rigid_body_update in
+// one launch: every thread forms dt and time as k_step_calc does, thread 0
+// stores them.
+__global__ void k_step_begin(DevParams* P) {
+  const float dtm = P->h / P->cs;
+  const float dt = P->cfl * dtm, t = float(P->time_d);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    P->dt = dt;
+    P->time = t;
+    P->sort_changed = 0;
+  }
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P->n_motion; i += gridDim.x * blockDim.x)
+    rigid_body_one(P, i, t, dt);
 }
 
 // ------------------------------------------------------------- Adami ----
@@ -629,16 +647,19 @@ __global__ void __launch_bounds__(NL_WAVE * NL_WAVES, 5) k_interaction_nl(DevPar
 
 // Pass B: same-row redistribution for deferred particles
 // (particle_interaction.cu:142-172, 209-237), reading pass-A values only.
-// A group of 16 lanes per deferred particle, splitting each soil range
-// (cells hold ~18 candidates).  The
-// serial loop keeps K = min(K, Kj) over the qualifying neighbours and
-// overwrites fr = fj * K / Kj at each, so its result is the minimum K and
-// the force of the last qualifying neighbour in traversal order scaled by
-// that minimum: the wave reduces exactly those (a min and an arg-max of the
-// traversal position) and evaluates the same expression.
+// A group of 32 lanes per deferred particle, lane r < 27 walking the soil
+// range of neighbour cell r, four candidates' loads in flight at a time
+// (the walk is a chain of dependent loads; splitting it by cell keeps it
+// short).  The serial loop keeps K = min(K, Kj) over the qualifying
+// neighbours and overwrites fr = fj * K / Kj at each, so its result is the
+// minimum K and the force of the last qualifying neighbour in traversal
+// order scaled by that minimum: the group reduces exactly those (a min and
+// an arg-max of the traversal position, cell r then slot) and evaluates the
+// same expression.
 // npos / ncell: where the neighbours' positions and cells are read, A.pos
 // and A.cellpos, or the copies a fused interaction keeps of its first-layer
 // particles before it moves them.
+constexpr int RD_GROUP = 32;  // lanes per deferred particle (27 cells)
 __global__ void __launch_bounds__(256) k_contact_redistribute(const DevParams* __restrict__ P, Arrays A, Buffers B,
                                                               const int* __restrict__ cbeg,
                                                               const int* __restrict__ cend,
@@ -647,7 +668,7 @@ __global__ void __launch_bounds__(256) k_contact_redistribute(const DevParams* _
   const int nd = P->n_deferred;
   const float two_h = P->two_h, band = P->band, half_dr = P->half_dr;
   const int ncm = P->ncell_material;
-  constexpr int G = 16;  // lanes per deferred particle
+  constexpr int G = RD_GROUP;
   const int lane = threadIdx.x & (G - 1);
   const int ngroups = int(gridDim.x * blockDim.x) / G;
   // k is uniform within a group and the shuffles below stay inside it
@@ -660,26 +681,33 @@ __global__ void __launch_bounds__(256) k_contact_redistribute(const DevParams* _
                                   pi.z + float(ci.z - cs.oz) * two_h);
     const float3 ni = ld3(A.cn[i]);
     float K = A.cvar[i].x;
-    int best = -1, bestj = 0;  // traversal position (range << 20 | offset) and slot of the last qualifying j
-    for (int r = 0; r < 27; r++) {
-      const int cz = r / 9 - 1, cy = (r / 3) % 3 - 1, cx = r % 3 - 1;
-      const int jx = ci.x + cx, jy = ci.y + cy, jz = ci.z + cz;
-      if (!in_grid(P, jx, jy, jz)) continue;
+    int best = -1, bestj = 0;  // traversal position (cell << 20 | offset) and slot of the last qualifying j
+    const int r = lane;
+    const int jx = ci.x + r % 3 - 1, jy = ci.y + (r / 3) % 3 - 1, jz = ci.z + r / 9 - 1;
+    int b = 0, e = 0;
+    if (r < 27 && in_grid(P, jx, jy, jz)) {
       const int c = grid_key(P, jx, jy, jz) + ncm;
-      const int b = cbeg[c], e = cend[c];
-      for (int j = b + lane; j < e; j += G) {
-        const float4 cvj = A.cvar[j];
-        if (!(cvj.y <= band)) continue;
+      b = cbeg[c];
+      e = cend[c];
+    }
+    for (int j0 = b; j0 < e; j0 += 4) {
+      float4 cv[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) cv[u] = j0 + u < e ? A.cvar[j0 + u] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int j = j0 + u;
+        if (j >= e || !(cv[u].y <= band)) continue;
         const int4 cjj = ncell[j];
         const float4 pj = npos[j];
         const float3 pja = make_float3(pj.x + float(cjj.x - cs.ox) * two_h, pj.y + float(cjj.y - cs.oy) * two_h,
                                        pj.z + float(cjj.z - cs.oz) * two_h);
         const float3 dc = f3sub(pa, pja);
         const float rcol = fabsf(dc.x * ni.z + dc.y * ni.y + dc.z * ni.x);
-        const float Kj = cvj.x;
+        const float Kj = cv[u].x;
         if (rcol <= half_dr && Kj != 0.f) {
           if (K > Kj) K = Kj;
-          best = (r << 20) | (j - b);  // j ascends per lane, so this is the lane's last so far
+          best = (r << 20) | (j - b);  // j ascends in the lane's one cell
           bestj = j;
         }
       }
@@ -755,9 +783,7 @@ __global__ void __launch_bounds__(256) k_predictor(DevParams* __restrict__ P, Ar
 
 // --------------------------------------------------------- corrector ----
 // cuCorrectorBound, predictor_corrector.cu:109-137.
-__global__ void __launch_bounds__(256) k_corrector_bound(DevParams* __restrict__ P, Arrays A, Buffers B) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P->npb) return;
+__device__ __forceinline__ void corrector_bound_slot(DevParams* __restrict__ P, Arrays A, const Buffers& B, int i) {
   int4 cd = A.codes[i];
   if (cd.y != LQ_NONFIX) return;
   const DevMotion& m = P->mv[cd.w];
@@ -792,11 +818,23 @@ __global__ void __launch_bounds__(256) k_corrector_bound(DevParams* __restrict__
   A.cellpos[i] = ci;
   if (!in_case_grid(P, A.case_id, i, ci)) set_err(P, DERR_DOMAIN);
 }
-
-__global__ void __launch_bounds__(256) k_corrector_material(const DevParams* __restrict__ P, Arrays A, Buffers B) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_corrector_bound(DevParams* __restrict__ P, Arrays A, Buffers B) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < P->npb) corrector_bound_slot(P, A, B, i);
+}
+// Both corrector kernels in one launch: the first nbb blocks take the wall
+// slots, the rest the soil slots (each slot reads and writes only its own
+// rows).
+__global__ void __launch_bounds__(256) k_corrector_all(DevParams* __restrict__ P, Arrays A, Buffers B, int nbb) {
+  if (int(blockIdx.x) < nbb) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < P->npb) corrector_bound_slot(P, A, B, i);
+    return;
+  }
+  const int idx = (blockIdx.x - nbb) * blockDim.x + threadIdx.x;
   if (idx < P->npm) corrector_slot(P, A, B, P->npb + idx);
 }
+
 
 // The slots a fused interaction leaves to the predictor (the snapshot of
 // walls and of deleted slots, and the deferred soil, whose acceleration pass
@@ -1038,6 +1076,9 @@ void launch_advance(hipStream_t s, DevParams* P) { hipLaunchKernelGGL(k_advance,
 void launch_rigid_body_update(hipStream_t s, DevParams* P) {
   hipLaunchKernelGGL(k_rigid_body_update, dim3(4), dim3(64), 0, s, P);
 }
+void launch_step_begin(hipStream_t s, DevParams* P, int n_motion) {
+  hipLaunchKernelGGL(k_step_begin, dim3(n_motion > 0 ? 4 : 1), dim3(64), 0, s, P);
+}
 void launch_adami(hipStream_t s, const DevParams* P, int npb, Arrays A, const int* cb, const int* ce) {
   // 64-thread blocks spread the wall slots evenly over the CUs; below 16
   // waves per CU (256 CUs) the two sums go to separate threads
@@ -1084,8 +1125,9 @@ void launch_contact(hipStream_t s, const DevParams* P, int npm_max, Arrays A, co
                     const int* ce, bool fused) {
   if (npm_max <= 0) return;
   int g = std::min(blocks(npm_max, 256), 1024);
-  // 16 lanes per deferred particle, up to 16 * 4096 at a time
-  hipLaunchKernelGGL(k_contact_redistribute, dim3(std::min(blocks(npm_max, 16), 4096)), dim3(256), 0, s, P, A, B, cb,
+  // RD_GROUP lanes per deferred particle, up to 8 * 4096 at a time
+  hipLaunchKernelGGL(k_contact_redistribute, dim3(std::min(blocks(npm_max, 256 / RD_GROUP), 4096)), dim3(256), 0, s, P,
+                     A, B, cb,
                      ce, fused ? (const float4*)B.fl_pos : (const float4*)A.pos,
                      fused ? (const int4*)B.fl_cell : (const int4*)A.cellpos);
   hipLaunchKernelGGL(k_contact_commit, dim3(g), dim3(256), 0, s, P, A, B);
@@ -1105,10 +1147,9 @@ void launch_corrector_finish(hipStream_t s, DevParams* P, int npb, int n_motion,
   hipLaunchKernelGGL(k_corrector_finish, dim3(256), dim3(256), 0, s, P, A, B);
 }
 void launch_corrector(hipStream_t s, DevParams* P, int npb, int npm_max, int n_motion, Arrays A, const Buffers& B) {
-  if (n_motion > 0 && npb > 0)
-    hipLaunchKernelGGL(k_corrector_bound, dim3(blocks(npb, 256)), dim3(256), 0, s, P, A, B);
-  if (npm_max > 0)
-    hipLaunchKernelGGL(k_corrector_material, dim3(blocks(npm_max, 256)), dim3(256), 0, s, P, A, B);
+  const int nbb = n_motion > 0 && npb > 0 ? blocks(npb, 256) : 0;
+  const int nbm = npm_max > 0 ? blocks(npm_max, 256) : 0;
+  if (nbb + nbm > 0) hipLaunchKernelGGL(k_corrector_all, dim3(nbb + nbm), dim3(256), 0, s, P, A, B, nbb);
 }
 void launch_wall_profile(hipStream_t s, const DevParams* P, int np, int npb, Arrays A, float4 geom, int nbins, int* last,
                          float4* out) {
