@@ -1178,7 +1178,9 @@ static void enq_plist_build(lq_ctx* c) {
 // (enq_walls_beside, COLUMNS only); the walls' pair records wait for Adami,
 // and the interaction leaves the wall contact of particles near a wall for
 // k_contact_after, which waits for the search.
-static int enq_interaction(lq_ctx* c, int stage, bool beside = false) {
+// defer_commit: pass C is left to the predictor / corrector that follows
+// (enq_step; a piecewise call publishes it at once).
+static int enq_interaction(lq_ctx* c, int stage, bool beside = false, bool defer_commit = false) {
   const bool build = stage == 0 && c->par.interaction_variant == LQ_INTERACTION_PAIR_LIST;
   launch_reset_deferred(c->stream, c->dP, build ? 1 : 0);
   // COLUMNS writes its records in aux order while it indexes the sub-cells
@@ -1213,7 +1215,7 @@ static int enq_interaction(lq_ctx* c, int stage, bool beside = false) {
       hipStreamWaitEvent(c->stream, c->ev_join, 0);
       launch_contact_after(c->stream, c->dP, c->npm0, c->B.cur, c->B);
     }
-    launch_contact(c->stream, c->dP, c->npm0, c->B.cur, c->B, c->B.cell_beg, c->B.cell_end);
+    launch_contact(c->stream, c->dP, c->npm0, c->B.cur, c->B, c->B.cell_beg, c->B.cell_end, false, !defer_commit);
   }
   return 0;
 }
@@ -1223,16 +1225,18 @@ static void enq_constitutive(lq_ctx* c, int stage) {
   if (c->hp.sur_on) launch_surrogate(c->stream, c->dP, c->npm0, c->sur_members, c->n_cu, c->B.cur, c->B, stage);
   if (c->hp.log_every > 0) launch_sur_log(c->stream, c->dP, c->npm0, c->B.cur, c->B, stage);
 }
-static int enq_predictor(lq_ctx* c) {
+// commit: publish the last interaction's pass C too (its enq_interaction
+// deferred it)
+static int enq_predictor(lq_ctx* c, bool commit = false) {
   Section s(c, LQ_K_PREDICTOR);
   launch_predictor(c->stream, c->dP, c->par.np, c->B.cur, c->B,
-                   c->par.interaction_variant == LQ_INTERACTION_PAIR_LIST ? 1 : 0);
+                   c->par.interaction_variant == LQ_INTERACTION_PAIR_LIST ? 1 : 0, commit ? 1 : 0);
   enq_constitutive(c, 0);
   return 0;
 }
-static int enq_corrector(lq_ctx* c) {
+static int enq_corrector(lq_ctx* c, bool commit = false) {
   Section s(c, LQ_K_CORRECTOR);
-  launch_corrector(c->stream, c->dP, c->par.npb, c->npm0, c->par.n_motion, c->B.cur, c->B);
+  launch_corrector(c->stream, c->dP, c->par.npb, c->npm0, c->par.n_motion, c->B.cur, c->B, commit ? 1 : 0);
   enq_constitutive(c, 1);
   return 0;
 }
@@ -1343,8 +1347,8 @@ static int enq_step(lq_ctx* c) {
   if (fused(c)) {
     enq_fused(c, 0);
   } else {
-    enq_interaction(c, 0, beside);
-    enq_predictor(c);
+    enq_interaction(c, 0, beside, true);
+    enq_predictor(c, true);
   }
   if (beside) {
     enq_walls_beside(c);
@@ -1355,8 +1359,8 @@ static int enq_step(lq_ctx* c) {
   if (fused(c)) {
     enq_fused(c, 1);
   } else {
-    enq_interaction(c, 1, beside);
-    enq_corrector(c);
+    enq_interaction(c, 1, beside, true);
+    enq_corrector(c, true);
   }
   launch_advance(c->stream, c->dP);
   c->host_step++;

@@ -33,14 +33,18 @@ void launch_interaction(hipStream_t s, DevParams* P, int npm_max, Arrays A, cons
                         const int* ce, int variant, int xcd_remap, bool periodic);
 // fused: the interaction already moved the non-deferred particles; pass B
 // reads first-layer positions and cells from B.fl_pos / B.fl_cell
+// commit false: pass C is left to the next predictor / corrector launch
 void launch_contact(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const Buffers& B, const int* cb,
-                    const int* ce, bool fused = false);
+                    const int* ce, bool fused = false, bool commit = true);
 // pass A of the wall contact left by a fuse-3 COLUMNS interaction
 void launch_contact_after(hipStream_t s, DevParams* P, int npm_max, Arrays A, const Buffers& B);
 void launch_predictor_finish(hipStream_t s, DevParams* P, Arrays A, const Buffers& B);
 void launch_corrector_finish(hipStream_t s, DevParams* P, int npb, int n_motion, Arrays A, const Buffers& B);
-void launch_predictor(hipStream_t s, DevParams* P, int np, Arrays A, const Buffers& B, int track_disp);
-void launch_corrector(hipStream_t s, DevParams* P, int npb, int npm_max, int n_motion, Arrays A, const Buffers& B);
+// commit: also publish the last interaction's pass-B results (pass C)
+void launch_predictor(hipStream_t s, DevParams* P, int np, Arrays A, const Buffers& B, int track_disp,
+                      int commit = 0);
+void launch_corrector(hipStream_t s, DevParams* P, int npb, int npm_max, int n_motion, Arrays A, const Buffers& B,
+                      int commit = 0);
 // pts: absolute points; out[2 k], out[2 k + 1] per point (see k_sensor_stress)
 void launch_sensor_stress(hipStream_t s, const DevParams* P, Arrays A, const int* cb, const int* ce, const float4* pts,
                           int n, int rows, float4* out);

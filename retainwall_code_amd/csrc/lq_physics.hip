@@ -753,24 +753,30 @@ __global__ void __launch_bounds__(256) k_contact_after(DevParams* __restrict__ P
 }
 
 // Pass C: publish pass-B forces and the min-reduced K.
+__device__ __forceinline__ void contact_commit_one(const DevParams* __restrict__ P, Arrays A, const Buffers& B, int k) {
+  const int i = B.deferred[k];
+  float4 f = B.fdef[k];
+  A.cf[i] = make_float4(f.x, f.y, f.z, 0.f);
+  float4 cv = A.cvar[i];
+  cv.x = f.w;
+  A.cvar[i] = cv;
+}
 __global__ void k_contact_commit(const DevParams* __restrict__ P, Arrays A, Buffers B) {
   const int nd = P->n_deferred;
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nd; k += gridDim.x * blockDim.x) {
-    const int i = B.deferred[k];
-    float4 f = B.fdef[k];
-    A.cf[i] = make_float4(f.x, f.y, f.z, 0.f);
-    float4 cv = A.cvar[i];
-    cv.x = f.w;
-    A.cvar[i] = cv;
-  }
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nd; k += gridDim.x * blockDim.x) contact_commit_one(P, A, B, k);
 }
 
 // With track set, the wave maximum of the squared displacement goes to
 // max_disp2_bits for the pair lists (non-negative floats order like their
 // bit patterns).  The running maximum only grows, so a wave whose maximum a
 // plain read already covers skips the atomic.
-__global__ void __launch_bounds__(256) k_predictor(DevParams* __restrict__ P, Arrays A, Buffers B, int track) {
+// commit: the pass-C publication of the last interaction's deferred forces
+// (k_contact_commit) rides along, one entry per thread: it writes cf and
+// cvar, which the predictor neither reads nor writes.
+__global__ void __launch_bounds__(256) k_predictor(DevParams* __restrict__ P, Arrays A, Buffers B, int track,
+                                                   int commit) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (commit && i < P->n_deferred) contact_commit_one(P, A, B, i);
   float d2 = i < P->np ? predictor_slot(P, A, B, i) : 0.f;
   if (!track) return;
   for (int o = 32; o > 0; o >>= 1) d2 = fmaxf(d2, __shfl_xor(d2, o));
@@ -825,7 +831,12 @@ __global__ void __launch_bounds__(256) k_corrector_bound(DevParams* __restrict__
 // Both corrector kernels in one launch: the first nbb blocks take the wall
 // slots, the rest the soil slots (each slot reads and writes only its own
 // rows).
-__global__ void __launch_bounds__(256) k_corrector_all(DevParams* __restrict__ P, Arrays A, Buffers B, int nbb) {
+__global__ void __launch_bounds__(256) k_corrector_all(DevParams* __restrict__ P, Arrays A, Buffers B, int nbb,
+                                                       int commit) {
+  // pass C of the last interaction as in k_predictor (the corrector does not
+  // touch cf or cvar either)
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (commit && g < P->n_deferred) contact_commit_one(P, A, B, g);
   if (int(blockIdx.x) < nbb) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < P->npb) corrector_bound_slot(P, A, B, i);
@@ -1122,7 +1133,7 @@ void launch_interaction(hipStream_t s, DevParams* P, int npm_max, Arrays A, cons
                        xcd_remap, 1);
 }
 void launch_contact(hipStream_t s, const DevParams* P, int npm_max, Arrays A, const Buffers& B, const int* cb,
-                    const int* ce, bool fused) {
+                    const int* ce, bool fused, bool commit) {
   if (npm_max <= 0) return;
   int g = std::min(blocks(npm_max, 256), 1024);
   // RD_GROUP lanes per deferred particle, up to 8 * 4096 at a time
@@ -1130,13 +1141,13 @@ void launch_contact(hipStream_t s, const DevParams* P, int npm_max, Arrays A, co
                      A, B, cb,
                      ce, fused ? (const float4*)B.fl_pos : (const float4*)A.pos,
                      fused ? (const int4*)B.fl_cell : (const int4*)A.cellpos);
-  hipLaunchKernelGGL(k_contact_commit, dim3(g), dim3(256), 0, s, P, A, B);
+  if (commit) hipLaunchKernelGGL(k_contact_commit, dim3(g), dim3(256), 0, s, P, A, B);
 }
 void launch_contact_after(hipStream_t s, DevParams* P, int npm_max, Arrays A, const Buffers& B) {
   if (npm_max > 0) hipLaunchKernelGGL(k_contact_after, dim3(blocks(npm_max, 256)), dim3(256), 0, s, P, A, B);
 }
-void launch_predictor(hipStream_t s, DevParams* P, int np, Arrays A, const Buffers& B, int track_disp) {
-  if (np > 0) hipLaunchKernelGGL(k_predictor, dim3(blocks(np, 256)), dim3(256), 0, s, P, A, B, track_disp);
+void launch_predictor(hipStream_t s, DevParams* P, int np, Arrays A, const Buffers& B, int track_disp, int commit) {
+  if (np > 0) hipLaunchKernelGGL(k_predictor, dim3(blocks(np, 256)), dim3(256), 0, s, P, A, B, track_disp, commit);
 }
 void launch_predictor_finish(hipStream_t s, DevParams* P, Arrays A, const Buffers& B) {
   hipLaunchKernelGGL(k_predictor_finish, dim3(1024), dim3(256), 0, s, P, A, B);
@@ -1146,10 +1157,11 @@ void launch_corrector_finish(hipStream_t s, DevParams* P, int npb, int n_motion,
     hipLaunchKernelGGL(k_corrector_bound, dim3(blocks(npb, 256)), dim3(256), 0, s, P, A, B);
   hipLaunchKernelGGL(k_corrector_finish, dim3(256), dim3(256), 0, s, P, A, B);
 }
-void launch_corrector(hipStream_t s, DevParams* P, int npb, int npm_max, int n_motion, Arrays A, const Buffers& B) {
+void launch_corrector(hipStream_t s, DevParams* P, int npb, int npm_max, int n_motion, Arrays A, const Buffers& B,
+                      int commit) {
   const int nbb = n_motion > 0 && npb > 0 ? blocks(npb, 256) : 0;
   const int nbm = npm_max > 0 ? blocks(npm_max, 256) : 0;
-  if (nbb + nbm > 0) hipLaunchKernelGGL(k_corrector_all, dim3(nbb + nbm), dim3(256), 0, s, P, A, B, nbb);
+  if (nbb + nbm > 0) hipLaunchKernelGGL(k_corrector_all, dim3(nbb + nbm), dim3(256), 0, s, P, A, B, nbb, commit);
 }
 void launch_wall_profile(hipStream_t s, const DevParams* P, int np, int npb, Arrays A, float4 geom, int nbins, int* last,
                          float4* out) {
