@@ -331,8 +331,8 @@ __device__ __forceinline__ void pair_terms_fast(const PairConst& P, float i_kdr,
 // overflow), so a window's records are summed while they are still in the
 // caches.  Draining earlier or later changes no sum: each lane's hits are
 // evaluated in the order they were found.
-constexpr int CL_CAP = 32;    // ring entries per lane (power of two)
-constexpr int CL_CHUNK = 32;  // candidates per hit mask
+constexpr int CL_CAP = 16;    // ring entries per lane (power of two): 4 KB of LDS per wave
+constexpr int CL_CHUNK = 16;  // candidates per hit mask
 
 __device__ __forceinline__ void cl_eval(const PairConst& K, float i_kdr, IAcc& S, const Buffers& B, int e, float3 pi,
                                         float two_h, float i_h, float3 vi, float rhoi, Sym sri, float artpi,
