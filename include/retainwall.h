@@ -402,8 +402,9 @@ int lq_set_tile(lq_ctx* ctx, int shape, int tz);
  * context accepts REFERENCE_ORDER and NEIGHBOR_LIST only.  No reference
  * counterpart. */
 int lq_set_interaction(lq_ctx* ctx, int variant, int xcd_remap);
-/* Register build of the COLUMNS interaction kernel: 0 the 96-VGPR build
- * (5 waves per SIMD), 1 the 128-VGPR one (4 waves per SIMD).  Both give
+/* Register budget of the COLUMNS interaction kernel: 0 at most 96 VGPRs
+ * (5 waves per SIMD), 1 at most 128 (the compiler may use fewer; the step
+ * form compiles to 96 either way).  Both give
  * bit-identical results; lq_autotune picks one for a context set to
  * COLUMNS; the default is 0.  No reference counterpart. */
 int lq_set_columns_build(lq_ctx* ctx, int build);

@@ -39,7 +39,7 @@ struct lq_ctx {
   int nbits = 0;
   int npm0 = 0;
   int tile_shape = 1, tile_tz = 7;
-  int col_wide = 0;  // COLUMNS: 128-VGPR build instead of the 96-VGPR one
+  int col_wide = 0;  // COLUMNS: register budget of 128 VGPRs instead of 96
   int fuse = 0;      // COLUMNS steps: predictor / corrector fused into the interaction
   int xcd_remap = 1;  // renumber interaction blocks per XCD (lq_set_interaction / lq_autotune)
   long long host_step = 0;
@@ -1875,7 +1875,7 @@ extern "C" int lq_set_interaction(lq_ctx* c, int variant, int xcd_remap) {
   return LQ_OK;
 }
 
-// COLUMNS register build: 0 the 96-VGPR kernel, 1 the 128-VGPR one.
+// COLUMNS register budget: 0 at most 96 VGPRs, 1 at most 128.
 extern "C" int lq_set_columns_build(lq_ctx* c, int build) {
   if (!c) return LQ_ERR_ARG;
   if (build < 0 || build > 1) return fail(c, LQ_ERR_ARG, "lq_set_columns_build: build 0 or 1");
@@ -1977,7 +1977,7 @@ extern "C" int lq_autotune(lq_ctx* c, int reps, int* variant_out, int* shape_out
   hipEventCreate(&e0);
   hipEventCreate(&e1);
   double best = 1e30;
-  // for COLUMNS, shape is the register build (1: 128 VGPRs)
+  // for COLUMNS, shape is the register budget (1: up to 128 VGPRs)
   auto apply = [&](const Cand& k) {
     c->par.interaction_variant = k.variant;
     if (k.variant == LQ_INTERACTION_COLUMNS) {

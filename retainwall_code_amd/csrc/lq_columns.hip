@@ -356,8 +356,9 @@ __device__ __forceinline__ void cl_drain(const PairConst& K, float i_kdr, IAcc& 
     cl_eval(K, i_kdr, S, B, lst[(head + k) & (CL_CAP - 1)][lane], pi, two_h, i_h, vi, rhoi, sri, artpi, shift);
 }
 
-// MINW waves per SIMD: 5 holds the kernel to 96 VGPRs (a few spill), 4
-// gives it 128; lq_autotune times both.  FUSE 1 / 2: a particle whose
+// MINW waves per SIMD: 5 holds the kernel to 96 VGPRs, 4 allows 128 (the
+// forms compile to 95-100, the steps' fuse-3 form to 96 either way);
+// lq_autotune times both.  FUSE 1 / 2: a particle whose
 // contact force needs no pass B goes on through cuPredictorMaterial /
 // cuCorrectorMaterial (predictor_slot / corrector_slot) right after its
 // pair sum.  FUSE 3: the kernel reads nothing the corner search writes

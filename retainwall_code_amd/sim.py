@@ -237,7 +237,7 @@ class Context:
         self._chk(self.lib.lq_set_interaction(self.h, variant, xcd_remap), "lq_set_interaction")
 
     def set_columns_build(self, build):
-        """COLUMNS register build (lq_set_columns_build): 0 96 VGPRs, 1 128 VGPRs."""
+        """COLUMNS register budget (lq_set_columns_build): 0 at most 96 VGPRs, 1 at most 128."""
         self._chk(self.lib.lq_set_columns_build(self.h, int(build)), "lq_set_columns_build")
 
     def set_overlap(self, on=True):
