@@ -566,6 +566,7 @@ static int create_impl(const lq_params* cases, int n_cases, int device, lq_ctx**
     st |= dalloc(c, &B.fl_pos, n);
     st |= dalloc(c, &B.fl_cell, n);
     st |= dalloc(c, &B.colinfo, ncol);
+    st |= dalloc(c, &B.coltile, ncol / 1024 + 1);
   }
   if (par.np < (1 << 25)) {  // pair-list entries hold the slot in 25 bits
     st |= dalloc(c, &B.plist, size_t(kPairListCap) * size_t(std::max(1, par.npm)));

@@ -31,56 +31,110 @@ namespace lq {
 __device__ __forceinline__ int sub_bits(float p, float h) { return p >= h ? 1 : 0; }
 
 // Wall and soil counts of every 2h column (cx, cy): cells (cx, cy, 0..cnz-1),
-// one wave per column, its lanes over z.
+// COL_CPW consecutive columns per wave (fewer, longer waves: a large grid of
+// mostly empty columns is otherwise bound by launching waves), its lanes
+// over z.
+constexpr int COL_CPW = 8;
 __global__ void __launch_bounds__(256) k_col_count(const DevParams* __restrict__ P, const int* __restrict__ cbeg,
                                                    const int* __restrict__ cend, int2* __restrict__ colcnt) {
-  const int col = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  if (col >= P->cnx * P->cny) return;
-  const int k0 = col * P->cnz, ncm = P->ncell_material;
-  int nw = 0, ns = 0;
-  for (int z = lane; z < P->cnz; z += 64) {
-    const int bw = cbeg[k0 + z], bs = cbeg[ncm + k0 + z];
-    if (bw >= 0) nw += cend[k0 + z] - bw;
-    if (bs >= 0) ns += cend[ncm + k0 + z] - bs;
-  }
+  const int wave = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int ncol = P->cnx * P->cny, cnz = P->cnz, ncm = P->ncell_material;
+#pragma unroll 2
+  for (int q = 0; q < COL_CPW; q++) {
+    const int col = wave * COL_CPW + q;
+    if (col >= ncol) break;  // uniform over the wave
+    const int k0 = col * cnz;
+    int nw = 0, ns = 0;
+    for (int z = lane; z < cnz; z += 64) {
+      const int bw = cbeg[k0 + z], bs = cbeg[ncm + k0 + z];
+      if (bw >= 0) nw += cend[k0 + z] - bw;
+      if (bs >= 0) ns += cend[ncm + k0 + z] - bs;
+    }
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    nw += __shfl_xor(nw, o);
-    ns += __shfl_xor(ns, o);
+    for (int o = 32; o > 0; o >>= 1) {
+      nw += __shfl_xor(nw, o);
+      ns += __shfl_xor(ns, o);
+    }
+    if (lane == 0) colcnt[col] = make_int2(nw, ns);
   }
-  if (lane == 0) colcnt[col] = make_int2(nw, ns);
 }
 
-// Exclusive scans of the column counts (one workgroup): the column's first
-// wall slot w0 and first soil slot s0 (columns are runs of consecutive keys,
-// so in slot order), with its counts.
+// Exclusive scan of one int2 per thread over a 1024-thread workgroup; tot
+// receives the workgroup's total.
+__device__ __forceinline__ int2 col_block_scan(int2 v, int2& tot, int2* wsum) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int x = v.x, y = v.y;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int a = __shfl_up(x, o), b = __shfl_up(y, o);
+    if (lane >= o) { x += a; y += b; }
+  }
+  if (lane == 63) wsum[wv] = make_int2(x, y);
+  __syncthreads();
+  int px = 0, py = 0, tx = 0, ty = 0;
+  for (int w = 0; w < 16; w++) {
+    const int2 s = wsum[w];
+    if (w < wv) { px += s.x; py += s.y; }
+    tx += s.x; ty += s.y;
+  }
+  __syncthreads();
+  tot = make_int2(tx, ty);
+  return make_int2(px + x - v.x, py + y - v.y);
+}
+
+// Exclusive scans of the column counts: the column's first wall slot w0 and
+// first soil slot s0 (columns are runs of consecutive keys, so in slot
+// order), with its counts.  One workgroup walks a grid of up to
+// COL_SCAN_ONE columns; beyond, per-tile totals (k_col_tile_sums), their
+// scan (k_col_tile_scan) and the tiles with their offsets (k_col_apply).
+constexpr int COL_SCAN_ONE = 8192;
 __global__ void __launch_bounds__(1024) k_col_scan(const DevParams* __restrict__ P, const int2* __restrict__ colcnt,
                                                    int4* __restrict__ colinfo) {
   __shared__ int2 wsum[16];
-  const int ncol = P->cnx * P->cny, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int ncol = P->cnx * P->cny;
   int cw = 0, cs = P->npb;
   for (int c0 = 0; c0 < ncol; c0 += 1024) {
     const int c = c0 + threadIdx.x;
     const int2 v = c < ncol ? colcnt[c] : make_int2(0, 0);
-    int x = v.x, y = v.y;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const int a = __shfl_up(x, o), b = __shfl_up(y, o);
-      if (lane >= o) { x += a; y += b; }
-    }
-    if (lane == 63) wsum[wv] = make_int2(x, y);
-    __syncthreads();
-    int px = 0, py = 0, tx = 0, ty = 0;
-    for (int w = 0; w < 16; w++) {
-      const int2 s = wsum[w];
-      if (w < wv) { px += s.x; py += s.y; }
-      tx += s.x; ty += s.y;
-    }
-    if (c < ncol) colinfo[c] = make_int4(cw + px + x - v.x, cs + py + y - v.y, v.x, v.y);
-    cw += tx;
-    cs += ty;
-    __syncthreads();
+    int2 tot;
+    const int2 e = col_block_scan(v, tot, wsum);
+    if (c < ncol) colinfo[c] = make_int4(cw + e.x, cs + e.y, v.x, v.y);
+    cw += tot.x;
+    cs += tot.y;
   }
+}
+__global__ void __launch_bounds__(1024) k_col_tile_sums(const DevParams* __restrict__ P,
+                                                        const int2* __restrict__ colcnt, int2* __restrict__ tsum) {
+  __shared__ int2 wsum[16];
+  const int c = blockIdx.x * 1024 + threadIdx.x;
+  const int2 v = c < P->cnx * P->cny ? colcnt[c] : make_int2(0, 0);
+  int2 tot;
+  col_block_scan(v, tot, wsum);
+  if (threadIdx.x == 0) tsum[blockIdx.x] = tot;
+}
+__global__ void __launch_bounds__(1024) k_col_tile_scan(int2* __restrict__ tsum, int nt) {
+  __shared__ int2 wsum[16];
+  int cw = 0, cs = 0;
+  for (int t0 = 0; t0 < nt; t0 += 1024) {
+    const int t = t0 + threadIdx.x;
+    const int2 v = t < nt ? tsum[t] : make_int2(0, 0);
+    int2 tot;
+    const int2 e = col_block_scan(v, tot, wsum);
+    if (t < nt) tsum[t] = make_int2(cw + e.x, cs + e.y);
+    cw += tot.x;
+    cs += tot.y;
+  }
+}
+__global__ void __launch_bounds__(1024) k_col_apply(const DevParams* __restrict__ P, const int2* __restrict__ colcnt,
+                                                    const int2* __restrict__ toff, int4* __restrict__ colinfo) {
+  __shared__ int2 wsum[16];
+  const int ncol = P->cnx * P->cny;
+  const int c = blockIdx.x * 1024 + threadIdx.x;
+  const int2 v = c < ncol ? colcnt[c] : make_int2(0, 0);
+  int2 tot;
+  const int2 e = col_block_scan(v, tot, wsum);
+  const int2 o = toff[blockIdx.x];
+  if (c < ncol) colinfo[c] = make_int4(o.x + e.x, P->npb + o.y + e.y, v.x, v.y);
 }
 
 // One workgroup per 2h column: counts per bucket b = q * 2cnz + z'
@@ -510,9 +564,17 @@ static inline int cblocks(long long n, int b) { return int((n + b - 1) / b); }
 
 void launch_col_index(hipStream_t s, const DevParams* P, int cnx, int cny, Arrays A, const Buffers& B, int wall_rec) {
   const int ncol = cnx * cny;
-  hipLaunchKernelGGL(k_col_count, dim3(cblocks(ncol, 4)), dim3(256), 0, s, P, (const int*)B.cell_beg,
+  hipLaunchKernelGGL(k_col_count, dim3(cblocks(ncol, 4 * COL_CPW)), dim3(256), 0, s, P, (const int*)B.cell_beg,
                      (const int*)B.cell_end, B.colcnt);
-  hipLaunchKernelGGL(k_col_scan, dim3(1), dim3(1024), 0, s, P, (const int2*)B.colcnt, B.colinfo);
+  if (ncol <= COL_SCAN_ONE) {
+    hipLaunchKernelGGL(k_col_scan, dim3(1), dim3(1024), 0, s, P, (const int2*)B.colcnt, B.colinfo);
+  } else {
+    const int nt = cblocks(ncol, 1024);
+    hipLaunchKernelGGL(k_col_tile_sums, dim3(nt), dim3(1024), 0, s, P, (const int2*)B.colcnt, B.coltile);
+    hipLaunchKernelGGL(k_col_tile_scan, dim3(1), dim3(1024), 0, s, B.coltile, nt);
+    hipLaunchKernelGGL(k_col_apply, dim3(nt), dim3(1024), 0, s, P, (const int2*)B.colcnt, (const int2*)B.coltile,
+                       B.colinfo);
+  }
   hipLaunchKernelGGL(k_col_build, dim3(ncol), dim3(COL_THREADS), 0, s, P, A, B, (const int4*)B.colinfo, wall_rec);
 }
 void launch_col_wall_records(hipStream_t s, const DevParams* P, int npb, Arrays A, const Buffers& B) {

@@ -154,6 +154,7 @@ struct Buffers {
   int4* fl_cell;
   int2* colcnt;   // per 2h column: wall and soil counts
   int4* colinfo;  // per 2h column: first wall slot, first soil slot, counts
+  int2* coltile;  // per 1024 columns: wall and soil totals, then their exclusive scan
   int* hist;         // radix histogram [256][nblocks]
   unsigned long long* wf_acc;  // per case: exact wall-force sum, 128-bit fixed point (lo, hi)
 };
