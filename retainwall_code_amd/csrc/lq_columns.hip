@@ -379,7 +379,9 @@ __device__ __forceinline__ void pair_terms_fast(const PairConst& P, float i_kdr,
 // they read neighbouring stretches of the same target sub-column, and their pair records (stored in
 // aux order, one plane per float4, so that 8 lanes share a 128-byte line)
 // lie close together.  Hits
-// go to per-lane FIFO rings in LDS as (aux index << 6 | cell-offset code) and
+// go to per-lane FIFO rings in LDS as (aux index << 2 | z cell-offset code:
+// the window fixes the x and y offsets of every entry, as the rings hold
+// only the current window's hits) and
 // are evaluated in ring order, as in the neighbour-list variant; the rings
 // are emptied at the end of every window (and earlier when one would
 // overflow), so a window's records are summed while they are still in the
@@ -388,26 +390,29 @@ __device__ __forceinline__ void pair_terms_fast(const PairConst& P, float i_kdr,
 constexpr int CL_CAP = 16;    // ring entries per lane (power of two): 4 KB of LDS per wave
 constexpr int CL_CHUNK = 16;  // candidates per hit mask
 
+// offx, offy: the window's cell offsets along x and y (times 2h)
 __device__ __forceinline__ void cl_eval(const PairConst& K, float i_kdr, IAcc& S, const Buffers& B, int e, float3 pi,
-                                        float two_h, float i_h, float3 vi, float rhoi, Sym sri, float artpi,
-                                        int shift) {
-  const size_t rows = size_t(B.rec_rows), k = unsigned(e) >> 6;
+                                        float offx, float offy, float two_h, float i_h, float3 vi, float rhoi,
+                                        Sym sri, float artpi, int shift) {
+  const size_t rows = size_t(B.rec_rows), k = unsigned(e) >> 2;
   const float4 np_ = B.rec[k], nv = B.rec[rows + k], na = B.rec[2 * rows + k], nb = B.rec[3 * rows + k];
   float3 d;
-  d.x = (pi.x - np_.x) - fmaf(float(e & 3), two_h, -two_h);
-  d.y = (pi.y - np_.y) - fmaf(float((e >> 2) & 3), two_h, -two_h);
-  d.z = (pi.z - np_.z) - fmaf(float((e >> 4) & 3), two_h, -two_h);
+  d.x = (pi.x - np_.x) - offx;
+  d.y = (pi.y - np_.y) - offy;
+  d.z = (pi.z - np_.z) - fmaf(float(e & 3), two_h, -two_h);
   const float r2 = fmaf(d.z, d.z, fmaf(d.y, d.y, d.x * d.x));
   const float r = __builtin_amdgcn_sqrtf(r2);
   pair_terms_fast(K, i_kdr, S, d, r, r * i_h, vi, rhoi, sri, artpi, shift, nv, np_.w, na, nb);
 }
 
 __device__ __forceinline__ void cl_drain(const PairConst& K, float i_kdr, IAcc& S, const Buffers& B,
-                                         const int (*lst)[64], int lane, int head, int pop, float3 pi, float two_h,
-                                         float i_h, float3 vi, float rhoi, Sym sri, float artpi, int shift) {
+                                         const int (*lst)[64], int lane, int head, int pop, float3 pi, float offx,
+                                         float offy, float two_h, float i_h, float3 vi, float rhoi, Sym sri,
+                                         float artpi, int shift) {
 #pragma unroll 1
   for (int k = 0; k < pop; k++)
-    cl_eval(K, i_kdr, S, B, lst[(head + k) & (CL_CAP - 1)][lane], pi, two_h, i_h, vi, rhoi, sri, artpi, shift);
+    cl_eval(K, i_kdr, S, B, lst[(head + k) & (CL_CAP - 1)][lane], pi, offx, offy, two_h, i_h, vi, rhoi, sri, artpi,
+            shift);
 }
 
 // MINW waves per SIMD: 5 holds the kernel to 96 VGPRs, 4 allows 128 (the
@@ -463,7 +468,6 @@ __global__ void __launch_bounds__(64, MINW) k_interaction_col(DevParams* __restr
     int j = nj, e = ne;
     const int ox = nox, oy = noy, zb0 = nzb0, zb1 = nzb1;
     window(a + 1, nj, ne, nox, noy, nzb0, nzb1);
-    const unsigned code_xy = unsigned((ox + 1) | ((oy + 1) << 2));
     const float offx = float(ox) * two_h, offy = float(oy) * two_h;
     while (j < e) {
       const int m = min(e - j, CL_CHUNK);
@@ -487,7 +491,7 @@ __global__ void __launch_bounds__(64, MINW) k_interaction_col(DevParams* __restr
         for (int b = CL_CHUNK; b; b >>= 1)
           if (__ballot(over >= need + b)) need += b;
         const int pop = min(cnt, need);
-        cl_drain(K, i_kdr, S, B, s_list, lane, head, pop, pi, two_h, i_h, vi, rhoi, sri, artpi, shift);
+        cl_drain(K, i_kdr, S, B, s_list, lane, head, pop, pi, offx, offy, two_h, i_h, vi, rhoi, sri, artpi, shift);
         head = (head + pop) & (CL_CAP - 1);
         cnt -= pop;
       }
@@ -495,13 +499,13 @@ __global__ void __launch_bounds__(64, MINW) k_interaction_col(DevParams* __restr
         const int t = __ffs(mask) - 1;
         mask &= mask - 1u;
         const unsigned oz = unsigned(j + t >= zb0) + unsigned(j + t >= zb1);  // oz + 1
-        s_list[(head + cnt) & (CL_CAP - 1)][lane] = int((unsigned(j + t) << 6) | code_xy | (oz << 4));
+        s_list[(head + cnt) & (CL_CAP - 1)][lane] = int((unsigned(j + t) << 2) | oz);
         cnt++;
       }
       j += m;
     }
     // drain at the window's end, while its records are still near in the caches
-    cl_drain(K, i_kdr, S, B, s_list, lane, head, cnt, pi, two_h, i_h, vi, rhoi, sri, artpi, shift);
+    cl_drain(K, i_kdr, S, B, s_list, lane, head, cnt, pi, offx, offy, two_h, i_h, vi, rhoi, sri, artpi, shift);
     head = (head + cnt) & (CL_CAP - 1);
     cnt = 0;
   }
