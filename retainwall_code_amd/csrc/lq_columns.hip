@@ -31,17 +31,15 @@ namespace lq {
 __device__ __forceinline__ int sub_bits(float p, float h) { return p >= h ? 1 : 0; }
 
 // Wall and soil counts of every 2h column (cx, cy): cells (cx, cy, 0..cnz-1),
-// COL_CPW consecutive columns per wave (fewer, longer waves: a large grid of
-// mostly empty columns is otherwise bound by launching waves), its lanes
-// over z.
-constexpr int COL_CPW = 8;
+// cpw consecutive columns per wave, its lanes over z (on a large grid of
+// mostly empty columns, several columns per wave: fewer, longer waves).
 __global__ void __launch_bounds__(256) k_col_count(const DevParams* __restrict__ P, const int* __restrict__ cbeg,
-                                                   const int* __restrict__ cend, int2* __restrict__ colcnt) {
+                                                   const int* __restrict__ cend, int2* __restrict__ colcnt, int cpw) {
   const int wave = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int ncol = P->cnx * P->cny, cnz = P->cnz, ncm = P->ncell_material;
 #pragma unroll 2
-  for (int q = 0; q < COL_CPW; q++) {
-    const int col = wave * COL_CPW + q;
+  for (int q = 0; q < cpw; q++) {
+    const int col = wave * cpw + q;
     if (col >= ncol) break;  // uniform over the wave
     const int k0 = col * cnz;
     int nw = 0, ns = 0;
@@ -568,8 +566,9 @@ static inline int cblocks(long long n, int b) { return int((n + b - 1) / b); }
 
 void launch_col_index(hipStream_t s, const DevParams* P, int cnx, int cny, Arrays A, const Buffers& B, int wall_rec) {
   const int ncol = cnx * cny;
-  hipLaunchKernelGGL(k_col_count, dim3(cblocks(ncol, 4 * COL_CPW)), dim3(256), 0, s, P, (const int*)B.cell_beg,
-                     (const int*)B.cell_end, B.colcnt);
+  const int cpw = ncol <= COL_SCAN_ONE ? 1 : 8;
+  hipLaunchKernelGGL(k_col_count, dim3(cblocks(ncol, 4 * cpw)), dim3(256), 0, s, P, (const int*)B.cell_beg,
+                     (const int*)B.cell_end, B.colcnt, cpw);
   if (ncol <= COL_SCAN_ONE) {
     hipLaunchKernelGGL(k_col_scan, dim3(1), dim3(1024), 0, s, P, (const int2*)B.colcnt, B.colinfo);
   } else {
