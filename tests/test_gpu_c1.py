@@ -33,7 +33,7 @@ import make_c1_wall_force as G  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
-WALL_SPEED = 1e-3  # m/s: the velocity scale the step's error is judged against
+WALL_SPEED = 1e-3  # m/s: the velocity scale the step's error is measured against
 
 
 def hist_err(a, b):
